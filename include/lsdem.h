@@ -1,0 +1,163 @@
+/*
+ * lsdem.h -- C ABI of the MI355X level-set DEM timestep library (liblsdem.so).
+ *
+ * One context = one simulation on one GPU and one HIP stream.  The context owns all device
+ * memory; every host array passed in is copied.  Calls are not re-entrant per context and
+ * the library starts no threads.  Every function returns LSDEM_OK (0) or an LSDEM_ERR_*
+ * code; lsdem_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Units and layouts: SI units; double3 values are 3 consecutive doubles (x, y, z);
+ * quaternions are 4 doubles (q0 = scalar, q1, q2, q3); symmetric 3x3 tensors are 6 doubles
+ * (xx, yy, zz, xy, xz, yz); SDF grids are float with x fastest (index ix + nx*(iy + ny*iz)).
+ *
+ * Each entry point names the interface of the reference (FunDEM, kaiqideng/LSDEM2.0) that it
+ * replaces; paths are relative to the reference checkout.
+ */
+#ifndef LSDEM_H
+#define LSDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSDEM_OK 0
+#define LSDEM_ERR_ARG 1      /* invalid argument */
+#define LSDEM_ERR_STATE 2    /* call not allowed in the current context state */
+#define LSDEM_ERR_HIP 3      /* HIP runtime error */
+#define LSDEM_ERR_CAPACITY 4 /* pair or contact buffer overflow; raise the capacities */
+#define LSDEM_ERR_NOMEM 5
+
+/* Reference behaviours kept by default (all on).  Each bit can be switched off. */
+#define LSDEM_QUIRK_INT_X_GRADIENT 1  /* LSParticleContactDetectionKernel.cuh:34 `const int x` */
+#define LSDEM_QUIRK_GHOST_Z_VS_MAXY 2 /* mySpatialGrid.cu:236 z tested against maxBound.y */
+#define LSDEM_QUIRK_WRAP_AFTER_BUILD 4 /* LSSolver.h:1071-1074 wrap after the contact build */
+#define LSDEM_QUIRK_ALL 7
+
+/* Contact lists (SolidInteraction instances of the reference). */
+#define LSDEM_LIST_PP 0  /* particle-particle, LSSolver.h:1136 LSParticleInteraction_ */
+#define LSDEM_LIST_PW 1  /* particle-wall, LSSolver.h:1137 LSParticleWallInteraction_ */
+#define LSDEM_LIST_GX 2  /* periodic ghost +x, PeriodicBoundary.h:306 */
+#define LSDEM_LIST_GY 3  /* periodic ghost +y, PeriodicBoundary.h:307 */
+#define LSDEM_LIST_GXY 4 /* periodic ghost +x+y, PeriodicBoundary.h:308 */
+
+typedef struct lsdem_ctx lsdem_ctx;
+
+typedef struct lsdem_config {
+  double domain_min[3];   /* solve(minDomain, ...), LSSolver.h:478 */
+  double domain_max[3];
+  double gravity[3];
+  double dt;
+  int32_t periodic_x;     /* addPeriodicBoundaryXD, LSSolver.h:426 */
+  int32_t periodic_y;     /* addPeriodicBoundaryYD, LSSolver.h:437 */
+  int32_t quirks;         /* LSDEM_QUIRK_* bitmask */
+  int32_t block_size;     /* threads per block of per-body kernels (maxGPUThread), 0 = 256 */
+  int64_t pair_capacity;  /* candidate pairs per step, 0 = automatic */
+  int64_t contact_capacity; /* contacts per step, 0 = automatic */
+  int32_t device;         /* HIP device ordinal */
+  int32_t reserved[7];
+} lsdem_config;
+
+typedef struct lsdem_counters {
+  int64_t steps;          /* steps taken */
+  int64_t pairs;          /* candidate (master particle, partner) pairs, last step */
+  int64_t node_tests;     /* nodes tested against a partner after cluster culling, last step */
+  int64_t sdf_queries;    /* 8-corner SDF interpolations, last step */
+  int64_t contacts;       /* contacts written (all lists), last step */
+  int64_t pair_capacity;
+  int64_t contact_capacity;
+  int64_t reserved[9];
+} lsdem_counters;
+
+/* Fill *cfg with defaults (quirks all on, gravity 0, dt 1e-5, automatic capacities). */
+void lsdem_config_default(lsdem_config* cfg);
+
+/* Create a context on cfg->device.  Replaces LSSolver::LSSolver + activateGPUDevice
+ * (LSSolver.h:14, :521-533); a missing device is an error return, not exit(1). */
+int lsdem_create(const lsdem_config* cfg, lsdem_ctx** out);
+void lsdem_destroy(lsdem_ctx* ctx);
+const char* lsdem_last_error(void);
+
+/* Register a level-set shape: SDF grid plus boundary nodes in the shape's local frame.
+ * Bodies reference shapes by id with a per-body scale s (geometry s*x, SDF values s*phi), so
+ * identical grids are stored once.  Replaces the per-particle SDF copies of
+ * LSParticle::add (LSParticle.h:115-212). */
+int lsdem_add_shape(lsdem_ctx* ctx, const float* sdf, int32_t nx, int32_t ny, int32_t nz,
+                    const double origin[3], double spacing, const double* nodes, int32_t n_nodes,
+                    int32_t* shape_id);
+
+/* Append n particles.  Same clamps as LSParticle::add (LSParticle.h:188-202): negative kn,
+ * ks, mu become 0, restitution outside (0,1] becomes 1, q is normalised, bounding radius
+ * = max |s * node|.  inv_inertia is the local-frame inverse inertia tensor. */
+int lsdem_add_particles(lsdem_ctx* ctx, int32_t n, const int32_t* shape, const double* scale,
+                        const double* pos, const double* quat, const double* vel,
+                        const double* angvel, const double* inv_mass, const double* inv_inertia,
+                        const double* kn, const double* ks, const double* mu,
+                        const double* restitution);
+
+/* Append n walls (immovable level-set bodies with prescribed v, w).  bound_radius is the
+ * largest |vertex| of the wall mesh, which sets the wall cell size as in LSSolver::addWall
+ * (LSSolver.h:245-272) + LSParticle::initialize (LSParticle.h:253). */
+int lsdem_add_walls(lsdem_ctx* ctx, int32_t n, const int32_t* shape, const double* bound_radius,
+                    const double* pos, const double* quat, const double* vel, const double* angvel,
+                    const double* mu, const double* restitution);
+
+/* Add a parallel ("V") bond between particles i and j.  Replaces VBondedInteraction::add
+ * (VBondedInteraction.h:74-195) via LSSolver::addBondedInteraction (LSSolver.h:344-372).
+ * Uses the particles' current pose.  Returns LSDEM_ERR_ARG for the inputs the reference
+ * rejects (zero normal, r <= 0, L <= 0, E <= 0, nu outside (-1, 0.5), ids out of range). */
+int lsdem_add_bond(lsdem_ctx* ctx, int32_t i, int32_t j, const double point[3],
+                   const double normal[3], double radius, double length, double youngs,
+                   double poisson, double tensile_strength, double cohesion, double friction);
+
+/* Upload and size device buffers.  Replaces LSSolver::upload (LSSolver.h:612-623). */
+int lsdem_initialize(lsdem_ctx* ctx);
+
+/* Advance n_steps timesteps on the context's stream; returns after the work is complete.
+ * One step is the loop body of LSSolver::compute* (LSSolver.h:1000-1020 / :1104-1128):
+ * half-kick, drift, broad phase, narrow phase with tangential history, contact and bond
+ * forces, per-body reduction, half-kick.  Replaces the launchParticle*Integration,
+ * launchUpdateSpatialGridHashStartEnd, launchBuildLevelSetBoundaryNode*Interactions1st/2nd,
+ * buildPrefixSum, launchAdd*ForceTorque and launchCalculateGhost* launchers
+ * (particleIntegrationKernel.cu:42,72; mySpatialGrid.h:33; LSParticleContactDetectionKernel.cuh:79-215;
+ * buildHashStartEnd.cu:69,139; contactKernel.cuh:201-333). */
+int lsdem_step(lsdem_ctx* ctx, int32_t n_steps);
+
+/* Copy particle state to / from the host; NULL pointers are skipped.  force/torque are the
+ * totals of the last step.  Replaces LSParticle::finalize / copyHostToDevice
+ * (LSParticle.h:699-743). */
+int lsdem_get_particles(lsdem_ctx* ctx, double* pos, double* quat, double* vel, double* angvel,
+                        double* force, double* torque);
+int lsdem_set_particles(lsdem_ctx* ctx, const double* pos, const double* quat, const double* vel,
+                        const double* angvel, const double* force, const double* torque);
+int lsdem_get_walls(lsdem_ctx* ctx, double* pos, double* quat, double* vel, double* angvel);
+int lsdem_set_walls(lsdem_ctx* ctx, const double* pos, const double* quat, const double* vel,
+                    const double* angvel);
+
+/* Contacts of one list from the last step, sorted by (master node, slave).  Writes at most
+ * capacity records; *count receives the total.  NULL arrays are skipped.  Replaces the
+ * SolidInteraction device arrays and their host copies (SolidInteraction.h:319-389). */
+int lsdem_get_contacts(lsdem_ctx* ctx, int32_t list, int64_t capacity, int64_t* count,
+                       int32_t* master, int32_t* slave, double* point, double* normal,
+                       double* overlap, double* spring, double* energy_n, double* energy_s);
+
+/* Bond state (VBondedInteraction.h device arrays): activated flag, centre point, Un, Us, Ub,
+ * Ut, max normal stress, max shear stress.  NULL arrays are skipped. */
+int lsdem_get_bonds(lsdem_ctx* ctx, int32_t* active, double* point, double* energies4,
+                    double* sigma_max, double* tau_max);
+
+int lsdem_get_counters(lsdem_ctx* ctx, lsdem_counters* out);
+
+/* Mass, centroid (local frame), inertia and inverse inertia from an SDF grid with a smoothed
+ * Heaviside of half-width 1.5 cells; host computation used by the first addLSParticle
+ * overload (LSSolver.h:51-163).  inertia6/inv_inertia6 may be NULL. */
+int lsdem_mass_properties(const double* sdf, int32_t nx, int32_t ny, int32_t nz,
+                          const double origin[3], double spacing, double density, double* mass,
+                          double centroid[3], double* inertia6, double* inv_inertia6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSDEM_H */

@@ -1,0 +1,30 @@
+"""Build recipe of liblsdem.so (hipcc, gfx950)."""
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "api.hip"]
+OUT = HERE / "liblsdem.so"
+HIPCC = "/opt/rocm/bin/hipcc"
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # bit-exact FP64 arithmetic: no contraction of a*b+c into fma
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose: bool = False) -> Path:
+    cmd = [HIPCC, *FLAGS, *map(str, SOURCES), "-o", str(OUT)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        print(" ".join(cmd))
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

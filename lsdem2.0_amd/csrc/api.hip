@@ -1,0 +1,1016 @@
+// Host side of liblsdem.so: context, uploads, per-step launch sequence and the C ABI of
+// include/lsdem.h.  No CPU fallback exists: every step runs on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/lsdem.h"
+#include "device.h"
+
+namespace lsd {
+// kernels.hip
+__global__ void k_fill_int(int* a, long long n, int v);
+__global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v);
+__global__ void k_scan_reduce(const int* in, long long n, int* blockSum);
+__global__ void k_scan_top(int* blockSum, int nb, int* totalOut);
+__global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out);
+__global__ void k_kick_drift(Params P);
+__global__ void k_walls(Params P);
+__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank);
+__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out);
+__global__ void k_bucket_sort(const int* start, int nb, int* items);
+template <bool WRITE> __global__ void k_pairs(Params P);
+__global__ void k_rc_count(Params P);
+__global__ void k_rc_scatter(Params P);
+__global__ void k_hash_insert(Params P);
+__global__ void k_narrow(Params P);
+__global__ void k_bonds(Params P);
+__global__ void k_reduce_kick(Params P);
+__global__ void k_step_begin(Params P);
+__global__ void k_step_end(Params P);
+}  // namespace lsd
+
+using namespace lsd;
+
+namespace {
+
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(LSDEM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct HostShape {
+  int nx, ny, nz;
+  d3 origin;
+  double h;
+  std::vector<float> sdf;
+  std::vector<d3> nodes;
+  std::vector<NodeDev> perm;        // nodes in cluster order
+  std::vector<ClusterDev> clusters;
+  double boxR = 0, lip = 0;
+};
+
+// Split nodes into spatially compact clusters of <= 64 (longest-axis median split, sizes kept
+// at multiples of 64 where possible) and record each cluster's bounding sphere.
+void buildClusters(HostShape& S) {
+  std::vector<int> idx(S.nodes.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  struct Seg { int first, count; };
+  std::vector<Seg> work{{0, int(idx.size())}}, leaves;
+  while (!work.empty()) {
+    Seg s = work.back();
+    work.pop_back();
+    if (s.count <= 64) { if (s.count > 0) leaves.push_back(s); continue; }
+    d3 lo = S.nodes[idx[s.first]], hi = lo;
+    for (int k = s.first; k < s.first + s.count; ++k) {
+      const d3 p = S.nodes[idx[k]];
+      lo = mk3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+      hi = mk3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+    }
+    const d3 ext = hi - lo;
+    const int axis = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
+    int left = ((s.count + 1) / 2 + 63) / 64 * 64;
+    left = std::min(left, s.count - 1);
+    auto coord = [&](int i) { const d3 p = S.nodes[i]; return axis == 0 ? p.x : axis == 1 ? p.y : p.z; };
+    std::nth_element(idx.begin() + s.first, idx.begin() + s.first + left, idx.begin() + s.first + s.count,
+                     [&](int a, int b) { return coord(a) < coord(b) || (coord(a) == coord(b) && a < b); });
+    work.push_back({s.first + left, s.count - left});
+    work.push_back({s.first, left});
+  }
+  std::sort(leaves.begin(), leaves.end(), [](const Seg& a, const Seg& b) { return a.first < b.first; });
+  S.perm.clear();
+  S.clusters.clear();
+  for (const Seg& s : leaves) {
+    d3 lo = S.nodes[idx[s.first]], hi = lo;
+    for (int k = s.first; k < s.first + s.count; ++k) {
+      const d3 p = S.nodes[idx[k]];
+      lo = mk3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+      hi = mk3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+    }
+    const d3 c = 0.5 * (lo + hi);
+    double rho = 0;
+    ClusterDev cl{};
+    cl.first = int(S.perm.size());
+    cl.count = s.count;
+    for (int k = s.first; k < s.first + s.count; ++k) {
+      const d3 p = S.nodes[idx[k]];
+      rho = std::max(rho, len(p - c));
+      NodeDev nd{};
+      nd.x = p.x; nd.y = p.y; nd.z = p.z; nd.orig = idx[k];
+      S.perm.push_back(nd);
+    }
+    cl.cx = c.x; cl.cy = c.y; cl.cz = c.z;
+    cl.rho = rho * (1.0 + 1e-12);
+    S.clusters.push_back(cl);
+  }
+}
+
+// Grid-box radius and a Lipschitz bound of the trilinear interpolant: inside one cell each
+// partial derivative is a bilinear blend of the 4 edge differences, so it is bounded by the
+// largest of them.
+void shapeBounds(HostShape& S) {
+  double br = 0;
+  for (int c = 0; c < 8; ++c) {
+    const d3 p = mk3(S.origin.x + ((c & 1) ? (S.nx - 1) * S.h : 0.), S.origin.y + ((c & 2) ? (S.ny - 1) * S.h : 0.),
+                     S.origin.z + ((c & 4) ? (S.nz - 1) * S.h : 0.));
+    br = std::max(br, len(p));
+  }
+  S.boxR = br * (1.0 + 1e-12);
+  auto at = [&](int i, int j, int k) { return double(S.sdf[(size_t(k) * S.ny + j) * S.nx + i]); };
+  double L = 0;
+  for (int k = 0; k + 1 < S.nz; ++k)
+    for (int j = 0; j + 1 < S.ny; ++j)
+      for (int i = 0; i + 1 < S.nx; ++i) {
+        double gx = 0, gy = 0, gz = 0;
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b) {
+            gx = std::max(gx, std::fabs(at(i + 1, j + a, k + b) - at(i, j + a, k + b)));
+            gy = std::max(gy, std::fabs(at(i + a, j + 1, k + b) - at(i + a, j, k + b)));
+            gz = std::max(gz, std::fabs(at(i + a, j + b, k + 1) - at(i + a, j + b, k)));
+          }
+        L = std::max(L, std::sqrt(gx * gx + gy * gy + gz * gz) / S.h);
+      }
+  S.lip = L * (1.0 + 1e-9);
+}
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, count * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct lsdem_ctx {
+  lsdem_config cfg;
+  hipStream_t stream = nullptr;
+  bool initialized = false;
+  int numCU = 256;
+  // host staging
+  std::vector<HostShape> shapes;
+  std::vector<int> pShape;
+  std::vector<double> pScale, pInvM, pKn, pKs, pMu, pRes, pRadius;
+  std::vector<d3> pPos, pVel, pAng;
+  std::vector<q4> pQuat;
+  std::vector<s6> pInvI;
+  std::vector<int> wShape;
+  std::vector<double> wRadius, wMu, wRes;
+  std::vector<d3> wPos, wVel, wAng;
+  std::vector<q4> wQuat;
+  // bonds (host)
+  std::vector<int> bI, bJ, bActive;
+  std::vector<double> bConst;
+  std::vector<d3> bLocal, bPoint;
+
+  // device
+  DBuf<ShapeDev> dShapes;
+  DBuf<ClusterDev> dClusters;
+  DBuf<NodeDev> dNodes;
+  DBuf<float> dSdf;
+  DBuf<d3> dPos, dPosDet, dVel, dAng, dForce, dTorque, dGorig;
+  DBuf<q4> dQuat;
+  DBuf<s6> dInvI;
+  DBuf<double> dInvM, dRadius, dKn, dKs, dMu, dRes, dScale, dInvSp;
+  DBuf<int> dShape, dNodeStart, dKeyRef, dScell;
+  DBuf<d3> dWPos, dWVel, dWAng;
+  DBuf<q4> dWQuat;
+  DBuf<double> dWMu, dWRes;
+  DBuf<int> dWShape, dWKey;
+  DBuf<d3> dGPos[3];
+  DBuf<int> dGKey[3], dGCell[3], dGCellStart[3], dGSorted[3], dGCount[3], dGRank[3];
+  DBuf<int> dCellCount, dCellStart, dSorted, dRank;
+  DBuf<int> dPairCount, dPairStart, dPairA, dPairFlag, dRcCount, dRcStart, dRcPair, dRcRank;
+  DBuf<uint32_t> dPairInfo;
+  DBuf<double> dPairSum;
+  DBuf<int> dCNode[2];
+  DBuf<uint32_t> dCInfo[2];
+  DBuf<d3> dCSpring[2], dCPoint[2], dCNormal[2];
+  DBuf<double> dCOverlap[2], dCEn[2], dCEs[2];
+  DBuf<unsigned long long> dHashKey;
+  DBuf<int> dHashVal;
+  DBuf<int> dBI, dBJ, dBActive, dBcStart, dBcItem;
+  DBuf<double> dBConst, dBOut;
+  DBuf<d3> dBLocal, dBPoint, dBF, dBMi, dBMj;
+  DBuf<int> dDevInt, dScanBlocks;
+  DBuf<unsigned long long> dDevStat;
+  long long hashSize = 0;
+  int scanBlocksCap = 0;
+  Params P{};
+  long long stepCount = 0;
+  int cur = 0;  // contact buffer written by the next step
+
+  ~lsdem_ctx() { releaseAll(); if (stream) (void)hipStreamDestroy(stream); }
+  void releaseAll();
+};
+
+void lsdem_ctx::releaseAll() {
+  dShapes.release(); dClusters.release(); dNodes.release(); dSdf.release();
+  dPos.release(); dPosDet.release(); dVel.release(); dAng.release(); dForce.release(); dTorque.release();
+  dGorig.release(); dQuat.release(); dInvI.release();
+  dInvM.release(); dRadius.release(); dKn.release(); dKs.release(); dMu.release(); dRes.release();
+  dScale.release(); dInvSp.release(); dShape.release(); dNodeStart.release(); dKeyRef.release(); dScell.release();
+  dWPos.release(); dWVel.release(); dWAng.release(); dWQuat.release(); dWMu.release(); dWRes.release();
+  dWShape.release(); dWKey.release();
+  for (int s = 0; s < 3; ++s) {
+    dGPos[s].release(); dGKey[s].release(); dGCell[s].release(); dGCellStart[s].release();
+    dGSorted[s].release(); dGCount[s].release(); dGRank[s].release();
+  }
+  dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
+  dPairCount.release(); dPairStart.release(); dPairA.release(); dPairFlag.release();
+  dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release();
+  dPairInfo.release(); dPairSum.release();
+  for (int b = 0; b < 2; ++b) {
+    dCNode[b].release(); dCInfo[b].release(); dCSpring[b].release(); dCPoint[b].release();
+    dCNormal[b].release(); dCOverlap[b].release(); dCEn[b].release(); dCEs[b].release();
+  }
+  dHashKey.release(); dHashVal.release();
+  dBI.release(); dBJ.release(); dBActive.release(); dBcStart.release(); dBcItem.release();
+  dBConst.release(); dBOut.release(); dBLocal.release(); dBPoint.release(); dBF.release(); dBMi.release();
+  dBMj.release();
+  dDevInt.release(); dScanBlocks.release(); dDevStat.release();
+}
+
+namespace {
+
+GridGeom makeGrid(d3 mn, d3 mx, double cell) {  // spatialGrid::set
+  GridGeom G{};
+  if (mx.x <= mn.x) mx.x = mn.x + cell;
+  if (mx.y <= mn.y) mx.y = mn.y + cell;
+  if (mx.z <= mn.z) mx.z = mn.z + cell;
+  G.mn = mn;
+  G.mx = mx;
+  G.size = i3{2, 2, 2};
+  G.inv = mk3(0, 0, 0);
+  if (isZero(cell)) return G;
+  const d3 d = mx - mn;
+  G.size.x = d.x > 2. * cell ? int(d.x / cell) : 2;
+  G.size.y = d.y > 2. * cell ? int(d.y / cell) : 2;
+  G.size.z = d.z > 2. * cell ? int(d.z / cell) : 2;
+  G.inv.x = (d.x > 0.0) ? (double(G.size.x) / d.x) : 0.0;
+  G.inv.y = (d.y > 0.0) ? (double(G.size.y) / d.y) : 0.0;
+  G.inv.z = (d.z > 0.0) ? (double(G.size.z) / d.z) : 0.0;
+  return G;
+}
+
+template <class T>
+int upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+  HIPCHK(b.alloc(std::max<size_t>(v.size(), 1)));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return LSDEM_OK;
+}
+
+inline unsigned blocksFor(long long n, int bs) { return unsigned(std::max<long long>(1, (n + bs - 1) / bs)); }
+
+int scan(lsdem_ctx* c, const int* in, long long n, int* out) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, out, 1ll, 0);
+    return LSDEM_OK;
+  }
+  const int nb = int((n + 2047) / 2048);
+  if (nb > c->scanBlocksCap) return fail(LSDEM_ERR_STATE, "scan buffer too small");
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->dScanBlocks.p);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, c->stream, c->dScanBlocks.p, nb, out + n);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, in, n, (const int*)c->dScanBlocks.p, out);
+  return LSDEM_OK;
+}
+
+// Deterministic counting sort of `n` items by bucket id (ids < 0 skipped).
+int bucketSort(lsdem_ctx* c, const int* bucket, int n, int nb, int* count, int* start, int* rank, int* out) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, count, (long long)nb, 0);
+  if (n > 0) hipLaunchKernelGGL(k_bucket_count, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n, count, rank);
+  int rc = scan(c, count, nb, start);
+  if (rc) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n,
+                       (const int*)start, (const int*)rank, out);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, (const int*)start, nb, out);
+  }
+  return LSDEM_OK;
+}
+
+void setCurrentBuffers(lsdem_ctx* c) {
+  Params& P = c->P;
+  const int b = c->cur, o = 1 - c->cur;
+  P.cNode = c->dCNode[b].p; P.cInfo = c->dCInfo[b].p; P.cSpring = c->dCSpring[b].p;
+  P.cPoint = c->dCPoint[b].p; P.cNormal = c->dCNormal[b].p; P.cOverlap = c->dCOverlap[b].p;
+  P.cEn = c->dCEn[b].p; P.cEs = c->dCEs[b].p;
+  P.prevSpring = c->dCSpring[o].p;
+}
+
+int runStep(lsdem_ctx* c) {
+  Params& P = c->P;
+  setCurrentBuffers(c);
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, P);
+  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  if (P.nW > 0) hipLaunchKernelGGL(k_walls, dim3(blocksFor(P.nW, bs)), dim3(bs), 0, s, P);
+  int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted);
+  if (rc) return rc;
+  for (int g = 0; g < 3; ++g) {
+    const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
+    if (!on) continue;
+    rc = bucketSort(c, P.gcell[g], P.nP, P.sg.ncell, P.gCount[g], P.gCellStart[g], c->dGRank[g].p, P.gSorted[g]);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_pairs<false>, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  rc = scan(c, P.pairCount, P.nP, P.pairStart);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pairs<true>, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  // reaction CSR
+  hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P.rcCount, (long long)P.nP, 0);
+  const unsigned gsPairs = unsigned(c->numCU * 8);
+  hipLaunchKernelGGL(k_rc_count, dim3(gsPairs), dim3(256), 0, s, P);
+  rc = scan(c, P.rcCount, P.nP, P.rcStart);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rc_scatter, dim3(gsPairs), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, (const int*)P.rcStart, P.nP,
+                     P.rcPair);
+  // narrow phase + contact law
+  hipLaunchKernelGGL(k_narrow, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  // history of this step's contacts for the next step
+  hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P.hashKey, c->hashSize,
+                     kHashEmpty);
+  hipLaunchKernelGGL(k_hash_insert, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
+  if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
+  hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, P);
+  c->cur = 1 - c->cur;
+  c->stepCount++;
+  return LSDEM_OK;
+}
+
+d3 rd3(const double* p, size_t i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+void wr3(double* p, size_t i, d3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+q4 rq4(const double* p, size_t i) { return q4{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]}; }
+
+s6 inverse6(s6 A) {
+  const double det = A.xx * (A.yy * A.zz - A.yz * A.yz) + A.xy * (A.yz * A.xz - A.xy * A.zz) +
+                     A.xz * (A.xy * A.yz - A.yy * A.xz);
+  if (std::fabs(det) < 1e-30) return s6{0, 0, 0, 0, 0, 0};
+  const double di = 1.0 / det;
+  return s6{(A.yy * A.zz - A.yz * A.yz) * di,  (A.xx * A.zz - A.xz * A.xz) * di,
+            (A.xx * A.yy - A.xy * A.xy) * di,  -(A.xy * A.zz - A.xz * A.yz) * di,
+            (A.xy * A.yz - A.xz * A.yy) * di,  -(A.xx * A.yz - A.xz * A.xy) * di};
+}
+
+}  // namespace
+
+extern "C" {
+
+void lsdem_config_default(lsdem_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->domain_max[0] = cfg->domain_max[1] = cfg->domain_max[2] = 1.0;
+  cfg->dt = 1e-5;
+  cfg->quirks = LSDEM_QUIRK_ALL;
+  cfg->block_size = 256;
+}
+
+const char* lsdem_last_error(void) { return g_err.c_str(); }
+
+int lsdem_create(const lsdem_config* cfg, lsdem_ctx** out) {
+  if (!cfg || !out) return fail(LSDEM_ERR_ARG, "null argument");
+  *out = nullptr;
+  const d3 ext = mk3(cfg->domain_max[0] - cfg->domain_min[0], cfg->domain_max[1] - cfg->domain_min[1],
+                     cfg->domain_max[2] - cfg->domain_min[2]);
+  if (!(ext.x > 0.0 && ext.y > 0.0 && ext.z > 0.0)) return fail(LSDEM_ERR_ARG, "invalid simulation domain size");
+  if (!(cfg->dt > 0.0)) return fail(LSDEM_ERR_ARG, "invalid timeStep");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LSDEM_ERR_HIP, "no HIP device available");
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(LSDEM_ERR_ARG, "device ordinal out of range");
+  HIPCHK(hipSetDevice(cfg->device));
+  lsdem_ctx* c = new lsdem_ctx();
+  c->cfg = *cfg;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(LSDEM_ERR_HIP, "hipStreamCreate failed");
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->numCU = prop.multiProcessorCount;
+  *out = c;
+  return LSDEM_OK;
+}
+
+void lsdem_destroy(lsdem_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->cfg.device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+int lsdem_add_shape(lsdem_ctx* c, const float* sdf, int32_t nx, int32_t ny, int32_t nz, const double origin[3],
+                    double spacing, const double* nodes, int32_t n_nodes, int32_t* shape_id) {
+  if (!c || !sdf || !origin) return fail(LSDEM_ERR_ARG, "null argument");
+  if (c->initialized) return fail(LSDEM_ERR_STATE, "shapes must be added before lsdem_initialize");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(LSDEM_ERR_ARG, "invalid level-set grid size (< 2)");
+  if (!(spacing > 0.0)) return fail(LSDEM_ERR_ARG, "invalid level-set grid spacing");
+  if (n_nodes < 0 || (n_nodes > 0 && !nodes)) return fail(LSDEM_ERR_ARG, "invalid boundary nodes");
+  HostShape S;
+  S.nx = nx; S.ny = ny; S.nz = nz;
+  S.origin = mk3(origin[0], origin[1], origin[2]);
+  S.h = spacing;
+  S.sdf.assign(sdf, sdf + size_t(nx) * ny * nz);
+  for (int i = 0; i < n_nodes; ++i) S.nodes.push_back(rd3(nodes, i));
+  buildClusters(S);
+  shapeBounds(S);
+  c->shapes.push_back(std::move(S));
+  if (shape_id) *shape_id = int(c->shapes.size()) - 1;
+  return LSDEM_OK;
+}
+
+int lsdem_add_particles(lsdem_ctx* c, int32_t n, const int32_t* shape, const double* scale, const double* pos,
+                        const double* quat, const double* vel, const double* angvel, const double* inv_mass,
+                        const double* inv_inertia, const double* kn, const double* ks, const double* mu,
+                        const double* restitution) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (c->initialized) return fail(LSDEM_ERR_STATE, "particles must be added before lsdem_initialize");
+  if (n < 0 || (n > 0 && (!shape || !pos || !quat || !vel || !angvel || !inv_mass || !inv_inertia || !kn || !ks ||
+                          !mu || !restitution)))
+    return fail(LSDEM_ERR_ARG, "null particle array");
+  for (int i = 0; i < n; ++i)
+    if (shape[i] < 0 || shape[i] >= int(c->shapes.size())) return fail(LSDEM_ERR_ARG, "bad shape id");
+  for (int i = 0; i < n; ++i) {
+    const double s = scale ? scale[i] : 1.0;
+    if (!(s > 0.0)) return fail(LSDEM_ERR_ARG, "scale must be > 0");
+    const HostShape& S = c->shapes[shape[i]];
+    double r = 0.;
+    for (const d3& nd : S.nodes) r = std::max(len(nd * s), r);
+    c->pShape.push_back(shape[i]);
+    c->pScale.push_back(s);
+    c->pRadius.push_back(r);
+    c->pPos.push_back(rd3(pos, i));
+    c->pVel.push_back(rd3(vel, i));
+    c->pAng.push_back(rd3(angvel, i));
+    c->pQuat.push_back(qnormalize(rq4(quat, i)));
+    c->pInvM.push_back(inv_mass[i]);
+    const double* I = inv_inertia + 6 * i;
+    c->pInvI.push_back(s6{I[0], I[1], I[2], I[3], I[4], I[5]});
+    c->pKn.push_back(kn[i] > 0. ? kn[i] : 0.);
+    c->pKs.push_back(ks[i] > 0. ? ks[i] : 0.);
+    c->pMu.push_back(mu[i] > 0. ? mu[i] : 0.);
+    double e = restitution[i];
+    if (e <= 0.0 || e > 1.0) e = 1.0;
+    c->pRes.push_back(e);
+  }
+  return LSDEM_OK;
+}
+
+int lsdem_add_walls(lsdem_ctx* c, int32_t n, const int32_t* shape, const double* bound_radius, const double* pos,
+                    const double* quat, const double* vel, const double* angvel, const double* mu,
+                    const double* restitution) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (c->initialized) return fail(LSDEM_ERR_STATE, "walls must be added before lsdem_initialize");
+  if (n < 0 || (n > 0 && (!shape || !bound_radius || !pos || !quat || !vel || !angvel || !mu || !restitution)))
+    return fail(LSDEM_ERR_ARG, "null wall array");
+  for (int i = 0; i < n; ++i)
+    if (shape[i] < 0 || shape[i] >= int(c->shapes.size())) return fail(LSDEM_ERR_ARG, "bad shape id");
+  for (int i = 0; i < n; ++i) {
+    c->wShape.push_back(shape[i]);
+    c->wRadius.push_back(bound_radius[i]);
+    c->wPos.push_back(rd3(pos, i));
+    c->wQuat.push_back(qnormalize(rq4(quat, i)));
+    c->wVel.push_back(rd3(vel, i));
+    c->wAng.push_back(rd3(angvel, i));
+    c->wMu.push_back(mu[i] > 0. ? mu[i] : 0.);
+    double e = restitution[i];
+    if (e <= 0.0 || e > 1.0) e = 1.0;
+    c->wRes.push_back(e);
+  }
+  return LSDEM_OK;
+}
+
+int lsdem_add_bond(lsdem_ctx* c, int32_t i, int32_t j, const double point[3], const double normal[3], double r,
+                   double L, double E, double nu, double sigmaT, double coh, double bmu) {
+  if (!c || !point || !normal) return fail(LSDEM_ERR_ARG, "null argument");
+  if (c->initialized) return fail(LSDEM_ERR_STATE, "bonds must be added before lsdem_initialize");
+  const int np = int(c->pPos.size());
+  if (i < 0 || j < 0 || i >= np || j >= np) return fail(LSDEM_ERR_ARG, "bond particle id out of range");
+  const d3 nrm = mk3(normal[0], normal[1], normal[2]);
+  if (isZero(len2(nrm))) return fail(LSDEM_ERR_ARG, "invalid bond normal");
+  if (r <= 0.) return fail(LSDEM_ERR_ARG, "invalid bond radius");
+  if (L <= 0.) return fail(LSDEM_ERR_ARG, "invalid bond length");
+  if (E <= 0.) return fail(LSDEM_ERR_ARG, "invalid Young's modulus");
+  if (nu <= -1.0 || nu >= 0.5) return fail(LSDEM_ERR_ARG, "invalid Poisson ratio");
+  // beam constants of the energy-based bond
+  const double G = E / (2. * (1. + nu));
+  const double A = kPi * r * r;
+  const double I = A * r * r / 4.;
+  const double J = 2. * I;
+  const double Kn = E * A / L, Kt = G * J / L, Ks = 12. * E * I / (L * L * L), Kb = E * I / L;
+  const double B1 = Kn, B2 = L * L * Ks, B4 = Kt, B3 = Kb - B2 / 4. - B4 / 2.;
+  const d3 n = nrm / len(nrm);
+  const d3 pt = mk3(point[0], point[1], point[2]);
+  const d3 gpi = pt + 0.5 * L * n, gpj = pt - 0.5 * L * n;
+  const q4 qi = c->pQuat[i], qj = c->pQuat[j];
+  const d3 N1i = -n;
+  const d3 ref = (std::fabs(N1i.x) < 0.9) ? mk3(1., 0., 0.) : mk3(0., 1., 0.);
+  const d3 N2i = normalize(cross(ref, N1i));
+  const d3 N3i = normalize(cross(N1i, N2i));
+  const d3 N1j = -N1i, N2j = N2i, N3j = N3i;
+  c->bI.push_back(i);
+  c->bJ.push_back(j);
+  c->bActive.push_back(1);
+  const double cst[9] = {B1, B2, B3, B4, r, L, sigmaT, coh, bmu};
+  c->bConst.insert(c->bConst.end(), cst, cst + 9);
+  c->bLocal.push_back(normalize(rotInv(N1i, qi)));
+  c->bLocal.push_back(normalize(rotInv(N2i, qi)));
+  c->bLocal.push_back(normalize(rotInv(N3i, qi)));
+  c->bLocal.push_back(rotInv(gpi - c->pPos[i], qi));
+  c->bLocal.push_back(normalize(rotInv(N1j, qj)));
+  c->bLocal.push_back(normalize(rotInv(N2j, qj)));
+  c->bLocal.push_back(normalize(rotInv(N3j, qj)));
+  c->bLocal.push_back(rotInv(gpj - c->pPos[j], qj));
+  c->bPoint.push_back(pt);
+  return LSDEM_OK;
+}
+
+int lsdem_initialize(lsdem_ctx* c) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (c->initialized) return fail(LSDEM_ERR_STATE, "already initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  hipStream_t s = c->stream;
+  const int nP = int(c->pPos.size()), nW = int(c->wPos.size()), nS = int(c->shapes.size());
+  const int nB = int(c->bI.size());
+  Params& P = c->P;
+  P = Params{};
+  P.nP = nP; P.nW = nW; P.nShapes = nS; P.nBonds = nB;
+  P.g = mk3(c->cfg.gravity[0], c->cfg.gravity[1], c->cfg.gravity[2]);
+  P.dt = c->cfg.dt;
+  P.quirks = c->cfg.quirks;
+  P.periodicX = c->cfg.periodic_x != 0;
+  P.periodicY = c->cfg.periodic_y != 0;
+  const d3 dmn = mk3(c->cfg.domain_min[0], c->cfg.domain_min[1], c->cfg.domain_min[2]);
+  const d3 dmx = mk3(c->cfg.domain_max[0], c->cfg.domain_max[1], c->cfg.domain_max[2]);
+
+  // ---- shapes ----
+  std::vector<ShapeDev> sd(std::max(nS, 1));
+  std::vector<ClusterDev> cls;
+  std::vector<NodeDev> nds;
+  std::vector<float> sdf;
+  for (int k = 0; k < nS; ++k) {
+    const HostShape& S = c->shapes[k];
+    ShapeDev& d = sd[k];
+    d = ShapeDev{};
+    d.nx = S.nx; d.ny = S.ny; d.nz = S.nz; d.nNodes = int(S.nodes.size());
+    d.nodeOff = int(nds.size()); d.nClusters = int(S.clusters.size()); d.clusterOff = int(cls.size());
+    d.sdfOff = (long long)sdf.size();
+    d.ox = S.origin.x; d.oy = S.origin.y; d.oz = S.origin.z; d.h = S.h;
+    d.boxR = S.boxR; d.lip = S.lip;
+    if (d.nClusters > 64) return fail(LSDEM_ERR_ARG, "shape has more than 4096 boundary nodes");
+    nds.insert(nds.end(), S.perm.begin(), S.perm.end());
+    cls.insert(cls.end(), S.clusters.begin(), S.clusters.end());
+    sdf.insert(sdf.end(), S.sdf.begin(), S.sdf.end());
+  }
+  int rc;
+  if ((rc = upload(c->dShapes, sd, s))) return rc;
+  if ((rc = upload(c->dClusters, cls, s))) return rc;
+  if ((rc = upload(c->dNodes, nds, s))) return rc;
+  if ((rc = upload(c->dSdf, sdf, s))) return rc;
+
+  // ---- particles ----
+  std::vector<double> invSp(nP);
+  std::vector<d3> gorig(nP), zero3(nP, mk3(0, 0, 0));
+  std::vector<int> nodeStart(nP + 1, 0);
+  double rMax = 0, boxRMax = 0;
+  for (int i = 0; i < nP; ++i) {
+    const HostShape& S = c->shapes[c->pShape[i]];
+    invSp[i] = 1. / (S.h * c->pScale[i]);
+    gorig[i] = S.origin * c->pScale[i];
+    nodeStart[i + 1] = nodeStart[i] + int(S.nodes.size());
+    rMax = std::max(rMax, c->pRadius[i]);
+    boxRMax = std::max(boxRMax, c->pScale[i] * S.boxR);
+  }
+  if ((long long)nodeStart[nP] >= (1ll << 31) - 1) return fail(LSDEM_ERR_ARG, "too many boundary nodes");
+  if ((rc = upload(c->dPos, c->pPos, s))) return rc;
+  if ((rc = upload(c->dPosDet, c->pPos, s))) return rc;
+  if ((rc = upload(c->dVel, c->pVel, s))) return rc;
+  if ((rc = upload(c->dAng, c->pAng, s))) return rc;
+  if ((rc = upload(c->dForce, zero3, s))) return rc;
+  if ((rc = upload(c->dTorque, zero3, s))) return rc;
+  if ((rc = upload(c->dQuat, c->pQuat, s))) return rc;
+  if ((rc = upload(c->dInvI, c->pInvI, s))) return rc;
+  if ((rc = upload(c->dInvM, c->pInvM, s))) return rc;
+  if ((rc = upload(c->dRadius, c->pRadius, s))) return rc;
+  if ((rc = upload(c->dKn, c->pKn, s))) return rc;
+  if ((rc = upload(c->dKs, c->pKs, s))) return rc;
+  if ((rc = upload(c->dMu, c->pMu, s))) return rc;
+  if ((rc = upload(c->dRes, c->pRes, s))) return rc;
+  if ((rc = upload(c->dScale, c->pScale, s))) return rc;
+  if ((rc = upload(c->dInvSp, invSp, s))) return rc;
+  if ((rc = upload(c->dGorig, gorig, s))) return rc;
+  if ((rc = upload(c->dShape, c->pShape, s))) return rc;
+  if ((rc = upload(c->dNodeStart, nodeStart, s))) return rc;
+  HIPCHK(c->dKeyRef.alloc(std::max(nP, 1)));
+  HIPCHK(c->dScell.alloc(std::max(nP, 1)));
+
+  // ---- walls ----
+  if ((rc = upload(c->dWPos, c->wPos, s))) return rc;
+  if ((rc = upload(c->dWVel, c->wVel, s))) return rc;
+  if ((rc = upload(c->dWAng, c->wAng, s))) return rc;
+  if ((rc = upload(c->dWQuat, c->wQuat, s))) return rc;
+  if ((rc = upload(c->dWMu, c->wMu, s))) return rc;
+  if ((rc = upload(c->dWRes, c->wRes, s))) return rc;
+  if ((rc = upload(c->dWShape, c->wShape, s))) return rc;
+  HIPCHK(c->dWKey.alloc(std::max(nW, 1)));
+
+  // ---- grids ----
+  double cellW = 0;
+  for (double r : c->wRadius) cellW = std::max(cellW, r);
+  P.gridP = makeGrid(dmn, dmx, nP ? rMax * 2.0 : 0.);
+  P.gridW = makeGrid(dmn, dmx, nW ? cellW * 2.0 : 0.);
+  P.rMax = rMax;
+  P.boxRMax = boxRMax;
+  {
+    const d3 ext = dmx - dmn;
+    double cell = rMax > 0 ? rMax : std::max(ext.x, std::max(ext.y, ext.z));
+    for (;;) {
+      const double nc = std::ceil(ext.x / cell) * std::ceil(ext.y / cell) * std::ceil(ext.z / cell);
+      if (nc <= 4.0e6) break;
+      cell *= 1.25;
+    }
+    SearchGrid& S = P.sg;
+    S.mn = dmn;
+    S.cell = cell;
+    S.invCell = 1.0 / cell;
+    S.size = i3{std::max(1, int(std::ceil(ext.x / cell))), std::max(1, int(std::ceil(ext.y / cell))),
+                std::max(1, int(std::ceil(ext.z / cell)))};
+    S.ncell = S.size.x * S.size.y * S.size.z;
+  }
+  const int ncell = P.sg.ncell;
+  HIPCHK(c->dCellCount.alloc(ncell));
+  HIPCHK(c->dCellStart.alloc(ncell + 1));
+  HIPCHK(c->dSorted.alloc(std::max(nP, 1)));
+  HIPCHK(c->dRank.alloc(std::max(nP, 1)));
+  const bool periodic = P.periodicX || P.periodicY;
+  for (int g = 0; g < 3; ++g) {
+    const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
+    const size_t n = on ? size_t(std::max(nP, 1)) : 1;
+    HIPCHK(c->dGPos[g].alloc(n));
+    HIPCHK(c->dGKey[g].alloc(n));
+    HIPCHK(c->dGCell[g].alloc(n));
+    HIPCHK(c->dGSorted[g].alloc(n));
+    HIPCHK(c->dGRank[g].alloc(n));
+    HIPCHK(c->dGCount[g].alloc(on ? ncell : 1));
+    HIPCHK(c->dGCellStart[g].alloc(on ? ncell + 1 : 1));
+  }
+
+  // ---- pairs and contacts ----
+  long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);
+  long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity : 64ll * nP + 16384;
+  pairCap = std::min<long long>(pairCap, (1ll << 31) - 1);
+  contactCap = std::min<long long>(contactCap, (1ll << 30));
+  P.pairCap = pairCap;
+  P.contactCap = contactCap;
+  HIPCHK(c->dPairCount.alloc(std::max(nP, 1)));
+  HIPCHK(c->dPairStart.alloc(nP + 1));
+  HIPCHK(c->dPairA.alloc(pairCap));
+  HIPCHK(c->dPairInfo.alloc(pairCap));
+  HIPCHK(c->dPairFlag.alloc(pairCap));
+  HIPCHK(c->dPairSum.alloc(9 * pairCap));
+  HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
+  HIPCHK(c->dRcStart.alloc(nP + 1));
+  HIPCHK(c->dRcPair.alloc(pairCap));
+  HIPCHK(c->dRcRank.alloc(pairCap));
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(c->dCNode[b].alloc(contactCap));
+    HIPCHK(c->dCInfo[b].alloc(contactCap));
+    HIPCHK(c->dCSpring[b].alloc(contactCap));
+    HIPCHK(c->dCPoint[b].alloc(contactCap));
+    HIPCHK(c->dCNormal[b].alloc(contactCap));
+    HIPCHK(c->dCOverlap[b].alloc(contactCap));
+    HIPCHK(c->dCEn[b].alloc(contactCap));
+    HIPCHK(c->dCEs[b].alloc(contactCap));
+  }
+  long long hs = 1024;
+  while (hs < 2 * contactCap) hs <<= 1;
+  c->hashSize = hs;
+  HIPCHK(c->dHashKey.alloc(hs));
+  HIPCHK(c->dHashVal.alloc(hs));
+  P.hashMask = (unsigned long long)(hs - 1);
+
+  // ---- bonds (+ per-particle CSR of bond ends, in bond order) ----
+  {
+    std::vector<int> cnt(nP + 1, 0), item(2 * nB);
+    for (int k = 0; k < nB; ++k) { cnt[c->bI[k] + 1]++; cnt[c->bJ[k] + 1]++; }
+    for (int i = 0; i < nP; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < nB; ++k) {
+      item[fillp[c->bI[k]]++] = (k << 1);
+      item[fillp[c->bJ[k]]++] = (k << 1) | 1;
+    }
+    if ((rc = upload(c->dBI, c->bI, s))) return rc;
+    if ((rc = upload(c->dBJ, c->bJ, s))) return rc;
+    if ((rc = upload(c->dBActive, c->bActive, s))) return rc;
+    if ((rc = upload(c->dBConst, c->bConst, s))) return rc;
+    if ((rc = upload(c->dBLocal, c->bLocal, s))) return rc;
+    if ((rc = upload(c->dBPoint, c->bPoint, s))) return rc;
+    if ((rc = upload(c->dBcStart, cnt, s))) return rc;
+    if ((rc = upload(c->dBcItem, item, s))) return rc;
+    HIPCHK(c->dBOut.alloc(std::max(6 * nB, 1)));
+    HIPCHK(c->dBF.alloc(std::max(nB, 1)));
+    HIPCHK(c->dBMi.alloc(std::max(nB, 1)));
+    HIPCHK(c->dBMj.alloc(std::max(nB, 1)));
+    HIPCHK(hipMemsetAsync(c->dBOut.p, 0, sizeof(double) * std::max(6 * nB, 1), s));
+  }
+
+  // ---- scratch ----
+  long long maxScan = std::max<long long>({(long long)nP, (long long)ncell, 1ll});
+  c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
+  HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
+  HIPCHK(c->dDevInt.alloc(8));
+  HIPCHK(c->dDevStat.alloc(4));
+  HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, 8 * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 4 * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
+  HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
+  hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, c->dHashKey.p, hs, kHashEmpty);
+
+  // ---- pointer table ----
+  P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
+  P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;
+  P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
+  P.radius = c->dRadius.p; P.kn = c->dKn.p; P.ks = c->dKs.p; P.mu = c->dMu.p; P.res = c->dRes.p;
+  P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
+  P.nodeStart = c->dNodeStart.p; P.keyRef = c->dKeyRef.p; P.scell = c->dScell.p;
+  P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
+  P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
+  for (int g = 0; g < 3; ++g) {
+    P.gpos[g] = c->dGPos[g].p; P.gkey[g] = c->dGKey[g].p; P.gcell[g] = c->dGCell[g].p;
+    P.gCellStart[g] = c->dGCellStart[g].p; P.gSorted[g] = c->dGSorted[g].p; P.gCount[g] = c->dGCount[g].p;
+  }
+  P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
+  P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
+  P.pairInfo = c->dPairInfo.p; P.pairFlag = c->dPairFlag.p; P.pairSum = c->dPairSum.p;
+  P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
+  P.hashKey = c->dHashKey.p; P.hashVal = c->dHashVal.p;
+  P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
+  P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
+  P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
+  P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
+  c->cur = 0;
+  setCurrentBuffers(c);
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipGetLastError());
+  c->initialized = true;
+  return LSDEM_OK;
+}
+
+int lsdem_step(lsdem_ctx* c, int32_t n) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (n < 0) return fail(LSDEM_ERR_ARG, "negative step count");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
+  for (int k = 0; k < n; ++k) {
+    int rc = runStep(c);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int flags[8];
+  HIPCHK(hipMemcpy(flags, c->dDevInt.p, sizeof flags, hipMemcpyDeviceToHost));
+  if (flags[3] & 1)
+    return fail(LSDEM_ERR_CAPACITY, "candidate-pair buffer overflow (peak " + std::to_string(flags[4]) +
+                                        " > capacity " + std::to_string(c->P.pairCap) + ")");
+  if (flags[3] & 2)
+    return fail(LSDEM_ERR_CAPACITY, "contact buffer overflow (peak " + std::to_string(flags[5]) + " > capacity " +
+                                        std::to_string(c->P.contactCap) + ")");
+  return LSDEM_OK;
+}
+
+int lsdem_get_particles(lsdem_ctx* c, double* pos, double* quat, double* vel, double* angvel, double* force,
+                        double* torque) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t n = size_t(c->P.nP);
+  if (n == 0) return LSDEM_OK;
+  if (pos) HIPCHK(hipMemcpy(pos, c->dPos.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  if (quat) HIPCHK(hipMemcpy(quat, c->dQuat.p, n * sizeof(q4), hipMemcpyDeviceToHost));
+  if (vel) HIPCHK(hipMemcpy(vel, c->dVel.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  if (angvel) HIPCHK(hipMemcpy(angvel, c->dAng.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  if (force) HIPCHK(hipMemcpy(force, c->dForce.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  if (torque) HIPCHK(hipMemcpy(torque, c->dTorque.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  return LSDEM_OK;
+}
+
+int lsdem_set_particles(lsdem_ctx* c, const double* pos, const double* quat, const double* vel, const double* angvel,
+                        const double* force, const double* torque) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t n = size_t(c->P.nP);
+  if (n == 0) return LSDEM_OK;
+  if (pos) {
+    HIPCHK(hipMemcpy(c->dPos.p, pos, n * sizeof(d3), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dPosDet.p, pos, n * sizeof(d3), hipMemcpyHostToDevice));
+  }
+  if (quat) HIPCHK(hipMemcpy(c->dQuat.p, quat, n * sizeof(q4), hipMemcpyHostToDevice));
+  if (vel) HIPCHK(hipMemcpy(c->dVel.p, vel, n * sizeof(d3), hipMemcpyHostToDevice));
+  if (angvel) HIPCHK(hipMemcpy(c->dAng.p, angvel, n * sizeof(d3), hipMemcpyHostToDevice));
+  if (force) HIPCHK(hipMemcpy(c->dForce.p, force, n * sizeof(d3), hipMemcpyHostToDevice));
+  if (torque) HIPCHK(hipMemcpy(c->dTorque.p, torque, n * sizeof(d3), hipMemcpyHostToDevice));
+  return LSDEM_OK;
+}
+
+int lsdem_get_walls(lsdem_ctx* c, double* pos, double* quat, double* vel, double* angvel) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t n = size_t(c->P.nW);
+  if (n == 0) return LSDEM_OK;
+  if (pos) HIPCHK(hipMemcpy(pos, c->dWPos.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  if (quat) HIPCHK(hipMemcpy(quat, c->dWQuat.p, n * sizeof(q4), hipMemcpyDeviceToHost));
+  if (vel) HIPCHK(hipMemcpy(vel, c->dWVel.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  if (angvel) HIPCHK(hipMemcpy(angvel, c->dWAng.p, n * sizeof(d3), hipMemcpyDeviceToHost));
+  return LSDEM_OK;
+}
+
+int lsdem_set_walls(lsdem_ctx* c, const double* pos, const double* quat, const double* vel, const double* angvel) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t n = size_t(c->P.nW);
+  if (n == 0) return LSDEM_OK;
+  if (pos) HIPCHK(hipMemcpy(c->dWPos.p, pos, n * sizeof(d3), hipMemcpyHostToDevice));
+  if (quat) HIPCHK(hipMemcpy(c->dWQuat.p, quat, n * sizeof(q4), hipMemcpyHostToDevice));
+  if (vel) HIPCHK(hipMemcpy(c->dWVel.p, vel, n * sizeof(d3), hipMemcpyHostToDevice));
+  if (angvel) HIPCHK(hipMemcpy(c->dWAng.p, angvel, n * sizeof(d3), hipMemcpyHostToDevice));
+  return LSDEM_OK;
+}
+
+int lsdem_get_contacts(lsdem_ctx* c, int32_t list, int64_t capacity, int64_t* count, int32_t* master,
+                       int32_t* slave, double* point, double* normal, double* overlap, double* spring,
+                       double* energy_n, double* energy_s) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  if (list < 0 || list > 4) return fail(LSDEM_ERR_ARG, "bad list id");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (count) *count = 0;
+  if (c->stepCount == 0 || c->P.nP == 0) return LSDEM_OK;
+  int di[8];
+  HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
+  const long long n = std::min<long long>(di[1], c->P.contactCap);
+  const int b = 1 - c->cur;  // buffer written by the last step
+  std::vector<int> node(n);
+  std::vector<uint32_t> info(n);
+  HIPCHK(hipMemcpy(node.data(), c->dCNode[b].p, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(info.data(), c->dCInfo[b].p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::vector<long long> sel;
+  for (long long k = 0; k < n; ++k)
+    if (int(info[k] >> 28) == list) sel.push_back(k);
+  std::sort(sel.begin(), sel.end(), [&](long long a, long long b2) {
+    if (node[a] != node[b2]) return node[a] < node[b2];
+    return (info[a] & kPartnerMask) < (info[b2] & kPartnerMask);
+  });
+  if (count) *count = (int64_t)sel.size();
+  const long long m = std::min<long long>(capacity, (long long)sel.size());
+  std::vector<d3> vp, vn, vs;
+  std::vector<double> vo, ven, ves;
+  if (point) { vp.resize(n); HIPCHK(hipMemcpy(vp.data(), c->dCPoint[b].p, n * sizeof(d3), hipMemcpyDeviceToHost)); }
+  if (normal) { vn.resize(n); HIPCHK(hipMemcpy(vn.data(), c->dCNormal[b].p, n * sizeof(d3), hipMemcpyDeviceToHost)); }
+  if (spring) { vs.resize(n); HIPCHK(hipMemcpy(vs.data(), c->dCSpring[b].p, n * sizeof(d3), hipMemcpyDeviceToHost)); }
+  if (overlap) { vo.resize(n); HIPCHK(hipMemcpy(vo.data(), c->dCOverlap[b].p, n * sizeof(double), hipMemcpyDeviceToHost)); }
+  if (energy_n) { ven.resize(n); HIPCHK(hipMemcpy(ven.data(), c->dCEn[b].p, n * sizeof(double), hipMemcpyDeviceToHost)); }
+  if (energy_s) { ves.resize(n); HIPCHK(hipMemcpy(ves.data(), c->dCEs[b].p, n * sizeof(double), hipMemcpyDeviceToHost)); }
+  for (long long k = 0; k < m; ++k) {
+    const long long s = sel[k];
+    if (master) master[k] = node[s];
+    if (slave) slave[k] = int(info[s] & kPartnerMask);
+    if (point) wr3(point, k, vp[s]);
+    if (normal) wr3(normal, k, vn[s]);
+    if (spring) wr3(spring, k, vs[s]);
+    if (overlap) overlap[k] = vo[s];
+    if (energy_n) energy_n[k] = ven[s];
+    if (energy_s) energy_s[k] = ves[s];
+  }
+  return LSDEM_OK;
+}
+
+int lsdem_get_bonds(lsdem_ctx* c, int32_t* active, double* point, double* energies4, double* sigma_max,
+                    double* tau_max) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int nB = c->P.nBonds;
+  if (nB == 0) return LSDEM_OK;
+  if (active) HIPCHK(hipMemcpy(active, c->dBActive.p, nB * sizeof(int), hipMemcpyDeviceToHost));
+  if (point) HIPCHK(hipMemcpy(point, c->dBPoint.p, nB * sizeof(d3), hipMemcpyDeviceToHost));
+  if (energies4 || sigma_max || tau_max) {
+    std::vector<double> o(6 * size_t(nB));
+    HIPCHK(hipMemcpy(o.data(), c->dBOut.p, o.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nB; ++k) {
+      if (energies4) for (int t = 0; t < 4; ++t) energies4[4 * k + t] = o[6 * k + t];
+      if (sigma_max) sigma_max[k] = o[6 * k + 4];
+      if (tau_max) tau_max[k] = o[6 * k + 5];
+    }
+  }
+  return LSDEM_OK;
+}
+
+int lsdem_get_counters(lsdem_ctx* c, lsdem_counters* out) {
+  if (!c || !out) return fail(LSDEM_ERR_ARG, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->steps = c->stepCount;
+  out->pair_capacity = c->P.pairCap;
+  out->contact_capacity = c->P.contactCap;
+  if (!c->initialized || c->P.nP == 0) return LSDEM_OK;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int di[8];
+  unsigned long long ds[4];
+  int np = 0;
+  HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&np, c->dPairStart.p + c->P.nP, sizeof(int), hipMemcpyDeviceToHost));
+  out->pairs = np;
+  out->contacts = di[1];
+  out->node_tests = (int64_t)ds[0];
+  out->sdf_queries = (int64_t)ds[1];
+  return LSDEM_OK;
+}
+
+int lsdem_mass_properties(const double* sdf, int32_t nx, int32_t ny, int32_t nz, const double origin[3],
+                          double h, double density, double* mass, double centroid[3], double* inertia6,
+                          double* inv_inertia6) {
+  if (!sdf || !origin || !mass || !centroid) return fail(LSDEM_ERR_ARG, "null argument");
+  if (nx < 1 || ny < 1 || nz < 1 || !(h > 0.0)) return fail(LSDEM_ERR_ARG, "invalid grid");
+  // smoothed Heaviside of half-width 1.5 cells
+  auto H = [](double phi) -> double {
+    const double eps = 1.5;
+    if (phi < -eps) return 1.0;
+    if (phi > eps) return 0.0;
+    const double x = -phi / eps;
+    return 0.5 * (1.0 + x + std::sin(kPi * x) / kPi);
+  };
+  const i3 n{nx, ny, nz};
+  const double mg = density * h * h * h;
+  double m = 0.;
+  for (int x = 0; x < nx; x++)
+    for (int y = 0; y < ny; y++)
+      for (int z = 0; z < nz; z++) m += H(sdf[lin3(i3{x, y, z}, n)] / h);
+  m *= mg;
+  *mass = m;
+  d3 c = mk3(0., 0., 0.);
+  s6 I{0, 0, 0, 0, 0, 0};
+  if (m > 0.) {
+    const d3 o = mk3(origin[0], origin[1], origin[2]);
+    for (int x = 0; x < nx; x++)
+      for (int y = 0; y < ny; y++)
+        for (int z = 0; z < nz; z++) {
+          const double Hv = H(sdf[lin3(i3{x, y, z}, n)] / h);
+          c.x += Hv * (o.x + double(x) * h);
+          c.y += Hv * (o.y + double(y) * h);
+          c.z += Hv * (o.z + double(z) * h);
+        }
+    c *= mg / m;
+    for (int x = 0; x < nx; x++)
+      for (int y = 0; y < ny; y++)
+        for (int z = 0; z < nz; z++) {
+          const double Hv = H(sdf[lin3(i3{x, y, z}, n)] / h);
+          const d3 r = o + h * mk3(double(x), double(y), double(z)) - c;
+          I.xx += Hv * (r.y * r.y + r.z * r.z) * mg;
+          I.yy += Hv * (r.x * r.x + r.z * r.z) * mg;
+          I.zz += Hv * (r.y * r.y + r.x * r.x) * mg;
+          I.xy -= Hv * r.x * r.y * mg;
+          I.xz -= Hv * r.x * r.z * mg;
+          I.yz -= Hv * r.y * r.z * mg;
+        }
+  }
+  centroid[0] = c.x; centroid[1] = c.y; centroid[2] = c.z;
+  if (inertia6) { inertia6[0] = I.xx; inertia6[1] = I.yy; inertia6[2] = I.zz; inertia6[3] = I.xy; inertia6[4] = I.xz; inertia6[5] = I.yz; }
+  if (inv_inertia6) {
+    const s6 Ii = m > 0. ? inverse6(I) : s6{0, 0, 0, 0, 0, 0};
+    inv_inertia6[0] = Ii.xx; inv_inertia6[1] = Ii.yy; inv_inertia6[2] = Ii.zz;
+    inv_inertia6[3] = Ii.xy; inv_inertia6[4] = Ii.xz; inv_inertia6[5] = Ii.yz;
+  }
+  return LSDEM_OK;
+}
+
+}  // extern "C"

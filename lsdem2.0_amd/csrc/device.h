@@ -1,0 +1,155 @@
+// Device data layout of one LS-DEM context.  All arrays live in HBM; the struct is passed by
+// value to every kernel.
+#pragma once
+#include "dmath.h"
+
+namespace lsd {
+
+enum PairType : uint32_t { PT_PP = 0, PT_PW = 1, PT_GX = 2, PT_GY = 3, PT_GXY = 4 };
+constexpr uint32_t kPartnerMask = 0x0FFFFFFFu;
+constexpr uint64_t kHashEmpty = ~0ull;
+
+// Shape: FP32 SDF grid (x fastest) + boundary nodes grouped into clusters of <= 64 nodes.
+struct ShapeDev {
+  int nx, ny, nz, nNodes;
+  int nodeOff, nClusters, clusterOff, pad0;
+  long long sdfOff;
+  double ox, oy, oz, h;  // unscaled grid origin and spacing
+  double boxR;           // max |corner| of the grid box (unscaled), used for fixed-partner culls
+  double lip;            // Lipschitz bound of the trilinear interpolant (scale invariant)
+};
+struct ClusterDev {
+  double cx, cy, cz, rho;  // bounding sphere of the cluster's nodes (unscaled)
+  int first, count, pad0, pad1;
+};
+struct NodeDev {
+  double x, y, z;  // unscaled local position
+  int orig;        // index of the node in the shape's node list as given by the caller
+  int pad;
+};
+
+struct GridGeom {  // geometry of the reference spatial grid (spatialGrid::set)
+  d3 mn, mx, inv;
+  i3 size;
+  int pad;
+};
+struct SearchGrid {  // this library's own broad-phase grid
+  d3 mn;
+  double cell, invCell;
+  i3 size;
+  int ncell;
+};
+
+struct Params {
+  // ---- sizes and scalars ----
+  int nP, nW, nShapes, nBonds;
+  d3 g;
+  double dt;
+  int quirks, periodicX, periodicY, pad0;
+  GridGeom gridP, gridW;
+  SearchGrid sg;
+  double rMax;        // max particle bounding radius
+  double boxRMax;     // max scaled grid-box radius over particles (ghost search reach)
+  long long pairCap, contactCap;
+  unsigned long long hashMask;
+
+  // ---- shapes ----
+  const ShapeDev* shapes;
+  const ClusterDev* clusters;
+  const NodeDev* nodes;
+  const float* sdf;
+
+  // ---- particles ----
+  d3* pos;      // state position (after any periodic wrap)
+  d3* posDet;   // position used by the particle-particle and wall detection of this step
+  q4* quat;
+  d3* vel;
+  d3* angvel;
+  d3* force;
+  d3* torque;
+  const s6* invI;
+  const double* invM;
+  const double* radius;
+  const double* kn;
+  const double* ks;
+  const double* mu;
+  const double* res;
+  const double* scale;
+  const double* invSp;   // 1 / (scale * spacing)
+  const d3* gorig;       // scale * grid origin
+  const int* shape;
+  const int* nodeStart;  // N+1 prefix of boundary-node counts
+  int* keyRef;           // reference-grid key of posDet (-1 = outside the domain)
+  int* scell;            // search-grid cell of posDet
+
+  // ---- walls ----
+  d3* wpos;
+  q4* wquat;
+  d3* wvel;
+  d3* wangvel;
+  const double* wmu;
+  const double* wres;
+  const int* wshape;
+  int* wkey;
+
+  // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
+  d3* gpos[3];
+  int* gkey[3];
+  int* gcell[3];     // search-grid cell (or -1 when the image has no key)
+  int* gCellStart[3];  // search-grid table of the images of each set (ncell + 1)
+  int* gSorted[3];
+  int* gCount[3];
+
+  // ---- search grid of particles ----
+  int* cellCount;   // ncell (scratch)
+  int* cellStart;   // ncell + 1
+  int* sorted;      // nP
+  int* rank;        // nP (scratch)
+
+  // ---- candidate pairs (current step) ----
+  int* pairCount;   // nP (+1) counts then exclusive prefix pairStart
+  int* pairStart;   // nP + 1
+  int* pairA;       // pairCap
+  uint32_t* pairInfo;  // (type << 28) | partner
+  int* pairFlag;    // contacts in the pair (0 = no sums written)
+  double* pairSum;  // 9 per pair: F (A side), torque on A, torque on partner
+  // reaction CSR (pairs whose partner is a particle that receives the reaction)
+  int* rcCount;     // nP
+  int* rcStart;     // nP + 1
+  int* rcPair;      // pairCap
+  int* rcRank;      // pairCap (scratch)
+
+  // ---- contacts (current step) ----
+  int* cNode;
+  uint32_t* cInfo;
+  d3* cSpring;
+  d3* cPoint;
+  d3* cNormal;
+  double* cOverlap;
+  double* cEn;
+  double* cEs;
+  // previous step's springs, addressed through the hash table
+  const d3* prevSpring;
+  unsigned long long* hashKey;
+  int* hashVal;
+
+  // ---- bonds ----
+  const int* bI;
+  const int* bJ;
+  int* bActive;
+  const double* bConst;  // 9 per bond: B1 B2 B3 B4 radius L0 sigmaT cohesion mu
+  const d3* bLocal;      // 8 per bond: n1i n2i n3i pi n1j n2j n3j pj
+  d3* bPoint;
+  double* bOut;          // 6 per bond: Un Us Ub Ut sigmaMax tauMax
+  d3* bF;                // force on i
+  d3* bMi;               // torque on i (moment + arm x F)
+  d3* bMj;               // torque on j
+  const int* bcStart;    // nP + 1 CSR of bond ends per particle
+  const int* bcItem;     // 2*nBonds: (bond << 1) | (0 = i end, 1 = j end)
+
+  // ---- device counters ----
+  int* devInt;                 // [0] nPairs, [1] nContacts, [2] work counter, [3] overflow flags
+  unsigned long long* devStat; // [0] node tests, [1] sdf queries
+};
+
+}  // namespace lsd

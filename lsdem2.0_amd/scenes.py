@@ -1,0 +1,328 @@
+"""Synthetic scenes of the benchmark configurations (numpy; host-side setup only).
+
+Shapes follow the reference's level-set generators: a superellipsoid SDF grid holds
+value/|grad value| of the implicit function (FunDEM/kernel/helper/LevelSetObjectGenerator.h:
+122-224, 989-1361), boundary nodes are Fibonacci-sphere directions projected radially onto the
+surface (the radial projection of :1011-1044), and mass properties come from a smoothed
+Heaviside over the grid (FunDEM/kernel/LSSolver.h:51-163).  Plane walls are thin slab grids
+holding phi = z, which trilinear interpolation reproduces exactly.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GOLDEN_ANGLE = np.pi * (3.0 - np.sqrt(5.0))
+
+
+# ----------------------------------------------------------------------------------------------
+# Shapes
+# ----------------------------------------------------------------------------------------------
+def superellipsoid_value_grad(p: np.ndarray, rx, ry, rz, ee, en):
+    """Implicit function f = (|x/rx|^(2/ee) + |y/ry|^(2/ee))^(ee/en) + |z/rz|^(2/en) - 1 and its gradient."""
+    ax, ay, az = np.abs(p[..., 0] / rx), np.abs(p[..., 1] / ry), np.abs(p[..., 2] / rz)
+    exy, ez, outer = 2.0 / ee, 2.0 / en, ee / en
+    xt, yt = ax ** exy, ay ** exy
+    s = xt + yt
+    f = s ** outer + az ** ez - 1.0
+    eps = 1e-30
+    so = np.maximum(s, eps) ** (outer - 1.0)
+    sgn = lambda v: np.where(v >= 0.0, 1.0, -1.0)
+    dsx = exy * np.maximum(ax, eps) ** (exy - 1.0) * sgn(p[..., 0]) / rx
+    dsy = exy * np.maximum(ay, eps) ** (exy - 1.0) * sgn(p[..., 1]) / ry
+    gx = outer * so * dsx
+    gy = outer * so * dsy
+    gz = ez * np.maximum(az, eps) ** (ez - 1.0) * sgn(p[..., 2]) / rz
+    return f, np.stack([gx, gy, gz], axis=-1)
+
+
+def superellipsoid_grid(rx, ry, rz, ee, en, n: int, spacing: float):
+    """Centred n^3 grid of value/|grad| (float32, indexed [z, y, x]) and its origin."""
+    o = -0.5 * (n - 1) * spacing
+    ax = o + spacing * np.arange(n)
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    f, g = superellipsoid_value_grad(np.stack([x, y, z], axis=-1), rx, ry, rz, ee, en)
+    gn = np.linalg.norm(g, axis=-1)
+    sdf = np.where(gn > 1e-12, f / np.where(gn > 1e-12, gn, 1.0), np.where(f <= 0.0, -0.5, 0.5) * spacing)
+    return sdf.astype(np.float32), np.array([o, o, o])
+
+
+def fibonacci_directions(m: int) -> np.ndarray:
+    i = np.arange(m)
+    z = 1.0 - (2.0 * i + 1.0) / m
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = i * GOLDEN_ANGLE
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def superellipsoid_nodes(rx, ry, rz, ee, en, m: int) -> np.ndarray:
+    d = fibonacci_directions(m)
+    exy, ez, outer = 2.0 / ee, 2.0 / en, ee / en
+    s = (np.abs(d[:, 0] / rx) ** exy + np.abs(d[:, 1] / ry) ** exy) ** outer + np.abs(d[:, 2] / rz) ** ez
+    return d * (s ** (-0.5 * en))[:, None]
+
+
+def plane_wall_shape(half_extent: float, thickness: float):
+    """Slab grid around the local plane z = 0 holding phi = z (normal +z).  A power-of-two
+    thickness keeps the corner values +-h/2 exact in float32."""
+    h = thickness
+    n = 2 * int(np.ceil(half_extent / h)) + 1
+    o = np.array([-0.5 * (n - 1) * h, -0.5 * (n - 1) * h, -0.5 * h])
+    zs = o[2] + h * np.arange(2)
+    sdf = np.broadcast_to(zs[:, None, None], (2, n, n)).astype(np.float32).copy()
+    return sdf, o, h
+
+
+def mass_properties(sdf: np.ndarray, origin, h: float, density: float):
+    """Mass, centroid, inertia (xx,yy,zz,xy,xz,yz) and inverse inertia, smoothed Heaviside eps = 1.5 cells."""
+    phi = sdf.astype(np.float64) / h
+    eps = 1.5
+    x = -phi / eps
+    H = np.where(phi < -eps, 1.0, np.where(phi > eps, 0.0, 0.5 * (1.0 + x + np.sin(np.pi * x) / np.pi)))
+    nz, ny, nx = sdf.shape
+    zc = origin[2] + h * np.arange(nz)
+    yc = origin[1] + h * np.arange(ny)
+    xc = origin[0] + h * np.arange(nx)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    mg = density * h ** 3
+    m = H.sum() * mg
+    c = np.array([(H * X).sum(), (H * Y).sum(), (H * Z).sum()]) * mg / m
+    rx, ry, rz = X - c[0], Y - c[1], Z - c[2]
+    I = np.array([(H * (ry * ry + rz * rz)).sum(), (H * (rx * rx + rz * rz)).sum(), (H * (rx * rx + ry * ry)).sum(),
+                  -(H * rx * ry).sum(), -(H * rx * rz).sum(), -(H * ry * rz).sum()]) * mg
+    return m, c, I, inv_sym(I)
+
+
+def inv_sym(I):
+    M = np.array([[I[0], I[3], I[4]], [I[3], I[1], I[5]], [I[4], I[5], I[2]]])
+    Mi = np.linalg.inv(M)
+    return np.array([Mi[0, 0], Mi[1, 1], Mi[2, 2], Mi[0, 1], Mi[0, 2], Mi[1, 2]])
+
+
+def random_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform random unit quaternions (q0 scalar first), Shoemake's method."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    s1, s2 = np.sqrt(1.0 - u1), np.sqrt(u1)
+    a, b = 2 * np.pi * u2, 2 * np.pi * u3
+    return np.stack([s2 * np.cos(b), s1 * np.sin(a), s1 * np.cos(a), s2 * np.sin(b)], axis=1)
+
+
+def quat_axis_angle(axis, angle):
+    axis = np.asarray(axis, float)
+    axis = axis / np.linalg.norm(axis)
+    return np.concatenate([[np.cos(angle / 2)], np.sin(angle / 2) * axis])
+
+
+# ----------------------------------------------------------------------------------------------
+# Scene container
+# ----------------------------------------------------------------------------------------------
+@dataclass
+class Scene:
+    cfg: dict
+    shapes: list = field(default_factory=list)      # dicts: sdf, origin, spacing, nodes (+ mass props)
+    particles: dict = field(default_factory=dict)
+    walls: dict = field(default_factory=dict)
+    bonds: list = field(default_factory=list)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_particles(self):
+        return len(self.particles.get("shape", []))
+
+    def load(self, sim):
+        for s in self.shapes:
+            sim.add_shape(s["sdf"], s["origin"], s["spacing"], s["nodes"])
+        p = self.particles
+        if self.n_particles:
+            sim.add_particles(p["shape"], p["scale"], p["pos"], p["quat"], p["vel"], p["angvel"], p["inv_mass"],
+                              p["inv_inertia"], p["kn"], p["ks"], p["mu"], p["e"])
+        w = self.walls
+        if w and len(w["shape"]):
+            sim.add_walls(w["shape"], w["radius"], w["pos"], w["quat"], w["vel"], w["angvel"], w["mu"], w["e"])
+        for b in self.bonds:
+            sim.add_bond(*b)
+        sim.initialize()
+        return sim
+
+
+def _add_shape(shapes, sdf, origin, spacing, nodes, density=None):
+    s = {"sdf": sdf, "origin": np.asarray(origin, float), "spacing": float(spacing), "nodes": np.asarray(nodes, float)}
+    if density is not None:
+        m, c, I, Ii = mass_properties(sdf, origin, spacing, density)
+        # shift the local frame to the centroid, as addLSParticle does
+        s["origin"] = s["origin"] - c
+        s["nodes"] = s["nodes"] - c
+        s["mass"], s["inertia"], s["inv_inertia"] = m, I, Ii
+    shapes.append(s)
+    return len(shapes) - 1
+
+
+def _walls_dict(n):
+    return {"shape": np.zeros(n, np.int32), "radius": np.zeros(n), "pos": np.zeros((n, 3)), "quat": np.zeros((n, 4)),
+            "vel": np.zeros((n, 3)), "angvel": np.zeros((n, 3)), "mu": np.zeros(n), "e": np.ones(n)}
+
+
+def box_walls(shapes, lo, hi, mu, e, thickness=2.0 ** -7):
+    """Six inward-facing plane walls on the faces of the box [lo, hi]."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    half = 0.5 * np.max(hi - lo) + thickness
+    sdf, o, h = plane_wall_shape(half, thickness)
+    sid = _add_shape(shapes, sdf, o, h, np.zeros((0, 3)))
+    c = 0.5 * (lo + hi)
+    faces = [
+        ((c[0], c[1], lo[2]), quat_axis_angle([1, 0, 0], 0.0)),
+        ((c[0], c[1], hi[2]), quat_axis_angle([1, 0, 0], np.pi)),
+        ((lo[0], c[1], c[2]), quat_axis_angle([0, 1, 0], np.pi / 2)),
+        ((hi[0], c[1], c[2]), quat_axis_angle([0, 1, 0], -np.pi / 2)),
+        ((c[0], lo[1], c[2]), quat_axis_angle([1, 0, 0], -np.pi / 2)),
+        ((c[0], hi[1], c[2]), quat_axis_angle([1, 0, 0], np.pi / 2)),
+    ]
+    w = _walls_dict(6)
+    for k, (p, q) in enumerate(faces):
+        w["shape"][k] = sid
+        w["radius"][k] = np.sqrt(2.0) * half
+        w["pos"][k] = p
+        w["quat"][k] = q
+        w["mu"][k] = mu
+        w["e"][k] = e
+    return w
+
+
+def floor_wall(shapes, center_xy, z, half_extent, mu, e, thickness=2.0 ** -6):
+    sdf, o, h = plane_wall_shape(half_extent, thickness)
+    sid = _add_shape(shapes, sdf, o, h, np.zeros((0, 3)))
+    w = _walls_dict(1)
+    w["shape"][0] = sid
+    w["radius"][0] = np.sqrt(2.0) * half_extent
+    w["pos"][0] = (center_xy[0], center_xy[1], z)
+    w["quat"][0] = (1.0, 0.0, 0.0, 0.0)
+    w["mu"][0] = mu
+    w["e"][0] = e
+    return w
+
+
+def _particles(n):
+    return {"shape": np.zeros(n, np.int32), "scale": np.ones(n), "pos": np.zeros((n, 3)), "quat": np.zeros((n, 4)),
+            "vel": np.zeros((n, 3)), "angvel": np.zeros((n, 3)), "inv_mass": np.zeros(n),
+            "inv_inertia": np.zeros((n, 6)), "kn": np.zeros(n), "ks": np.zeros(n), "mu": np.zeros(n),
+            "e": np.ones(n)}
+
+
+# ----------------------------------------------------------------------------------------------
+# Config 1: 1,000 superellipsoids (5/4/3 mm, exponent 4) settling in a 0.1 x 0.1 x 0.3 m box
+# ----------------------------------------------------------------------------------------------
+def scene_c1(n: int = 1000, seed: int = 20261203, n_grid: int = 40, spacing: float = 5e-4, n_nodes: int = 500,
+             lattice: float = 12.5e-3, jitter: float = 0.5e-3, box=(0.1, 0.1, 0.3), dt: float = 1e-5,
+             kn: float = 1e5, restitution: float = 0.5, mu: float = 0.5, density: float = 2500.0,
+             gravity: float = 9.81, quirks: int = 7, walls: bool = True) -> Scene:
+    rng = np.random.default_rng(seed)
+    a, b, c, p = 5e-3, 4e-3, 3e-3, 4.0
+    ee = en = 2.0 / p
+    shapes: list = []
+    sdf, org = superellipsoid_grid(a, b, c, ee, en, n_grid, spacing)
+    nodes = superellipsoid_nodes(a, b, c, ee, en, n_nodes)
+    sid = _add_shape(shapes, sdf, org, spacing, nodes, density)
+    S = shapes[sid]
+    box = np.asarray(box, float)
+    per_row = int(np.floor((box[0] - lattice) / lattice)) + 1
+    per_col = int(np.floor((box[1] - lattice) / lattice)) + 1
+    P = _particles(n)
+    k = np.arange(n)
+    ix, iy, iz = k % per_row, (k // per_row) % per_col, k // (per_row * per_col)
+    base = np.zeros((n, 3))
+    base[:, 0] = lattice * (ix + 0.5) + 0.5 * (box[0] - per_row * lattice)
+    base[:, 1] = lattice * (iy + 0.5) + 0.5 * (box[1] - per_col * lattice)
+    base[:, 2] = lattice * (iz + 0.5) + 1e-3
+    P["pos"] = base + rng.uniform(-jitter, jitter, size=(n, 3))
+    P["quat"] = random_quaternions(rng, n)
+    P["inv_mass"][:] = 1.0 / S["mass"]
+    P["inv_inertia"][:] = S["inv_inertia"]
+    P["kn"][:] = kn
+    P["ks"][:] = 0.4 * kn
+    P["mu"][:] = mu
+    P["e"][:] = restitution
+    margin = 0.01
+    cfg = {"domain_min": (-margin, -margin, -margin), "domain_max": tuple(box + margin),
+           "gravity": (0.0, 0.0, -gravity), "dt": dt, "quirks": quirks}
+    sc = Scene(cfg=cfg, shapes=shapes, particles=P, meta={"name": "C1", "seed": seed})
+    sc.walls = box_walls(shapes, (0, 0, 0), box, mu, restitution) if walls else _walls_dict(0)
+    return sc
+
+
+# ----------------------------------------------------------------------------------------------
+# Config 2: 200,000 superquadrics of 10 types, periodic x/y (0.3 m cell), floor wall
+# ----------------------------------------------------------------------------------------------
+def unit_superquadric_types(rng, n_types: int, n_grid: int, n_nodes: int):
+    """Shapes of unit volume-equivalent radius: exponents U[2,8], axis ratios (1, U[.5,1], U[.5,1])."""
+    out = []
+    for _ in range(n_types):
+        p1, p2 = rng.uniform(2.0, 8.0, 2)
+        ee, en = 2.0 / p1, 2.0 / p2
+        ax = np.array([1.0, rng.uniform(0.5, 1.0), rng.uniform(0.5, 1.0)])
+        # normalise to the volume of the unit sphere, volume from a fine point count
+        g = np.linspace(-1, 1, 121)
+        Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+        f, _ = superellipsoid_value_grad(np.stack([X * ax[0], Y * ax[1], Z * ax[2]], -1), *ax, ee, en)
+        vol = (f <= 0).mean() * 8.0 * np.prod(ax)
+        ax = ax * (4.0 / 3.0 * np.pi / vol) ** (1.0 / 3.0)
+        h = 2.0 * ax.max() / (n_grid - 5)
+        sdf, org = superellipsoid_grid(*ax, ee, en, n_grid, h)
+        nodes = superellipsoid_nodes(*ax, ee, en, n_nodes)
+        out.append((sdf, org, h, nodes, ax, (ee, en)))
+    return out
+
+
+def scene_c2(n: int = 200_000, seed: int = 20261204, n_types: int = 10, n_grid: int = 50, n_nodes: int = 512,
+             cell_xy: float = 0.3, median_r: float = 3e-3, sigma: float = 0.25, dt: float = 5e-6, kn: float = 1e5,
+             restitution: float = 0.5, mu: float = 0.5, density: float = 2500.0, gravity: float = 9.81,
+             gap: float = 0.02, quirks: int = 7) -> Scene:
+    """Random-lattice start: particles sorted by size fill square layers whose pitch is the
+    largest bounding diameter of the layer (plus `gap`, relative)."""
+    rng = np.random.default_rng(seed)
+    shapes: list = []
+    types = unit_superquadric_types(rng, n_types, n_grid, n_nodes)
+    rb = []
+    for sdf, org, h, nodes, ax, _ in types:
+        sid = _add_shape(shapes, sdf, org, h, nodes, density)
+        rb.append(np.linalg.norm(shapes[sid]["nodes"], axis=1).max())
+    rb = np.asarray(rb)
+    P = _particles(n)
+    P["shape"] = rng.integers(0, n_types, n).astype(np.int32)
+    P["scale"] = np.exp(np.log(median_r) + sigma * rng.standard_normal(n))
+    rbound = rb[P["shape"]] * P["scale"]
+    order = np.argsort(-rbound, kind="stable")
+    z = 0.0
+    k = 0
+    while k < n:
+        pitch = 2.0 * rbound[order[k]] * (1.0 + gap)
+        m = max(1, int(np.floor(cell_xy / pitch)))
+        take = order[k:k + m * m]
+        idx = np.arange(len(take))
+        slack = cell_xy / m - pitch
+        P["pos"][take, 0] = (idx % m + 0.5) * (cell_xy / m) + rng.uniform(-0.5, 0.5, len(take)) * slack
+        P["pos"][take, 1] = (idx // m + 0.5) * (cell_xy / m) + rng.uniform(-0.5, 0.5, len(take)) * slack
+        P["pos"][take, 2] = z + 0.5 * pitch
+        z += pitch
+        k += len(take)
+    top = z
+    P["quat"] = random_quaternions(rng, n)
+    for t in range(n_types):
+        sel = P["shape"] == t
+        S = shapes[t]
+        s = P["scale"][sel]
+        P["inv_mass"][sel] = 1.0 / (S["mass"] * s ** 3)
+        P["inv_inertia"][sel] = S["inv_inertia"][None, :] / (s ** 5)[:, None]
+    P["kn"][:] = kn
+    P["ks"][:] = 0.4 * kn
+    P["mu"][:] = mu
+    P["e"][:] = restitution
+    zmax = top + 0.05
+    # y range chosen so that max.y >= max.z: the reference tests ghost z against max.y
+    y0 = zmax - cell_xy
+    P["pos"][:, 1] += y0
+    cfg = {"domain_min": (0.0, y0, 0.0), "domain_max": (cell_xy, y0 + cell_xy, zmax), "gravity": (0.0, 0.0, -gravity),
+           "dt": dt, "periodic_x": 1, "periodic_y": 1, "quirks": quirks}
+    sc = Scene(cfg=cfg, shapes=shapes, particles=P, meta={"name": "C2", "seed": seed, "bed_top": top})
+    sc.walls = floor_wall(shapes, (0.5 * cell_xy, y0 + 0.5 * cell_xy), 0.0, 0.75 * cell_xy, mu, restitution)
+    return sc

@@ -1,0 +1,24 @@
+"""Loads the oracle (test infrastructure) through the same binding as the product library."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liblsdem_oracle.so"
+
+
+def oracle_lib() -> C.CDLL:
+    if not ORACLE_SO.exists():
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+    return C.CDLL(str(ORACLE_SO))
+
+
+def oracle_sim(cfg: dict, literal: bool = False):
+    import lsdem_amd.lsdem as L
+    lib = oracle_lib()
+    s = L.Sim(cfg, lib=lib, prefix="lsdem_oracle_")
+    if literal:
+        lib.lsdem_oracle_set_literal(s.ctx, C.c_int(1))
+    return s
