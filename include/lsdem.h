@@ -149,6 +149,23 @@ int lsdem_get_bonds(lsdem_ctx* ctx, int32_t* active, double* point, double* ener
 
 int lsdem_get_counters(lsdem_ctx* ctx, lsdem_counters* out);
 
+/* Per-phase timing of lsdem_step with HIP events recorded on the context's stream.  Phases:
+ * 0 integrate (half-kick + drift + walls), 1 broad-phase sort, 2 candidate pairs,
+ * 3 reaction lists, 4 narrow phase + contact law, 5 history table, 6 bonds,
+ * 7 reduction + second half-kick.  Work counters are summed over the profiled steps.
+ * No counterpart in the reference (it has no timers). */
+#define LSDEM_NUM_PHASES 8
+typedef struct lsdem_profile {
+  int64_t steps;               /* profiled steps */
+  double ms[LSDEM_NUM_PHASES]; /* summed GPU time per phase */
+  int64_t pairs, node_tests, sdf_queries, contacts; /* sums over the profiled steps */
+  int64_t reserved[8];
+} lsdem_profile;
+/* Start recording; at most max_steps steps are timed (events are preallocated). */
+int lsdem_profile_begin(lsdem_ctx* ctx, int32_t max_steps);
+/* Stop recording, wait for the work and return the totals. */
+int lsdem_profile_end(lsdem_ctx* ctx, lsdem_profile* out);
+
 /* Mass, centroid (local frame), inertia and inverse inertia from an SDF grid with a smoothed
  * Heaviside of half-width 1.5 cells; host computation used by the first addLSParticle
  * overload (LSSolver.h:51-163).  inertia6/inv_inertia6 may be NULL. */

@@ -27,7 +27,11 @@ template <bool WRITE> __global__ void k_pairs(Params P);
 __global__ void k_rc_count(Params P);
 __global__ void k_rc_scatter(Params P);
 __global__ void k_hash_insert(Params P);
+__global__ void k_hash_clear(Params P);
+__global__ void k_gather_particles(Params P);
+__global__ void k_gather_images(Params P, int set);
 __global__ void k_narrow(Params P);
+__global__ void k_contacts(Params P);
 __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P);
 __global__ void k_step_begin(Params P);
@@ -169,6 +173,7 @@ struct lsdem_ctx {
   hipStream_t stream = nullptr;
   bool initialized = false;
   int numCU = 256;
+  int narrowBlocks = 2048;
   // host staging
   std::vector<HostShape> shapes;
   std::vector<int> pShape;
@@ -189,6 +194,7 @@ struct lsdem_ctx {
   DBuf<ShapeDev> dShapes;
   DBuf<ClusterDev> dClusters;
   DBuf<NodeDev> dNodes;
+  DBuf<float4> dNodes32;
   DBuf<float> dSdf;
   DBuf<d3> dPos, dPosDet, dVel, dAng, dForce, dTorque, dGorig;
   DBuf<q4> dQuat;
@@ -202,15 +208,18 @@ struct lsdem_ctx {
   DBuf<d3> dGPos[3];
   DBuf<int> dGKey[3], dGCell[3], dGCellStart[3], dGSorted[3], dGCount[3], dGRank[3];
   DBuf<int> dCellCount, dCellStart, dSorted, dRank;
-  DBuf<int> dPairCount, dPairStart, dPairA, dPairFlag, dRcCount, dRcStart, dRcPair, dRcRank;
+  DBuf<d3> dSPos, dGSPos[3];
+  DBuf<double> dSRad, dGSBox[3];
+  DBuf<int> dPairCount, dPairStart, dPairA, dPairCoff, dPairCcnt, dRcCount, dRcStart, dRcPair, dRcRank;
   DBuf<uint32_t> dPairInfo;
-  DBuf<double> dPairSum;
-  DBuf<int> dCNode[2];
+  DBuf<int> dCNode[2], dCPair;
+  DBuf<d3> dCF, dCTA, dCTB;
   DBuf<uint32_t> dCInfo[2];
   DBuf<d3> dCSpring[2], dCPoint[2], dCNormal[2];
   DBuf<double> dCOverlap[2], dCEn[2], dCEs[2];
   DBuf<unsigned long long> dHashKey;
-  DBuf<int> dHashVal;
+  DBuf<int> dHashVal, dHashSlot;
+  DBuf<unsigned long long> dStatSlots;
   DBuf<int> dBI, dBJ, dBActive, dBcStart, dBcItem;
   DBuf<double> dBConst, dBOut;
   DBuf<d3> dBLocal, dBPoint, dBF, dBMi, dBMj;
@@ -221,13 +230,22 @@ struct lsdem_ctx {
   Params P{};
   long long stepCount = 0;
   int cur = 0;  // contact buffer written by the next step
+  // profiling: kPhaseEvents events per profiled step
+  std::vector<hipEvent_t> events;
+  int profMax = 0, profSteps = 0;
+  bool profiling = false;
 
-  ~lsdem_ctx() { releaseAll(); if (stream) (void)hipStreamDestroy(stream); }
+  ~lsdem_ctx() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    releaseAll();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
   void releaseAll();
 };
 
 void lsdem_ctx::releaseAll() {
-  dShapes.release(); dClusters.release(); dNodes.release(); dSdf.release();
+  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dSdf.release();
+  dCPair.release(); dCF.release(); dCTA.release(); dCTB.release();
   dPos.release(); dPosDet.release(); dVel.release(); dAng.release(); dForce.release(); dTorque.release();
   dGorig.release(); dQuat.release(); dInvI.release();
   dInvM.release(); dRadius.release(); dKn.release(); dKs.release(); dMu.release(); dRes.release();
@@ -239,9 +257,11 @@ void lsdem_ctx::releaseAll() {
     dGSorted[s].release(); dGCount[s].release(); dGRank[s].release();
   }
   dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
-  dPairCount.release(); dPairStart.release(); dPairA.release(); dPairFlag.release();
+  dSPos.release(); dSRad.release(); dHashSlot.release(); dStatSlots.release();
+  for (int g = 0; g < 3; ++g) { dGSPos[g].release(); dGSBox[g].release(); }
+  dPairCount.release(); dPairStart.release(); dPairA.release(); dPairCoff.release(); dPairCcnt.release();
   dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release();
-  dPairInfo.release(); dPairSum.release();
+  dPairInfo.release();
   for (int b = 0; b < 2; ++b) {
     dCNode[b].release(); dCInfo[b].release(); dCSpring[b].release(); dCPoint[b].release();
     dCNormal[b].release(); dCOverlap[b].release(); dCEn[b].release(); dCEs[b].release();
@@ -321,26 +341,41 @@ void setCurrentBuffers(lsdem_ctx* c) {
   P.prevSpring = c->dCSpring[o].p;
 }
 
+constexpr int kPhaseEvents = LSDEM_NUM_PHASES + 1;
+
 int runStep(lsdem_ctx* c) {
   Params& P = c->P;
   setCurrentBuffers(c);
   const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
   hipStream_t s = c->stream;
+  const bool prof = c->profiling && c->profSteps < c->profMax;
+  hipEvent_t* ev = prof ? &c->events[size_t(c->profSteps) * kPhaseEvents] : nullptr;
+  int mark = 0;
+  auto phase = [&]() {
+    if (prof) (void)hipEventRecord(ev[mark], s);
+    ++mark;
+  };
+  phase();
   hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, P);
   hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
   if (P.nW > 0) hipLaunchKernelGGL(k_walls, dim3(blocksFor(P.nW, bs)), dim3(bs), 0, s, P);
+  phase();
   int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
   for (int g = 0; g < 3; ++g) {
     const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
     if (!on) continue;
     rc = bucketSort(c, P.gcell[g], P.nP, P.sg.ncell, P.gCount[g], P.gCellStart[g], c->dGRank[g].p, P.gSorted[g]);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_gather_images, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P, g);
   }
+  phase();
   hipLaunchKernelGGL(k_pairs<false>, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
   rc = scan(c, P.pairCount, P.nP, P.pairStart);
   if (rc) return rc;
   hipLaunchKernelGGL(k_pairs<true>, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  phase();
   // reaction CSR
   hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P.rcCount, (long long)P.nP, 0);
   const unsigned gsPairs = unsigned(c->numCU * 8);
@@ -350,15 +385,21 @@ int runStep(lsdem_ctx* c) {
   hipLaunchKernelGGL(k_rc_scatter, dim3(gsPairs), dim3(256), 0, s, P);
   hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, (const int*)P.rcStart, P.nP,
                      P.rcPair);
+  phase();
   // narrow phase + contact law
-  hipLaunchKernelGGL(k_narrow, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_narrow, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_contacts, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  phase();
   // history of this step's contacts for the next step
-  hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P.hashKey, c->hashSize,
-                     kHashEmpty);
+  hipLaunchKernelGGL(k_hash_clear, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
   hipLaunchKernelGGL(k_hash_insert, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
+  phase();
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
+  phase();
   hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, P);
+  phase();
+  if (prof) c->profSteps++;
   c->cur = 1 - c->cur;
   c->stepCount++;
   return LSDEM_OK;
@@ -590,6 +631,13 @@ int lsdem_initialize(lsdem_ctx* c) {
   if ((rc = upload(c->dShapes, sd, s))) return rc;
   if ((rc = upload(c->dClusters, cls, s))) return rc;
   if ((rc = upload(c->dNodes, nds, s))) return rc;
+  {
+    std::vector<float4> n32(nds.size());
+    for (size_t k = 0; k < nds.size(); ++k) n32[k] = make_float4(float(nds[k].x), float(nds[k].y), float(nds[k].z), 0.f);
+    if ((rc = upload(c->dNodes32, n32, s))) return rc;
+  }
+  int maxClusters = 1;
+  for (int k = 0; k < nS; ++k) maxClusters = std::max(maxClusters, sd[k].nClusters);
   if ((rc = upload(c->dSdf, sdf, s))) return rc;
 
   // ---- particles ----
@@ -666,6 +714,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dCellStart.alloc(ncell + 1));
   HIPCHK(c->dSorted.alloc(std::max(nP, 1)));
   HIPCHK(c->dRank.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSPos.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSRad.alloc(std::max(nP, 1)));
   const bool periodic = P.periodicX || P.periodicY;
   for (int g = 0; g < 3; ++g) {
     const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
@@ -677,11 +727,17 @@ int lsdem_initialize(lsdem_ctx* c) {
     HIPCHK(c->dGRank[g].alloc(n));
     HIPCHK(c->dGCount[g].alloc(on ? ncell : 1));
     HIPCHK(c->dGCellStart[g].alloc(on ? ncell + 1 : 1));
+    HIPCHK(c->dGSPos[g].alloc(n));
+    HIPCHK(c->dGSBox[g].alloc(n));
   }
 
   // ---- pairs and contacts ----
   long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);
-  long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity : 64ll * nP + 16384;
+  // narrow-phase grid: enough waves for ~16 candidate pairs per particle, at most 8 per SIMD
+  c->narrowBlocks = int(std::min<long long>(c->numCU * 8ll, std::max<long long>(1, (16ll * nP + 8 * 4 - 1) / (8 * 4))));
+  // each wave may leave one partly used slot chunk (512) behind
+  long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity
+                                                     : 64ll * nP + 16384 + 512ll * 4 * c->narrowBlocks;
   pairCap = std::min<long long>(pairCap, (1ll << 31) - 1);
   contactCap = std::min<long long>(contactCap, (1ll << 30));
   P.pairCap = pairCap;
@@ -690,12 +746,16 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairStart.alloc(nP + 1));
   HIPCHK(c->dPairA.alloc(pairCap));
   HIPCHK(c->dPairInfo.alloc(pairCap));
-  HIPCHK(c->dPairFlag.alloc(pairCap));
-  HIPCHK(c->dPairSum.alloc(9 * pairCap));
+  HIPCHK(c->dPairCoff.alloc(pairCap));
+  HIPCHK(c->dPairCcnt.alloc(pairCap));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
   HIPCHK(c->dRcRank.alloc(pairCap));
+  HIPCHK(c->dCPair.alloc(contactCap));
+  HIPCHK(c->dCF.alloc(contactCap));
+  HIPCHK(c->dCTA.alloc(contactCap));
+  HIPCHK(c->dCTB.alloc(contactCap));
   for (int b = 0; b < 2; ++b) {
     HIPCHK(c->dCNode[b].alloc(contactCap));
     HIPCHK(c->dCInfo[b].alloc(contactCap));
@@ -711,6 +771,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   c->hashSize = hs;
   HIPCHK(c->dHashKey.alloc(hs));
   HIPCHK(c->dHashVal.alloc(hs));
+  HIPCHK(c->dHashSlot.alloc(contactCap));
+  HIPCHK(c->dStatSlots.alloc(3 * kStatSlots));
   P.hashMask = (unsigned long long)(hs - 1);
 
   // ---- bonds (+ per-particle CSR of bond ends, in bond order) ----
@@ -743,9 +805,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
   HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
   HIPCHK(c->dDevInt.alloc(8));
-  HIPCHK(c->dDevStat.alloc(4));
+  HIPCHK(c->dDevStat.alloc(8));
+  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 3 * kStatSlots * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, 8 * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 4 * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 8 * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
@@ -753,6 +816,8 @@ int lsdem_initialize(lsdem_ctx* c) {
 
   // ---- pointer table ----
   P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
+  P.nodes32 = c->dNodes32.p; P.maxClusters = maxClusters;
+  P.cPair = c->dCPair.p; P.cF = c->dCF.p; P.cTA = c->dCTA.p; P.cTB = c->dCTB.p;
   P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;
   P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
   P.radius = c->dRadius.p; P.kn = c->dKn.p; P.ks = c->dKs.p; P.mu = c->dMu.p; P.res = c->dRes.p;
@@ -765,8 +830,11 @@ int lsdem_initialize(lsdem_ctx* c) {
     P.gCellStart[g] = c->dGCellStart[g].p; P.gSorted[g] = c->dGSorted[g].p; P.gCount[g] = c->dGCount[g].p;
   }
   P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
+  P.sPos = c->dSPos.p; P.sRad = c->dSRad.p;
+  for (int g = 0; g < 3; ++g) { P.gsPos[g] = c->dGSPos[g].p; P.gsBox[g] = c->dGSBox[g].p; }
+  P.hashSlot = c->dHashSlot.p; P.statSlots = c->dStatSlots.p;
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
-  P.pairInfo = c->dPairInfo.p; P.pairFlag = c->dPairFlag.p; P.pairSum = c->dPairSum.p;
+  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p;
   P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
   P.hashKey = c->dHashKey.p; P.hashVal = c->dHashVal.p;
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
@@ -944,15 +1012,58 @@ int lsdem_get_counters(lsdem_ctx* c, lsdem_counters* out) {
   HIPCHK(hipSetDevice(c->cfg.device));
   HIPCHK(hipStreamSynchronize(c->stream));
   int di[8];
-  unsigned long long ds[4];
+  unsigned long long ds[8];
   int np = 0;
   HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(&np, c->dPairStart.p + c->P.nP, sizeof(int), hipMemcpyDeviceToHost));
   out->pairs = np;
-  out->contacts = di[1];
+  out->contacts = (int64_t)ds[2];
   out->node_tests = (int64_t)ds[0];
   out->sdf_queries = (int64_t)ds[1];
+  return LSDEM_OK;
+}
+
+int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  if (max_steps < 0) return fail(LSDEM_ERR_ARG, "negative max_steps");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const size_t need = size_t(max_steps) * kPhaseEvents;
+  while (c->events.size() < need) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->events.push_back(e);
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemset(c->dDevStat.p + 3, 0, 4 * sizeof(unsigned long long)));
+  c->profMax = max_steps;
+  c->profSteps = 0;
+  c->profiling = true;
+  return LSDEM_OK;
+}
+
+int lsdem_profile_end(lsdem_ctx* c, lsdem_profile* out) {
+  if (!c || !out) return fail(LSDEM_ERR_ARG, "null argument");
+  if (!c->profiling) return fail(LSDEM_ERR_STATE, "profiling not started");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->steps = c->profSteps;
+  for (int st = 0; st < c->profSteps; ++st) {
+    hipEvent_t* ev = &c->events[size_t(st) * kPhaseEvents];
+    for (int ph = 0; ph < LSDEM_NUM_PHASES; ++ph) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[ph], ev[ph + 1]));
+      out->ms[ph] += ms;
+    }
+  }
+  unsigned long long ds[8];
+  HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
+  out->pairs = (int64_t)ds[3];
+  out->node_tests = (int64_t)ds[4];
+  out->sdf_queries = (int64_t)ds[5];
+  out->contacts = (int64_t)ds[6];
+  c->profiling = false;
   return LSDEM_OK;
 }
 

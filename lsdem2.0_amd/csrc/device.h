@@ -57,7 +57,9 @@ struct Params {
   const ShapeDev* shapes;
   const ClusterDev* clusters;
   const NodeDev* nodes;
+  const float4* nodes32;  // same nodes in FP32 (x, y, z, 0) for the conservative prefilter
   const float* sdf;
+  int maxClusters;        // largest cluster count of any shape
 
   // ---- particles ----
   d3* pos;      // state position (after any periodic wrap)
@@ -103,16 +105,21 @@ struct Params {
   // ---- search grid of particles ----
   int* cellCount;   // ncell (scratch)
   int* cellStart;   // ncell + 1
-  int* sorted;      // nP
+  int* sorted;      // nP: particle ids in (cell, id) order
   int* rank;        // nP (scratch)
+  // particle data gathered into sorted order (contiguous runs per search-grid row)
+  d3* sPos;         // posDet
+  double* sRad;     // bounding radius, negative when the particle has no reference key
+  d3* gsPos[3];     // ghost images of each set in their sorted order
+  double* gsBox[3]; // scaled grid-box radius of the image's source particle
 
   // ---- candidate pairs (current step) ----
   int* pairCount;   // nP (+1) counts then exclusive prefix pairStart
   int* pairStart;   // nP + 1
   int* pairA;       // pairCap
   uint32_t* pairInfo;  // (type << 28) | partner
-  int* pairFlag;    // contacts in the pair (0 = no sums written)
-  double* pairSum;  // 9 per pair: F (A side), torque on A, torque on partner
+  int* pairCoff;    // first contact slot of the pair (its contacts are contiguous)
+  int* pairCcnt;    // contacts of the pair
   // reaction CSR (pairs whose partner is a particle that receives the reaction)
   int* rcCount;     // nP
   int* rcStart;     // nP + 1
@@ -121,7 +128,11 @@ struct Params {
 
   // ---- contacts (current step) ----
   int* cNode;
-  uint32_t* cInfo;
+  uint32_t* cInfo;      // (type << 28) | partner, kInvalidInfo for an unused slot
+  int* cPair;           // candidate pair of the contact
+  d3* cF;               // force on the master particle
+  d3* cTA;              // torque on the master particle
+  d3* cTB;              // torque on the partner
   d3* cSpring;
   d3* cPoint;
   d3* cNormal;
@@ -132,6 +143,7 @@ struct Params {
   const d3* prevSpring;
   unsigned long long* hashKey;
   int* hashVal;
+  int* hashSlot;    // slot taken by each inserted contact (cleared at the next insert)
 
   // ---- bonds ----
   const int* bI;
@@ -148,8 +160,16 @@ struct Params {
   const int* bcItem;     // 2*nBonds: (bond << 1) | (0 = i end, 1 = j end)
 
   // ---- device counters ----
-  int* devInt;                 // [0] nPairs, [1] nContacts, [2] work counter, [3] overflow flags
-  unsigned long long* devStat; // [0] node tests, [1] sdf queries
+  // devInt: [1] contact slots allocated this step, [3] overflow flags (1 pairs, 2 contacts),
+  //         [4] peak pairs, [5] peak contact slots, [6] contacts inserted in the history table
+  int* devInt;
+  // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
+  //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts (summed while profiling)
+  unsigned long long* devStat;
+  unsigned long long* statSlots;  // kStatSlots x 3 spread accumulators of the narrow phase
 };
+
+constexpr int kStatSlots = 32;
+constexpr uint32_t kInvalidInfo = 0xFFFFFFFFu;  // unused contact slot
 
 }  // namespace lsd

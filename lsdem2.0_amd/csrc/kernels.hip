@@ -6,11 +6,13 @@
 //   counting sort     particles (and ghost images) by search-grid cell, deterministic order
 //   k_pairs (x2)      bounding-sphere / grid-box candidate pairs per master particle (count, write)
 //   reaction CSR      pairs grouped by the partner that receives the reaction force
-//   k_narrow          one wavefront per candidate pair: cluster cull, node-vs-SDF, contact law,
-//                     tangential history, per-pair force/torque sums
+//   k_narrow          node-vs-SDF detection, one wavefront per batch of 8 candidate pairs:
+//                     conservative cluster cull, FP32 node prefilter, exact FP64 test, contacts
+//                     of a pair written to a contiguous range
+//   k_contacts        one thread per contact: tangential history + linear spring-dashpot
 //   k_hash_*          history table of this step's contacts for the next step
 //   k_bonds           parallel bonds
-//   k_reduce_kick     per-body segmented sum of pair and bond contributions + second half-kick
+//   k_reduce_kick     per-body segmented sum of contact and bond contributions + second half-kick
 #include "device.h"
 
 namespace lsd {
@@ -244,10 +246,32 @@ __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, con
   const int b = bucket[i];
   if (b >= 0) out[start[b] + rank[i]] = i;
 }
+// Buckets of up to 16 items are sorted in registers (odd-even transposition network),
+// larger ones by insertion sort in memory.
 __global__ void k_bucket_sort(const int* start, int nb, int* items) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int s = start[b], e = start[b + 1];
+  const int n = e - s;
+  if (n <= 1) return;
+  if (n <= 16) {
+    int v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = k < n ? items[s + k] : 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+#pragma unroll
+      for (int k = (r & 1); k + 1 < 16; k += 2) {
+        const int a = v[k], c = v[k + 1];
+        v[k] = min(a, c);
+        v[k + 1] = max(a, c);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < n) items[s + k] = v[k];
+    return;
+  }
   for (int k = s + 1; k < e; ++k) {
     const int v = items[k];
     int j = k - 1;
@@ -259,6 +283,24 @@ __global__ void k_bucket_sort(const int* start, int nb, int* items) {
   }
 }
 
+// Gather particle (or ghost-image) data into the sorted order of the search-grid table so
+// that the candidate loops stream contiguous memory.
+__global__ void k_gather_particles(Params P) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P.nP) return;
+  const int i = P.sorted[k];
+  P.sPos[k] = P.posDet[i];
+  P.sRad[k] = P.keyRef[i] >= 0 ? P.radius[i] : -1.0;
+}
+__global__ void k_gather_images(Params P, int set) {
+  const int n = P.gCellStart[set][P.sg.ncell];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int j = P.gSorted[set][k];
+    P.gsPos[set][k] = P.gpos[set][j];
+    P.gsBox[set][k] = P.scale[j] * P.shapes[P.shape[j]].boxR;
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // Candidate pairs
 // ------------------------------------------------------------------------------------------
@@ -266,8 +308,7 @@ __global__ void k_bucket_sort(const int* start, int nb, int* items) {
 // of x.  Rows (y, z) are pruned by their distance to x; the x-run of cells of a row is
 // contiguous in the sorted order, so each row costs one range lookup.
 template <class F>
-__device__ __forceinline__ void visitNear(const SearchGrid& S, const int* cellStart, const int* sorted, d3 x,
-                                          double reach, F&& f) {
+__device__ __forceinline__ void visitNear(const SearchGrid& S, const int* cellStart, d3 x, double reach, F&& f) {
   const i3 c = searchCell(S, x);
   const int R = max(1, int(ceil(reach * S.invCell)));
   const double r2 = reach * reach;
@@ -287,7 +328,7 @@ __device__ __forceinline__ void visitNear(const SearchGrid& S, const int* cellSt
       const int x1 = clampCell(floor((x.x + hw - S.mn.x) * S.invCell), S.size.x);
       const int row = (cz * S.size.y + cy) * S.size.x;
       const int s = cellStart[row + x0], e = cellStart[row + x1 + 1];
-      for (int k = s; k < e; ++k) f(sorted[k]);
+      for (int k = s; k < e; ++k) f(k);
     }
   }
 }
@@ -324,8 +365,9 @@ __device__ __forceinline__ double boxDist2(d3 loc, d3 o, double h, const ShapeDe
 
 template <bool WRITE>
 __global__ void k_pairs(Params P) {
-  const int A = blockIdx.x * blockDim.x + threadIdx.x;
-  if (A >= P.nP) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.nP) return;
+  const int A = P.sorted[t];  // masters in spatial order: neighbouring lanes share cells
   int cnt = 0;
   long long out = WRITE ? P.pairStart[A] : 0;
   auto emit = [&](uint32_t type, int partner) {
@@ -341,12 +383,16 @@ __global__ void k_pairs(Params P) {
   const double rA = P.radius[A];
   // particle-particle: B > A, B inside the domain, bounding spheres overlap
   {
-    const d3 xA = P.posDet[A];
+    const d3 xA = P.sPos[t];
     const double reach = (rA + P.rMax) * (1.0 + 2e-9) + 2e-12;
-    visitNear(P.sg, P.cellStart, P.sorted, xA, reach, [&](int B) {
-      if (B <= A || P.keyRef[B] < 0) return;
-      const double rr = (rA + P.radius[B]) * (1.0 + 1e-9) + 1e-12;
-      if (len2(xA - P.posDet[B]) > rr * rr) return;
+    visitNear(P.sg, P.cellStart, xA, reach, [&](int k) {
+      const double rB = P.sRad[k];
+      if (rB < 0.0) return;
+      const d3 d = xA - P.sPos[k];
+      const double rr = (rA + rB) * (1.0 + 1e-9) + 1e-12;
+      if (len2(d) > rr * rr) return;
+      const int B = P.sorted[k];
+      if (B <= A) return;
       emit(PT_PP, B);
     });
   }
@@ -374,12 +420,13 @@ __global__ void k_pairs(Params P) {
     for (int s = 0; s < 3; ++s) {
       const bool on = (s == 0 && P.periodicX) || (s == 1 && P.periodicY) || (s == 2 && P.periodicX && P.periodicY);
       if (!on) continue;
-      const d3* gp = P.gpos[s];
-      visitNear(P.sg, P.gCellStart[s], P.gSorted[s], xA, reach, [&](int j) {
-        const double bj = P.scale[j] * P.shapes[P.shape[j]].boxR;
-        const double rr = (rA + bj) * (1.0 + 1e-9) + 1e-12;
-        if (len2(xA - gp[j]) > rr * rr) return;
-        emit(PT_GX + s, j);
+      const d3* gp = P.gsPos[s];
+      const double* gb = P.gsBox[s];
+      const int* gs = P.gSorted[s];
+      visitNear(P.sg, P.gCellStart[s], xA, reach, [&](int k) {
+        const double rr = (rA + gb[k]) * (1.0 + 1e-9) + 1e-12;
+        if (len2(xA - gp[k]) > rr * rr) return;
+        emit(PT_GX + s, gs[k]);
       });
     }
   }
@@ -422,22 +469,37 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
   x ^= x >> 31;
   return x;
 }
+// Remove the previous step's entries (their slots were recorded at insertion).
+__global__ void k_hash_clear(Params P) {
+  const int n = P.devInt[6];
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int slot = P.hashSlot[c];
+    if (slot >= 0) P.hashKey[slot] = kHashEmpty;
+  }
+}
 __global__ void k_hash_insert(Params P) {
   const int n = min(P.devInt[1], (int)P.contactCap);
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     const uint32_t info = P.cInfo[c];
-    const unsigned long long key = histKey(info >> 28, P.cNode[c], int(info & kPartnerMask));
-    unsigned long long slot = mix64(key) & P.hashMask;
-    for (;;) {
-      const unsigned long long old = atomicCAS(&P.hashKey[slot], kHashEmpty, key);
-      if (old == kHashEmpty || old == key) {
-        P.hashVal[slot] = c;
-        break;
+    int slotOut = -1;
+    if (info != kInvalidInfo) {
+      const unsigned long long key = histKey(info >> 28, P.cNode[c], int(info & kPartnerMask));
+      unsigned long long slot = mix64(key) & P.hashMask;
+      for (;;) {
+        const unsigned long long old = atomicCAS(&P.hashKey[slot], kHashEmpty, key);
+        if (old == kHashEmpty || old == key) {
+          P.hashVal[slot] = c;
+          break;
+        }
+        slot = (slot + 1) & P.hashMask;
       }
-      slot = (slot + 1) & P.hashMask;
+      slotOut = int(slot);
     }
+    P.hashSlot[c] = slotOut;  // -1 for unused contact slots
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[6] = n;
 }
+
 __device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key) {
   unsigned long long slot = mix64(key) & P.hashMask;
   for (;;) {
@@ -449,19 +511,9 @@ __device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key
 }
 
 // ------------------------------------------------------------------------------------------
-// Narrow phase + contact law, one wavefront per candidate pair
+// Narrow phase: node-vs-SDF detection, one wavefront per batch of 8 candidate pairs
 // ------------------------------------------------------------------------------------------
-struct Partner {
-  d3 x;       // pose used for detection
-  q4 q;
-  d3 xf;      // position used by the force law (torque arm)
-  d3 v, w;
-  double invM, kn, ks, mu, res;
-  double r;   // particle bounding radius (particle-particle cull)
-  double sc, invSp;
-  d3 go;
-  int shape, key;
-};
+constexpr int kNarrowBlock = 256;
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -469,195 +521,317 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-constexpr int kNarrowBlock = 256;
+// Detection-side view of a pair (everything the node tests need).
+struct PairView {
+  int A, partner, nodeBase, valid;
+  uint32_t type, info;
+  d3 xA;        // master position used for detection
+  q4 qA;
+  double sA;
+  int clusterOff, nClusters, nodeOff, shapeB;
+  d3 xB;        // partner pose used for detection
+  q4 qB;
+  double rB;    // particle-particle radius cull (0 for fixed partners)
+  double scB, invSpB;
+  d3 goB;
+  int keyB;
+};
+
+__device__ __forceinline__ PairView loadPair(const Params& P, int p) {
+  PairView v;
+  v.info = P.pairInfo[p];
+  v.type = v.info >> 28;
+  v.partner = int(v.info & kPartnerMask);
+  v.A = P.pairA[p];
+  const bool ghost = v.type >= PT_GX;
+  v.xA = ghost ? P.pos[v.A] : P.posDet[v.A];
+  v.qA = P.quat[v.A];
+  v.sA = P.scale[v.A];
+  const ShapeDev SA = P.shapes[P.shape[v.A]];
+  v.clusterOff = SA.clusterOff;
+  v.nClusters = SA.nClusters;
+  v.nodeOff = SA.nodeOff;
+  v.nodeBase = P.nodeStart[v.A];
+  const int j = v.partner;
+  if (v.type == PT_PW) {
+    v.xB = P.wpos[j]; v.qB = P.wquat[j]; v.rB = 0.; v.shapeB = P.wshape[j]; v.scB = 1.0;
+    const ShapeDev SB = P.shapes[v.shapeB];
+    v.invSpB = 1. / SB.h; v.goB = mk3(SB.ox, SB.oy, SB.oz); v.keyB = P.wkey[j];
+  } else {
+    if (v.type == PT_PP) { v.xB = P.posDet[j]; v.keyB = P.keyRef[j]; }
+    else { const int g = int(v.type - PT_GX); v.xB = P.gpos[g][j]; v.keyB = P.gkey[g][j]; }
+    v.qB = P.quat[j]; v.rB = P.radius[j]; v.shapeB = P.shape[j]; v.scB = P.scale[j];
+    v.invSpB = P.invSp[j]; v.goB = P.gorig[j];
+  }
+  v.valid = 1;
+  return v;
+}
+
+// Conservative cluster test: can any node of cluster `c` of A meet the partner?
+__device__ __forceinline__ bool clusterMayHit(const Params& P, const PairView& v, int c) {
+  const ClusterDev cl = P.clusters[v.clusterOff + c];
+  const d3 cw = rot(v.qA, mk3(cl.cx, cl.cy, cl.cz) * v.sA) + v.xA;  // cluster centre, world
+  const double rho = cl.rho * v.sA * (1.0 + 1e-9) + 1e-12;
+  if (v.type == PT_PP) {
+    const double rr = rho + v.rB * (1.0 + 1e-9) + 1e-12;
+    return len2(cw - v.xB) <= rr * rr;
+  }
+  const ShapeDev SB = P.shapes[v.shapeB];
+  const d3 loc = rotInv(cw - v.xB, v.qB);
+  if (boxDist2(loc, v.goB, SB.h * v.scB, SB) > rho * rho) return false;
+  if (v.type == PT_PW) {
+    bool inside;
+    const double phi = triAt(P, SB, 1.0, loc, v.goB, v.invSpB, inside);
+    if (inside && phi > SB.lip * rho * (1.0 + 1e-6) + 1e-9 * SB.h) return false;
+  }
+  return true;
+}
+
+constexpr int kContactChunk = 512;  // contact slots reserved per atomic
+constexpr int kBatch = 8;           // candidate pairs per wave batch
+constexpr int kMaxClusterIters = 8; // 8 x 8 = 64 clusters per shape at most
+
+struct NodeHit {
+  d3 point, normal;
+  double ov;
+  int gnode;
+};
+
+// Exact node-vs-SDF test of node `lane` of cluster `cl` of the pair (the reference's membership:
+// radius cull or cell window, grid bounds, trilinear overlap >= 0).
+__device__ __forceinline__ bool nodeTest(const Params& P, const PairView& v, const ClusterDev& cl, int lane,
+                                         float cbx, float cby, float cbz, float cr2, NodeHit& h,
+                                         unsigned long long& tests, unsigned long long& queries) {
+  if (lane >= cl.count) return false;
+  const int ni = v.nodeOff + cl.first + lane;
+  if (v.type == PT_PP) {
+    const float4 n32 = P.nodes32[ni];
+    const float dx = n32.x - cbx, dy = n32.y - cby, dz = n32.z - cbz;
+    if (dx * dx + dy * dy + dz * dz > cr2) return false;
+  }
+  const NodeDev nd = P.nodes[ni];
+  h.gnode = v.nodeBase + nd.orig;
+  const d3 xw = rot(v.qA, mk3(nd.x * v.sA, nd.y * v.sA, nd.z * v.sA)) + v.xA;
+  tests++;
+  if (v.type == PT_PP) {
+    if (len2(xw - v.xB) > v.rB * v.rB) return false;
+  } else {
+    if (!keyInWindow(v.type == PT_PW ? P.gridW : P.gridP, v.keyB, xw)) return false;
+  }
+  const ShapeDev SB = P.shapes[v.shapeB];
+  const i3 nB{SB.nx, SB.ny, SB.nz};
+  const d3 loc = rotInv(xw - v.xB, v.qB);
+  const double gx = (loc.x - v.goB.x) * v.invSpB, gy = (loc.y - v.goB.y) * v.invSpB,
+               gz = (loc.z - v.goB.z) * v.invSpB;
+  const int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+  if (i0 < 0 || j0 < 0 || k0 < 0 || i0 >= nB.x - 1 || j0 >= nB.y - 1 || k0 >= nB.z - 1) return false;
+  const int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+  const double x = gx - double(i0), y = gy - double(j0), z = gz - double(k0);
+  const float* g = P.sdf + SB.sdfOff;
+  double cc[8];
+  cc[0] = double(g[lin3(i3{i0, j0, k0}, nB)]) * v.scB;
+  cc[1] = double(g[lin3(i3{i1, j0, k0}, nB)]) * v.scB;
+  cc[2] = double(g[lin3(i3{i0, j1, k0}, nB)]) * v.scB;
+  cc[3] = double(g[lin3(i3{i1, j1, k0}, nB)]) * v.scB;
+  cc[4] = double(g[lin3(i3{i0, j0, k1}, nB)]) * v.scB;
+  cc[5] = double(g[lin3(i3{i1, j0, k1}, nB)]) * v.scB;
+  cc[6] = double(g[lin3(i3{i0, j1, k1}, nB)]) * v.scB;
+  cc[7] = double(g[lin3(i3{i1, j1, k1}, nB)]) * v.scB;
+  queries++;
+  h.ov = -triValue(x, y, z, cc);
+  if (!(h.ov >= 0.)) return false;
+  d3 n = triGradient((P.quirks & 1) ? double(int(x)) : x, y, z, cc);
+  n = rot(v.qB, n);
+  n = normalize(n);
+  h.normal = n;
+  h.point = xw + 0.5 * h.ov * n;
+  return true;
+}
 
 __global__ __launch_bounds__(kNarrowBlock) void k_narrow(Params P) {
-  const int lane = threadIdx.x & 63;
+  // per wave: clusters of the current pair that produced contacts, with their hit masks
+  __shared__ unsigned long long sMask[kNarrowBlock / 64][64];
+  __shared__ int sClus[kNarrowBlock / 64][64];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long statTests = 0, statQueries = 0;
+  unsigned long long statTests = 0, statQueries = 0, statContacts = 0;
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
-  for (;;) {
-    int p = 0;
-    if (lane == 0) p = atomicAdd(&P.devInt[2], 1);
-    p = __builtin_amdgcn_readfirstlane(__shfl(p, 0, 64));
-    if (p >= nPairs) break;
-
-    const uint32_t info = P.pairInfo[p];
-    const uint32_t type = info >> 28;
-    const int partner = int(info & kPartnerMask);
-    const int A = P.pairA[p];
-    const bool ghostPair = type >= PT_GX;
-    // master particle
-    const d3 xA = ghostPair ? P.pos[A] : P.posDet[A];
-    const q4 qA = P.quat[A];
-    const double sA = P.scale[A];
-    const ShapeDev SA = P.shapes[P.shape[A]];
-    const int nodeBase = P.nodeStart[A];
-    const d3 riA = P.pos[A];
-    const d3 vA = P.vel[A], wA = P.angvel[A];
-    const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A], muA = P.mu[A], resA = P.res[A];
-    // partner
-    Partner B;
-    if (type == PT_PW) {
-      B.x = P.wpos[partner]; B.q = P.wquat[partner]; B.xf = B.x;
-      B.v = P.wvel[partner]; B.w = P.wangvel[partner];
-      B.invM = 0.; B.kn = 0.; B.ks = 0.; B.mu = P.wmu[partner]; B.res = P.wres[partner];
-      B.r = 0.; B.shape = P.wshape[partner]; B.sc = 1.0;
-      const ShapeDev SB = P.shapes[B.shape];
-      B.invSp = 1. / SB.h; B.go = mk3(SB.ox, SB.oy, SB.oz);
-      B.key = P.wkey[partner];
-    } else {
-      const int j = partner;
-      if (type == PT_PP) { B.x = P.posDet[j]; B.xf = P.pos[j]; B.v = P.vel[j]; B.key = P.keyRef[j]; }
-      else { const int s = int(type - PT_GX); B.x = P.gpos[s][j]; B.xf = B.x; B.v = P.vel[j]; B.key = P.gkey[s][j]; }
-      B.q = P.quat[j]; B.w = P.angvel[j];
-      B.invM = P.invM[j]; B.kn = P.kn[j]; B.ks = P.ks[j]; B.mu = P.mu[j]; B.res = P.res[j];
-      B.r = P.radius[j]; B.shape = P.shape[j]; B.sc = P.scale[j]; B.invSp = P.invSp[j]; B.go = P.gorig[j];
-    }
-    const ShapeDev SB = P.shapes[B.shape];
-    // pair-level material constants
-    double kn = 0., ks = 0., effM = 0., mu, res;
-    if (type == PT_PW) {
-      kn = knA; ks = ksA;
-      if (invMA > 0.) effM = 1. / invMA;
-    } else {
-      if (knA > 0. && B.kn > 0.) kn = knA * B.kn / (knA + B.kn);
-      if (ksA > 0. && B.ks > 0.) ks = ksA * B.ks / (ksA + B.ks);
-      if (invMA > 0. || B.invM > 0.) effM = 1. / (invMA + B.invM);
-    }
-    res = fmin(resA, B.res);
-    mu = fmin(muA, B.mu);
-
-    // cluster cull (conservative): lane k tests cluster k
-    unsigned long long cmask;
+  const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
+  const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
+  const int iters = (P.maxClusters + 7) >> 3;
+  int chunkNext = 0, chunkEnd = 0;
+  bool overflow = false;
+  for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
+    // ---- stage 1: lanes (j, k) test cluster it*8+k of pair base+j ----
+    unsigned long long M[kMaxClusterIters];
     {
-      bool keep = false;
-      if (lane < SA.nClusters) {
-        const ClusterDev cl = P.clusters[SA.clusterOff + lane];
-        const d3 cw = rot(qA, mk3(cl.cx, cl.cy, cl.cz) * sA) + xA;  // cluster centre, world
-        const double rho = cl.rho * sA * (1.0 + 1e-9) + 1e-12;
-        if (type == PT_PP) {
-          const double rr = rho + B.r * (1.0 + 1e-9) + 1e-12;
-          keep = len2(cw - B.x) <= rr * rr;
-        } else {
-          const d3 loc = rotInv(cw - B.x, B.q);
-          keep = boxDist2(loc, B.go, SB.h * B.sc, SB) <= rho * rho;
-          if (keep && type == PT_PW) {
-            bool inside;
-            const double phi = triAt(P, SB, 1.0, loc, B.go, B.invSp, inside);
-            if (inside && phi > SB.lip * rho * (1.0 + 1e-6) + 1e-9 * SB.h) keep = false;
-          }
+      const int j = lane >> 3, k = lane & 7;
+      const int p = base + j;
+      PairView v;
+      v.valid = 0;
+      v.nClusters = 0;
+      if (p < nPairs) v = loadPair(P, p);
+#pragma unroll
+      for (int it = 0; it < kMaxClusterIters; ++it) {
+        M[it] = 0;
+        if (it < iters) {
+          const int c = it * 8 + k;
+          const bool keep = v.valid && c < v.nClusters && clusterMayHit(P, v, c);
+          M[it] = __ballot(keep);
         }
       }
-      cmask = __ballot(keep);
     }
-
-    d3 FA = mk3(0., 0., 0.), TA = mk3(0., 0., 0.), TB = mk3(0., 0., 0.);
-    int pairContacts = 0;
-    const float* sdfB = P.sdf + SB.sdfOff;
-    const i3 nB{SB.nx, SB.ny, SB.nz};
-    const GridGeom& G = (type == PT_PW) ? P.gridW : P.gridP;
-    while (cmask) {
-      const int k = __builtin_ctzll(cmask);
-      cmask &= cmask - 1;
-      const ClusterDev cl = P.clusters[SA.clusterOff + k];
-      bool hit = false;
-      int gnode = 0;
-      d3 point, normal, spring, F;
-      double ov = 0., En = 0., Es = 0.;
-      if (lane < cl.count) {
-        const NodeDev nd = P.nodes[SA.nodeOff + cl.first + lane];
-        gnode = nodeBase + nd.orig;
-        const d3 xw = rot(qA, mk3(nd.x * sA, nd.y * sA, nd.z * sA)) + xA;
-        bool eligible = true;
-        statTests++;
-        if (type == PT_PP) {
-          const d3 rel = xw - B.x;
-          eligible = !(len2(rel) > B.r * B.r);
-        } else {
-          eligible = keyInWindow(G, B.key, xw);
+    // ---- stage 2: pair-major walk of the surviving clusters ----
+    for (int j = 0; j < kBatch; ++j) {
+      const int p = base + j;
+      if (p >= nPairs) break;
+      int nSurv = 0;
+#pragma unroll
+      for (int it = 0; it < kMaxClusterIters; ++it) nSurv += __popcll((M[it] >> (8 * j)) & 0xFFull);
+      int total = 0;
+      int first = 0;
+      if (nSurv > 0) {
+        const PairView v = loadPair(P, p);
+        // partner centre in A's unscaled frame (FP32 prefilter of particle-particle nodes)
+        float cbx = 0.f, cby = 0.f, cbz = 0.f, cr2 = 0.f;
+        if (v.type == PT_PP) {
+          const d3 cb = rotInv(v.xB - v.xA, v.qA) / v.sA;
+          cbx = float(cb.x); cby = float(cb.y); cbz = float(cb.z);
+          const double rr = v.rB / v.sA * (1.0 + 1e-4) + 1e-12;
+          cr2 = float(rr * rr);
         }
-        if (eligible) {
-          const d3 loc = rotInv(xw - B.x, B.q);
-          const double gx = (loc.x - B.go.x) * B.invSp, gy = (loc.y - B.go.y) * B.invSp,
-                       gz = (loc.z - B.go.z) * B.invSp;
-          const int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
-          if (!(i0 < 0 || j0 < 0 || k0 < 0 || i0 >= nB.x - 1 || j0 >= nB.y - 1 || k0 >= nB.z - 1)) {
-            const int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
-            const double x = gx - double(i0), y = gy - double(j0), z = gz - double(k0);
-            double c[8];
-            c[0] = double(sdfB[lin3(i3{i0, j0, k0}, nB)]) * B.sc;
-            c[1] = double(sdfB[lin3(i3{i1, j0, k0}, nB)]) * B.sc;
-            c[2] = double(sdfB[lin3(i3{i0, j1, k0}, nB)]) * B.sc;
-            c[3] = double(sdfB[lin3(i3{i1, j1, k0}, nB)]) * B.sc;
-            c[4] = double(sdfB[lin3(i3{i0, j0, k1}, nB)]) * B.sc;
-            c[5] = double(sdfB[lin3(i3{i1, j0, k1}, nB)]) * B.sc;
-            c[6] = double(sdfB[lin3(i3{i0, j1, k1}, nB)]) * B.sc;
-            c[7] = double(sdfB[lin3(i3{i1, j1, k1}, nB)]) * B.sc;
-            statQueries++;
-            ov = -triValue(x, y, z, c);
-            if (ov >= 0.) {
-              hit = true;
-              d3 n = triGradient((P.quirks & 1) ? double(int(x)) : x, y, z, c);
-              n = rot(B.q, n);
-              n = normalize(n);
-              normal = n;
-              point = xw + 0.5 * ov * n;
+        // pass 1: hit masks of the surviving clusters
+        int nHitClus = 0;
+#pragma unroll
+        for (int it = 0; it < kMaxClusterIters; ++it) {
+          unsigned long long bits = (M[it] >> (8 * j)) & 0xFFull;
+          while (bits) {
+            const int c = it * 8 + __builtin_ctzll(bits);
+            bits &= bits - 1;
+            const ClusterDev cl = P.clusters[v.clusterOff + c];
+            NodeHit h;
+            const bool hit = nodeTest(P, v, cl, lane, cbx, cby, cbz, cr2, h, statTests, statQueries);
+            const unsigned long long hm = __ballot(hit);
+            if (hm) {
+              if (lane == 0) {
+                sMask[wl][nHitClus] = hm;
+                sClus[wl][nHitClus] = c;
+              }
+              ++nHitClus;
+              total += __popcll(hm);
             }
           }
         }
-      }
-      if (hit) {
-        // tangential history of the same (list, node, partner) from the previous step
-        spring = histLookup(P, histKey(type, gnode, partner));
-        const d3 vc = vA + cross(wA, point - riA) - (B.v + cross(B.w, point - B.xf));
-        F = mk3(0., 0., 0.);
-        linearContact(F, spring, vc, normal, ov, P.dt, kn, ks, mu, res, effM);
-        En = 0.5 * ov * ov * kn;
-        Es = 0.5 * len2(spring) * ks;
-        FA += F;
-        TA += cross(point - riA, F);
-        TB += cross(point - B.xf, -F);
-      }
-      const unsigned long long hm = __ballot(hit);
-      if (hm) {
-        const int cnt = __popcll(hm);
-        int base = 0;
-        if (lane == 0) base = atomicAdd(&P.devInt[1], cnt);
-        base = __shfl(base, 0, 64);
-        pairContacts += cnt;
-        if ((long long)base + cnt > P.contactCap) {
-          if (lane == 0) atomicOr(&P.devInt[3], 2);
-        } else if (hit) {
-          const int o = base + __popcll(hm & ltMask);
-          P.cNode[o] = gnode;
-          P.cInfo[o] = info;
-          P.cSpring[o] = spring;
-          P.cPoint[o] = point;
-          P.cNormal[o] = normal;
-          P.cOverlap[o] = ov;
-          P.cEn[o] = En;
-          P.cEs[o] = Es;
+        if (total > 0) {
+          // exact contiguous range for the pair
+          if (chunkNext + total > chunkEnd) {
+            for (int o = chunkNext + lane; o < chunkEnd; o += 64) P.cInfo[o] = kInvalidInfo;
+            const int want = total > kContactChunk ? total : kContactChunk;
+            int b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&P.devInt[1], want);
+            b0 = __builtin_amdgcn_readfirstlane(__shfl(b0, 0, 64));
+            if ((long long)b0 + want > P.contactCap) {
+              overflow = true;
+              chunkNext = chunkEnd = 0;
+            } else {
+              chunkNext = b0;
+              chunkEnd = b0 + want;
+            }
+          }
+          if (overflow) {
+            if (lane == 0) atomicOr(&P.devInt[3], 2);
+            total = 0;
+          } else {
+            first = chunkNext;
+            // pass 2: the hit lanes recompute their contact and write it
+            unsigned long long dummyT = 0, dummyQ = 0;
+            int o = first;
+            for (int h = 0; h < nHitClus; ++h) {
+              const unsigned long long hm = sMask[wl][h];
+              const int c = sClus[wl][h];
+              if ((hm >> lane) & 1ull) {
+                const ClusterDev cl = P.clusters[v.clusterOff + c];
+                NodeHit nh;
+                nodeTest(P, v, cl, lane, cbx, cby, cbz, cr2, nh, dummyT, dummyQ);
+                const int dst = o + __popcll(hm & ltMask);
+                P.cNode[dst] = nh.gnode;
+                P.cInfo[dst] = v.info;
+                P.cPair[dst] = p;
+                P.cPoint[dst] = nh.point;
+                P.cNormal[dst] = nh.normal;
+                P.cOverlap[dst] = nh.ov;
+              }
+              o += __popcll(hm);
+            }
+            chunkNext = first + total;
+          }
         }
       }
-    }
-    if (pairContacts > 0) {
-      const double s0 = wsum(FA.x), s1 = wsum(FA.y), s2 = wsum(FA.z);
-      const double s3 = wsum(TA.x), s4 = wsum(TA.y), s5 = wsum(TA.z);
-      const double s6v = wsum(TB.x), s7 = wsum(TB.y), s8 = wsum(TB.z);
       if (lane == 0) {
-        double* o = P.pairSum + 9ll * p;
-        o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3; o[4] = s4; o[5] = s5; o[6] = s6v; o[7] = s7; o[8] = s8;
+        P.pairCoff[p] = first;
+        P.pairCcnt[p] = total;
       }
+      statContacts += (lane == 0) ? (unsigned long long)total : 0ull;
     }
-    if (lane == 0) P.pairFlag[p] = pairContacts;
   }
-  // statistics, one atomic per wave
+  for (int o = chunkNext + lane; o < chunkEnd; o += 64) P.cInfo[o] = kInvalidInfo;
+  // statistics: wave sums spread over kStatSlots accumulators
   const unsigned long long t = (unsigned long long)wsum(double(statTests));
   const unsigned long long q = (unsigned long long)wsum(double(statQueries));
+  const unsigned long long cc = (unsigned long long)wsum(double(statContacts));
   if (lane == 0) {
-    atomicAdd(&P.devStat[0], t);
-    atomicAdd(&P.devStat[1], q);
+    const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
+    atomicAdd(&P.statSlots[slot], t);
+    atomicAdd(&P.statSlots[kStatSlots + slot], q);
+    atomicAdd(&P.statSlots[2 * kStatSlots + slot], cc);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Contact law with tangential history, one thread per contact slot
+// ------------------------------------------------------------------------------------------
+__global__ void k_contacts(Params P) {
+  const int n = min(P.devInt[1], (int)P.contactCap);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint32_t info = P.cInfo[c];
+    if (info == kInvalidInfo) continue;
+    const uint32_t type = info >> 28;
+    const int j = int(info & kPartnerMask);
+    const int A = P.pairA[P.cPair[c]];
+    const d3 point = P.cPoint[c], normal = P.cNormal[c];
+    const double ov = P.cOverlap[c];
+    const d3 riA = P.pos[A];
+    const d3 vA = P.vel[A], wA = P.angvel[A];
+    const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A];
+    d3 xf, vB, wB;
+    double kn = 0., ks = 0., effM = 0., mu, res;
+    if (type == PT_PW) {
+      xf = P.wpos[j]; vB = P.wvel[j]; wB = P.wangvel[j];
+      kn = knA; ks = ksA;
+      if (invMA > 0.) effM = 1. / invMA;
+      mu = fmin(P.mu[A], P.wmu[j]);
+      res = fmin(P.res[A], P.wres[j]);
+    } else {
+      xf = (type == PT_PP) ? P.pos[j] : P.gpos[type - PT_GX][j];
+      vB = P.vel[j]; wB = P.angvel[j];
+      const double knB = P.kn[j], ksB = P.ks[j], invMB = P.invM[j];
+      if (knA > 0. && knB > 0.) kn = knA * knB / (knA + knB);
+      if (ksA > 0. && ksB > 0.) ks = ksA * ksB / (ksA + ksB);
+      if (invMA > 0. || invMB > 0.) effM = 1. / (invMA + invMB);
+      res = fmin(P.res[A], P.res[j]);
+      mu = fmin(P.mu[A], P.mu[j]);
+    }
+    // tangential history of the same (list, node, partner) from the previous step
+    d3 spring = histLookup(P, histKey(type, P.cNode[c], j));
+    const d3 vc = vA + cross(wA, point - riA) - (vB + cross(wB, point - xf));
+    d3 F = mk3(0., 0., 0.);
+    linearContact(F, spring, vc, normal, ov, P.dt, kn, ks, mu, res, effM);
+    P.cSpring[c] = spring;
+    P.cEn[c] = 0.5 * ov * ov * kn;
+    P.cEs[c] = 0.5 * len2(spring) * ks;
+    P.cF[c] = F;
+    P.cTA[c] = cross(point - riA, F);
+    P.cTB[c] = cross(point - xf, -F);
   }
 }
 
@@ -727,18 +901,22 @@ __global__ void k_reduce_kick(Params P) {
   if (b >= P.nP) return;
   d3 F = mk3(0., 0., 0.), T = mk3(0., 0., 0.);
   const long long cap = P.pairCap;
+  // contacts where b is the master, pair by pair, contact by contact
   for (int p = P.pairStart[b]; p < P.pairStart[b + 1] && p < cap; ++p) {
-    if (P.pairFlag[p] == 0) continue;
-    const double* s = P.pairSum + 9ll * p;
-    F += mk3(s[0], s[1], s[2]);
-    T += mk3(s[3], s[4], s[5]);
+    const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
+    for (int c = c0; c < c0 + n; ++c) {
+      F += P.cF[c];
+      T += P.cTA[c];
+    }
   }
+  // reactions on b as the partner
   for (int q = P.rcStart[b]; q < P.rcStart[b + 1]; ++q) {
     const int p = P.rcPair[q];
-    if (P.pairFlag[p] == 0) continue;
-    const double* s = P.pairSum + 9ll * p;
-    F += -mk3(s[0], s[1], s[2]);
-    T += mk3(s[6], s[7], s[8]);
+    const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
+    for (int c = c0; c < c0 + n; ++c) {
+      F += -P.cF[c];
+      T += P.cTB[c];
+    }
   }
   if (P.nBonds > 0) {
     for (int q = P.bcStart[b]; q < P.bcStart[b + 1]; ++q) {
@@ -757,12 +935,8 @@ __global__ void k_reduce_kick(Params P) {
 // Step bookkeeping
 // ------------------------------------------------------------------------------------------
 __global__ void k_step_begin(Params P) {
-  if (threadIdx.x == 0) {
-    P.devInt[1] = 0;  // contacts written
-    P.devInt[2] = 0;  // narrow-phase work counter
-    P.devStat[0] = 0;
-    P.devStat[1] = 0;
-  }
+  if (threadIdx.x == 0) P.devInt[1] = 0;  // contact slots allocated
+  for (int k = threadIdx.x; k < 3 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
 }
 __global__ void k_step_end(Params P) {
   if (threadIdx.x == 0) {
@@ -770,10 +944,25 @@ __global__ void k_step_end(Params P) {
     if ((long long)np > P.pairCap) P.devInt[3] |= 1;
     P.devInt[4] = max(P.devInt[4], np);
     P.devInt[5] = max(P.devInt[5], P.devInt[1]);
+    unsigned long long t = 0, q = 0, c = 0;
+    for (int k = 0; k < kStatSlots; ++k) {
+      t += P.statSlots[k];
+      q += P.statSlots[kStatSlots + k];
+      c += P.statSlots[2 * kStatSlots + k];
+    }
+    P.devStat[0] = t;
+    P.devStat[1] = q;
+    P.devStat[2] = c;
+    // sums over steps (profiling)
+    P.devStat[3] += (unsigned long long)np;
+    P.devStat[4] += t;
+    P.devStat[5] += q;
+    P.devStat[6] += c;
   }
 }
 
 template __global__ void k_pairs<false>(Params P);
+__global__ void k_gather_images(Params P, int set);
 template __global__ void k_pairs<true>(Params P);
 
 }  // namespace lsd

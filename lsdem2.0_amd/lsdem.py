@@ -55,6 +55,22 @@ class Counters(C.Structure):
     ]
 
 
+NUM_PHASES = 8
+PHASES = ("integrate", "sort", "pairs", "reaction", "narrow", "history", "bonds", "reduce")
+
+
+class Profile(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("ms", C.c_double * NUM_PHASES),
+        ("pairs", C.c_int64),
+        ("node_tests", C.c_int64),
+        ("sdf_queries", C.c_int64),
+        ("contacts", C.c_int64),
+        ("reserved", C.c_int64 * 8),
+    ]
+
+
 class LSDEMError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"lsdem error {code}: {msg}")
@@ -232,6 +248,15 @@ class Sim:
             self._call("get_bonds", self.ctx, _p(out["active"]), _p(out["point"]), _p(out["energies"]),
                        _p(out["sigma_max"]), _p(out["tau_max"]))
         return out
+
+    def profile_begin(self, max_steps: int):
+        self._call("profile_begin", self.ctx, C.c_int32(max_steps))
+
+    def profile_end(self) -> dict:
+        p = Profile()
+        self._call("profile_end", self.ctx, C.byref(p))
+        return {"steps": p.steps, "ms": dict(zip(PHASES, list(p.ms))), "pairs": p.pairs, "node_tests": p.node_tests,
+                "sdf_queries": p.sdf_queries, "contacts": p.contacts}
 
     def counters(self) -> dict:
         c = Counters()

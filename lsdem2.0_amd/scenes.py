@@ -302,8 +302,10 @@ def scene_c2(n: int = 200_000, seed: int = 20261204, n_types: int = 10, n_grid: 
         slack = cell_xy / m - pitch
         P["pos"][take, 0] = (idx % m + 0.5) * (cell_xy / m) + rng.uniform(-0.5, 0.5, len(take)) * slack
         P["pos"][take, 1] = (idx // m + 0.5) * (cell_xy / m) + rng.uniform(-0.5, 0.5, len(take)) * slack
-        P["pos"][take, 2] = z + 0.5 * pitch
-        z += pitch
+        # the first layer clears the floor by its bounding radius
+        zc = z + (0.5 * pitch if z > 0.0 else rbound[order[k]] * (1.0 + abs(gap)))
+        P["pos"][take, 2] = zc
+        z = zc + 0.5 * pitch
         k += len(take)
     top = z
     P["quat"] = random_quaternions(rng, n)
@@ -326,3 +328,90 @@ def scene_c2(n: int = 200_000, seed: int = 20261204, n_types: int = 10, n_grid: 
     sc = Scene(cfg=cfg, shapes=shapes, particles=P, meta={"name": "C2", "seed": seed, "bed_top": top})
     sc.walls = floor_wall(shapes, (0.5 * cell_xy, y0 + 0.5 * cell_xy), 0.0, 0.75 * cell_xy, mu, restitution)
     return sc
+
+
+def _relocate_on_top(sc: Scene, idx: np.ndarray, rng) -> None:
+    """Move particles `idx` into loose layers (pitch = 1.02 x their largest bounding diameter)
+    above the bed, then re-derive the domain (z top, and the y range so that max.y >= max.z)."""
+    P = sc.particles
+    cell_xy = sc.cfg["domain_max"][0] - sc.cfg["domain_min"][0]
+    y0 = sc.cfg["domain_min"][1]
+    rb = np.array([np.linalg.norm(s["nodes"], axis=1).max() if len(s["nodes"]) else 0.0
+                   for s in sc.shapes])[P["shape"]] * P["scale"]
+    keep = np.ones(len(rb), bool)
+    keep[idx] = False
+    z = float((P["pos"][keep, 2] + rb[keep]).max()) if keep.any() else 0.0
+    k = 0
+    while k < len(idx):
+        pitch = 2.0 * rb[idx].max() * 1.02
+        m = max(1, int(np.floor(cell_xy / pitch)))
+        take = idx[k:k + m * m]
+        j = np.arange(len(take))
+        P["pos"][take, 0] = (j % m + 0.5) * (cell_xy / m)
+        P["pos"][take, 1] = y0 + (j // m + 0.5) * (cell_xy / m)
+        P["pos"][take, 2] = z + 0.5 * pitch
+        z += pitch
+        k += len(take)
+    top = float((P["pos"][:, 2] + rb).max())
+    zmax = top + 0.05
+    new_y0 = zmax - cell_xy
+    P["pos"][:, 1] += new_y0 - y0
+    if sc.walls and len(sc.walls["shape"]):
+        sc.walls["pos"][:, 1] += new_y0 - y0
+    sc.cfg["domain_min"] = (sc.cfg["domain_min"][0], new_y0, sc.cfg["domain_min"][2])
+    sc.cfg["domain_max"] = (sc.cfg["domain_max"][0], new_y0 + cell_xy, zmax)
+    sc.meta["bed_top"] = top
+
+
+def remove_initial_overlaps(make_sim, sc: Scene, seed: int = 7, rounds: int = 12, tol: float = 5e-6,
+                            log=None) -> int:
+    """Random-lattice start without significant initial overlaps: resample the orientation of
+    every particle overlapping a neighbour, a wall or a periodic image by more than `tol`;
+    particles still overlapping after `rounds` rounds are moved into loose layers on top of the
+    bed (the particle count is unchanged).  `make_sim(cfg)` returns an un-loaded simulation
+    context; detection runs with gravity off and a vanishing time step.  Returns the number of
+    relocated particles."""
+    rng = np.random.default_rng(seed)
+    P = sc.particles
+    n = len(P["shape"])
+    node0 = np.concatenate([[0], np.cumsum([len(sc.shapes[s]["nodes"]) for s in P["shape"]])])
+    zero = np.zeros((n, 3))
+
+    def overlapping(sim):
+        sim.set_particles(pos=P["pos"], quat=P["quat"], vel=zero, angvel=zero, force=zero, torque=zero)
+        sim.step(1)
+        bad = set()
+        for lst in range(5):
+            c = sim.contacts(lst)
+            deep = c["overlap"] > tol
+            if deep.any():
+                bad.update((np.searchsorted(node0, c["master"][deep], side="right") - 1).tolist())
+        return np.array(sorted(bad), dtype=np.int64)
+
+    def detector():
+        cfg = dict(sc.cfg)
+        cfg["gravity"] = (0.0, 0.0, 0.0)
+        cfg["dt"] = 1e-30
+        return sc.load(make_sim(cfg))
+
+    sim = detector()
+    for it in range(rounds):
+        bad = overlapping(sim)
+        if log:
+            log(f"[scene] overlap round {it}: {len(bad)} particles overlap")
+        if len(bad) == 0:
+            sim.close()
+            return 0
+        P["quat"][bad] = random_quaternions(rng, len(bad))
+    bad = overlapping(sim)
+    sim.close()
+    moved = 0
+    while len(bad):
+        moved += len(bad)
+        _relocate_on_top(sc, bad, rng)
+        sim = detector()
+        bad = overlapping(sim)
+        sim.close()
+        if log:
+            log(f"[scene] relocated {moved} particles; {len(bad)} still overlap")
+    return moved
