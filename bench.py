@@ -123,7 +123,8 @@ def main():
     gpu_ms = sum(prof["ms"].values()) / K
     log(f"[bench] timed {args.steps} steps in {wall:.2f}s ({ms_step:.3f} ms/step, GPU phases {gpu_ms:.3f} ms/step)")
     log("[bench] phases ms/step: " + " ".join(f"{k}={v / K:.4f}" for k, v in prof["ms"].items()))
-    log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} node_tests={prof['node_tests'] / K:.0f} "
+    log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} cluster_tests={prof['cluster_tests'] / K:.0f} "
+        f"node_tests={prof['node_tests'] / K:.0f} "
         f"sdf_queries={prof['sdf_queries'] / K:.0f} contacts={prof['contacts'] / K:.0f}")
     base = None
     if not args.no_cpu:
@@ -150,7 +151,8 @@ def main():
                      "bytes_per_launch": per_launch_bytes, "ms_per_launch": narrow_ms},
         "cpu_baseline": base,
         "phases_ms_per_step": {k: v / K for k, v in prof["ms"].items()},
-        "work_per_step": {k: prof[k] / K for k in ("pairs", "node_tests", "sdf_queries", "contacts")},
+        "work_per_step": {k: prof[k] / K for k in ("pairs", "cluster_tests", "node_tests", "sdf_queries",
+                                                    "contacts")},
         "node_sdf_queries_per_s": prof["sdf_queries"] / K / (ms_step * 1e-3),
         "surrogate": None,
     }

@@ -30,8 +30,10 @@ __global__ void k_hash_insert(Params P);
 __global__ void k_hash_clear(Params P);
 __global__ void k_gather_particles(Params P);
 __global__ void k_gather_images(Params P, int set);
-__global__ void k_narrow(Params P);
-__global__ void k_contacts(Params P);
+__global__ void k_pair_maps(Params P);
+__global__ void k_candidates(Params P);
+size_t candidatesLdsBytes(int maxClusters);
+__global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P);
 __global__ void k_step_begin(Params P);
@@ -120,7 +122,9 @@ void buildClusters(HostShape& S) {
   }
 }
 
-// Grid-box radius and a Lipschitz bound of the trilinear interpolant: inside one cell each
+// Grid-box radius and a Lipschitz bound of the trilinear interpolant over the cells with a
+// positive corner (the only cells a path from a point with phi > 0 to the first zero of phi
+// crosses, so phi(x) <= lip * |x - y| for the nearest y with phi(y) <= 0).  Inside one cell each
 // partial derivative is a bilinear blend of the 4 edge differences, so it is bounded by the
 // largest of them.
 void shapeBounds(HostShape& S) {
@@ -136,6 +140,9 @@ void shapeBounds(HostShape& S) {
   for (int k = 0; k + 1 < S.nz; ++k)
     for (int j = 0; j + 1 < S.ny; ++j)
       for (int i = 0; i + 1 < S.nx; ++i) {
+        bool pos = false;
+        for (int c = 0; c < 8 && !pos; ++c) pos = at(i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2)) > 0.0;
+        if (!pos) continue;
         double gx = 0, gy = 0, gz = 0;
         for (int a = 0; a < 2; ++a)
           for (int b = 0; b < 2; ++b) {
@@ -194,7 +201,8 @@ struct lsdem_ctx {
   DBuf<ShapeDev> dShapes;
   DBuf<ClusterDev> dClusters;
   DBuf<NodeDev> dNodes;
-  DBuf<float4> dNodes32;
+  DBuf<float4> dNodes32, dPairMap;
+  DBuf<ClusterF32> dClusters32;
   DBuf<float> dSdf;
   DBuf<d3> dPos, dPosDet, dVel, dAng, dForce, dTorque, dGorig;
   DBuf<q4> dQuat;
@@ -244,7 +252,7 @@ struct lsdem_ctx {
 };
 
 void lsdem_ctx::releaseAll() {
-  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dSdf.release();
+  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dPairMap.release(); dClusters32.release(); dSdf.release();
   dCPair.release(); dCF.release(); dCTA.release(); dCTB.release();
   dPos.release(); dPosDet.release(); dVel.release(); dAng.release(); dForce.release(); dTorque.release();
   dGorig.release(); dQuat.release(); dInvI.release();
@@ -387,8 +395,10 @@ int runStep(lsdem_ctx* c) {
                      P.rcPair);
   phase();
   // narrow phase + contact law
-  hipLaunchKernelGGL(k_narrow, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
-  hipLaunchKernelGGL(k_contacts, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_candidates, dim3(unsigned(c->narrowBlocks)), dim3(256),
+                     unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
+  hipLaunchKernelGGL(k_exact_contacts, dim3(unsigned(c->numCU * 16)), dim3(256), 0, s, P);
   phase();
   // history of this step's contacts for the next step
   hipLaunchKernelGGL(k_hash_clear, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
@@ -635,6 +645,11 @@ int lsdem_initialize(lsdem_ctx* c) {
     std::vector<float4> n32(nds.size());
     for (size_t k = 0; k < nds.size(); ++k) n32[k] = make_float4(float(nds[k].x), float(nds[k].y), float(nds[k].z), 0.f);
     if ((rc = upload(c->dNodes32, n32, s))) return rc;
+    std::vector<ClusterF32> c32(cls.size());
+    for (size_t k = 0; k < cls.size(); ++k)
+      c32[k] = ClusterF32{float(cls[k].cx), float(cls[k].cy), float(cls[k].cz), float(cls[k].rho * (1.0 + 1e-6)),
+                          cls[k].first, cls[k].count, 0, 0};
+    if ((rc = upload(c->dClusters32, c32, s))) return rc;
   }
   int maxClusters = 1;
   for (int k = 0; k < nS; ++k) maxClusters = std::max(maxClusters, sd[k].nClusters);
@@ -735,9 +750,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);
   // narrow-phase grid: enough waves for ~16 candidate pairs per particle, at most 8 per SIMD
   c->narrowBlocks = int(std::min<long long>(c->numCU * 8ll, std::max<long long>(1, (16ll * nP + 8 * 4 - 1) / (8 * 4))));
-  // each wave may leave one partly used slot chunk (512) behind
-  long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity
-                                                     : 64ll * nP + 16384 + 512ll * 4 * c->narrowBlocks;
+  // candidate nodes (slots of the exact test); reserved per batch at their exact count
+  long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity : 64ll * nP + 16384;
   pairCap = std::min<long long>(pairCap, (1ll << 31) - 1);
   contactCap = std::min<long long>(contactCap, (1ll << 30));
   P.pairCap = pairCap;
@@ -748,6 +762,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairInfo.alloc(pairCap));
   HIPCHK(c->dPairCoff.alloc(pairCap));
   HIPCHK(c->dPairCcnt.alloc(pairCap));
+  HIPCHK(c->dPairMap.alloc(pairCap * 8));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
@@ -772,7 +787,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dHashKey.alloc(hs));
   HIPCHK(c->dHashVal.alloc(hs));
   HIPCHK(c->dHashSlot.alloc(contactCap));
-  HIPCHK(c->dStatSlots.alloc(3 * kStatSlots));
+  HIPCHK(c->dStatSlots.alloc(4 * kStatSlots));
   P.hashMask = (unsigned long long)(hs - 1);
 
   // ---- bonds (+ per-particle CSR of bond ends, in bond order) ----
@@ -805,10 +820,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
   HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
   HIPCHK(c->dDevInt.alloc(8));
-  HIPCHK(c->dDevStat.alloc(8));
-  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 3 * kStatSlots * sizeof(unsigned long long), s));
+  HIPCHK(c->dDevStat.alloc(9));
+  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, 8 * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 8 * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 9 * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
@@ -816,7 +831,7 @@ int lsdem_initialize(lsdem_ctx* c) {
 
   // ---- pointer table ----
   P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
-  P.nodes32 = c->dNodes32.p; P.maxClusters = maxClusters;
+  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.maxClusters = maxClusters;
   P.cPair = c->dCPair.p; P.cF = c->dCF.p; P.cTA = c->dCTA.p; P.cTB = c->dCTB.p;
   P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;
   P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
@@ -834,7 +849,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   for (int g = 0; g < 3; ++g) { P.gsPos[g] = c->dGSPos[g].p; P.gsBox[g] = c->dGSBox[g].p; }
   P.hashSlot = c->dHashSlot.p; P.statSlots = c->dStatSlots.p;
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
-  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p;
+  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p;
   P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
   P.hashKey = c->dHashKey.p; P.hashVal = c->dHashVal.p;
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
@@ -1035,7 +1050,7 @@ int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
     c->events.push_back(e);
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemset(c->dDevStat.p + 3, 0, 4 * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(c->dDevStat.p + 3, 0, 6 * sizeof(unsigned long long)));
   c->profMax = max_steps;
   c->profSteps = 0;
   c->profiling = true;
@@ -1057,8 +1072,9 @@ int lsdem_profile_end(lsdem_ctx* c, lsdem_profile* out) {
       out->ms[ph] += ms;
     }
   }
-  unsigned long long ds[8];
+  unsigned long long ds[9];
   HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
+  out->cluster_tests = (int64_t)ds[8];
   out->pairs = (int64_t)ds[3];
   out->node_tests = (int64_t)ds[4];
   out->sdf_queries = (int64_t)ds[5];

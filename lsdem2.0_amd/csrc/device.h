@@ -16,10 +16,15 @@ struct ShapeDev {
   long long sdfOff;
   double ox, oy, oz, h;  // unscaled grid origin and spacing
   double boxR;           // max |corner| of the grid box (unscaled), used for fixed-partner culls
-  double lip;            // Lipschitz bound of the trilinear interpolant (scale invariant)
+  double lip;            // Lipschitz bound of the trilinear interpolant over cells with a positive
+                         // corner (scale invariant)
 };
 struct ClusterDev {
   double cx, cy, cz, rho;  // bounding sphere of the cluster's nodes (unscaled)
+  int first, count, pad0, pad1;
+};
+struct ClusterF32 {
+  float cx, cy, cz, rho;  // FP32 copy of the bounding sphere, rho rounded up
   int first, count, pad0, pad1;
 };
 struct NodeDev {
@@ -57,6 +62,8 @@ struct Params {
   const ShapeDev* shapes;
   const ClusterDev* clusters;
   const NodeDev* nodes;
+  const ClusterF32* clusters32;
+  float4* pairMap;        // per candidate pair: 128-byte FP32 map record (8 float4)
   const float4* nodes32;  // same nodes in FP32 (x, y, z, 0) for the conservative prefilter
   const float* sdf;
   int maxClusters;        // largest cluster count of any shape
@@ -161,12 +168,13 @@ struct Params {
 
   // ---- device counters ----
   // devInt: [1] contact slots allocated this step, [3] overflow flags (1 pairs, 2 contacts),
-  //         [4] peak pairs, [5] peak contact slots, [6] contacts inserted in the history table
+  //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert
   int* devInt;
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
-  //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts (summed while profiling)
+  //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
+  //          (summed while profiling)
   unsigned long long* devStat;
-  unsigned long long* statSlots;  // kStatSlots x 3 spread accumulators of the narrow phase
+  unsigned long long* statSlots;  // kStatSlots x 4 spread accumulators of the narrow phase
 };
 
 constexpr int kStatSlots = 32;

@@ -6,10 +6,10 @@
 //   counting sort     particles (and ghost images) by search-grid cell, deterministic order
 //   k_pairs (x2)      bounding-sphere / grid-box candidate pairs per master particle (count, write)
 //   reaction CSR      pairs grouped by the partner that receives the reaction force
-//   k_narrow          node-vs-SDF detection, one wavefront per batch of 8 candidate pairs:
-//                     conservative cluster cull, FP32 node prefilter, exact FP64 test, contacts
-//                     of a pair written to a contiguous range
-//   k_contacts        one thread per contact: tangential history + linear spring-dashpot
+//   k_candidates      one wavefront per batch of 8 candidate pairs: FP32 cluster cull and node
+//                     prefilter (conservative), survivors written to a contiguous range per pair
+//   k_exact_contacts  one thread per candidate node: exact FP64 node-vs-SDF test; for contacts,
+//                     tangential history + linear spring-dashpot
 //   k_hash_*          history table of this step's contacts for the next step
 //   k_bonds           parallel bonds
 //   k_reduce_kick     per-body segmented sum of contact and bond contributions + second half-kick
@@ -334,8 +334,9 @@ __device__ __forceinline__ void visitNear(const SearchGrid& S, const int* cellSt
 }
 
 __device__ __forceinline__ double triAt(const Params& P, const ShapeDev& S, double sc, d3 loc, d3 go, double invSp,
-                                        bool& inside) {
-  // trilinear SDF value at a local point (for pair-level culls), inside = within the grid box
+                                        bool& inside, double& cmax) {
+  // trilinear SDF value at a local point (for pair-level culls), inside = within the grid box,
+  // cmax = largest |corner| (bounds the rounding of the value)
   const double gx = (loc.x - go.x) * invSp, gy = (loc.y - go.y) * invSp, gz = (loc.z - go.z) * invSp;
   const int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
   inside = !(i0 < 0 || j0 < 0 || k0 < 0 || i0 >= S.nx - 1 || j0 >= S.ny - 1 || k0 >= S.nz - 1);
@@ -351,6 +352,8 @@ __device__ __forceinline__ double triAt(const Params& P, const ShapeDev& S, doub
   p[5] = double(g[lin3(i3{i0 + 1, j0, k0 + 1}, n)]) * sc;
   p[6] = double(g[lin3(i3{i0, j0 + 1, k0 + 1}, n)]) * sc;
   p[7] = double(g[lin3(i3{i0 + 1, j0 + 1, k0 + 1}, n)]) * sc;
+  cmax = 0.0;
+  for (int c = 0; c < 8; ++c) cmax = fmax(cmax, fabs(p[c]));
   return triValue(gx - double(i0), gy - double(j0), gz - double(k0), p);
 }
 
@@ -407,8 +410,9 @@ __global__ void k_pairs(Params P) {
       const double rr = rA * (1.0 + 1e-9) + 1e-12;
       if (boxDist2(loc, o, S.h, S) > rr * rr) continue;
       bool inside;
-      const double phi = triAt(P, S, 1.0, loc, o, 1. / S.h, inside);
-      if (inside && phi > S.lip * rA * (1.0 + 1e-6) + 1e-9 * S.h) continue;
+      double cmax;
+      const double phi = triAt(P, S, 1.0, loc, o, 1. / S.h, inside, cmax);
+      if (inside && phi > S.lip * rA * (1.0 + 1e-6) + 1e-9 * S.h + 1e-12 * cmax) continue;
       emit(PT_PW, w);
     }
   }
@@ -567,29 +571,7 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p) {
   return v;
 }
 
-// Conservative cluster test: can any node of cluster `c` of A meet the partner?
-__device__ __forceinline__ bool clusterMayHit(const Params& P, const PairView& v, int c) {
-  const ClusterDev cl = P.clusters[v.clusterOff + c];
-  const d3 cw = rot(v.qA, mk3(cl.cx, cl.cy, cl.cz) * v.sA) + v.xA;  // cluster centre, world
-  const double rho = cl.rho * v.sA * (1.0 + 1e-9) + 1e-12;
-  if (v.type == PT_PP) {
-    const double rr = rho + v.rB * (1.0 + 1e-9) + 1e-12;
-    return len2(cw - v.xB) <= rr * rr;
-  }
-  const ShapeDev SB = P.shapes[v.shapeB];
-  const d3 loc = rotInv(cw - v.xB, v.qB);
-  if (boxDist2(loc, v.goB, SB.h * v.scB, SB) > rho * rho) return false;
-  if (v.type == PT_PW) {
-    bool inside;
-    const double phi = triAt(P, SB, 1.0, loc, v.goB, v.invSpB, inside);
-    if (inside && phi > SB.lip * rho * (1.0 + 1e-6) + 1e-9 * SB.h) return false;
-  }
-  return true;
-}
-
-constexpr int kContactChunk = 512;  // contact slots reserved per atomic
-constexpr int kBatch = 8;           // candidate pairs per wave batch
-constexpr int kMaxClusterIters = 8; // 8 x 8 = 64 clusters per shape at most
+constexpr int kContactChunk = 1024;  // candidate slots reserved per atomic
 
 struct NodeHit {
   d3 point, normal;
@@ -597,18 +579,10 @@ struct NodeHit {
   int gnode;
 };
 
-// Exact node-vs-SDF test of node `lane` of cluster `cl` of the pair (the reference's membership:
-// radius cull or cell window, grid bounds, trilinear overlap >= 0).
-__device__ __forceinline__ bool nodeTest(const Params& P, const PairView& v, const ClusterDev& cl, int lane,
-                                         float cbx, float cby, float cbz, float cr2, NodeHit& h,
-                                         unsigned long long& tests, unsigned long long& queries) {
-  if (lane >= cl.count) return false;
-  const int ni = v.nodeOff + cl.first + lane;
-  if (v.type == PT_PP) {
-    const float4 n32 = P.nodes32[ni];
-    const float dx = n32.x - cbx, dy = n32.y - cby, dz = n32.z - cbz;
-    if (dx * dx + dy * dy + dz * dz > cr2) return false;
-  }
+// Exact node-vs-SDF test of global node `ni` (cluster-ordered node array) in the reference's
+// arithmetic: node transform, radius cull or cell window, grid bounds, trilinear overlap >= 0.
+__device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, int ni, bool full, NodeHit& h,
+                                          unsigned long long& tests, unsigned long long& queries) {
   const NodeDev nd = P.nodes[ni];
   h.gnode = v.nodeBase + nd.orig;
   const d3 xw = rot(v.qA, mk3(nd.x * v.sA, nd.y * v.sA, nd.z * v.sA)) + v.xA;
@@ -640,166 +614,303 @@ __device__ __forceinline__ bool nodeTest(const Params& P, const PairView& v, con
   queries++;
   h.ov = -triValue(x, y, z, cc);
   if (!(h.ov >= 0.)) return false;
-  d3 n = triGradient((P.quirks & 1) ? double(int(x)) : x, y, z, cc);
-  n = rot(v.qB, n);
-  n = normalize(n);
-  h.normal = n;
-  h.point = xw + 0.5 * h.ov * n;
+  if (full) {
+    d3 n = triGradient((P.quirks & 1) ? double(int(x)) : x, y, z, cc);
+    n = rot(v.qB, n);
+    n = normalize(n);
+    h.normal = n;
+    h.point = xw + 0.5 * h.ov * n;
+  }
   return true;
 }
 
-__global__ __launch_bounds__(kNarrowBlock) void k_narrow(Params P) {
-  // per wave: clusters of the current pair that produced contacts, with their hit masks
-  __shared__ unsigned long long sMask[kNarrowBlock / 64][64];
-  __shared__ int sClus[kNarrowBlock / 64][64];
-  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
-  const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long statTests = 0, statQueries = 0, statContacts = 0;
-  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
-  const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
-  const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
-  const int iters = (P.maxClusters + 7) >> 3;
-  int chunkNext = 0, chunkEnd = 0;
-  bool overflow = false;
-  for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
-    // ---- stage 1: lanes (j, k) test cluster it*8+k of pair base+j ----
-    unsigned long long M[kMaxClusterIters];
-    {
-      const int j = lane >> 3, k = lane & 7;
-      const int p = base + j;
-      PairView v;
-      v.valid = 0;
-      v.nClusters = 0;
-      if (p < nPairs) v = loadPair(P, p);
-#pragma unroll
-      for (int it = 0; it < kMaxClusterIters; ++it) {
-        M[it] = 0;
-        if (it < iters) {
-          const int c = it * 8 + k;
-          const bool keep = v.valid && c < v.nClusters && clusterMayHit(P, v, c);
-          M[it] = __ballot(keep);
-        }
-      }
-    }
-    // ---- stage 2: pair-major walk of the surviving clusters ----
-    for (int j = 0; j < kBatch; ++j) {
-      const int p = base + j;
-      if (p >= nPairs) break;
-      int nSurv = 0;
-#pragma unroll
-      for (int it = 0; it < kMaxClusterIters; ++it) nSurv += __popcll((M[it] >> (8 * j)) & 0xFFull);
-      int total = 0;
-      int first = 0;
-      if (nSurv > 0) {
-        const PairView v = loadPair(P, p);
-        // partner centre in A's unscaled frame (FP32 prefilter of particle-particle nodes)
-        float cbx = 0.f, cby = 0.f, cbz = 0.f, cr2 = 0.f;
-        if (v.type == PT_PP) {
-          const d3 cb = rotInv(v.xB - v.xA, v.qA) / v.sA;
-          cbx = float(cb.x); cby = float(cb.y); cbz = float(cb.z);
-          const double rr = v.rB / v.sA * (1.0 + 1e-4) + 1e-12;
-          cr2 = float(rr * rr);
-        }
-        // pass 1: hit masks of the surviving clusters
-        int nHitClus = 0;
-#pragma unroll
-        for (int it = 0; it < kMaxClusterIters; ++it) {
-          unsigned long long bits = (M[it] >> (8 * j)) & 0xFFull;
-          while (bits) {
-            const int c = it * 8 + __builtin_ctzll(bits);
-            bits &= bits - 1;
-            const ClusterDev cl = P.clusters[v.clusterOff + c];
-            NodeHit h;
-            const bool hit = nodeTest(P, v, cl, lane, cbx, cby, cbz, cr2, h, statTests, statQueries);
-            const unsigned long long hm = __ballot(hit);
-            if (hm) {
-              if (lane == 0) {
-                sMask[wl][nHitClus] = hm;
-                sClus[wl][nHitClus] = c;
-              }
-              ++nHitClus;
-              total += __popcll(hm);
-            }
-          }
-        }
-        if (total > 0) {
-          // exact contiguous range for the pair
-          if (chunkNext + total > chunkEnd) {
-            for (int o = chunkNext + lane; o < chunkEnd; o += 64) P.cInfo[o] = kInvalidInfo;
-            const int want = total > kContactChunk ? total : kContactChunk;
-            int b0 = 0;
-            if (lane == 0) b0 = atomicAdd(&P.devInt[1], want);
-            b0 = __builtin_amdgcn_readfirstlane(__shfl(b0, 0, 64));
-            if ((long long)b0 + want > P.contactCap) {
-              overflow = true;
-              chunkNext = chunkEnd = 0;
-            } else {
-              chunkNext = b0;
-              chunkEnd = b0 + want;
-            }
-          }
-          if (overflow) {
-            if (lane == 0) atomicOr(&P.devInt[3], 2);
-            total = 0;
-          } else {
-            first = chunkNext;
-            // pass 2: the hit lanes recompute their contact and write it
-            unsigned long long dummyT = 0, dummyQ = 0;
-            int o = first;
-            for (int h = 0; h < nHitClus; ++h) {
-              const unsigned long long hm = sMask[wl][h];
-              const int c = sClus[wl][h];
-              if ((hm >> lane) & 1ull) {
-                const ClusterDev cl = P.clusters[v.clusterOff + c];
-                NodeHit nh;
-                nodeTest(P, v, cl, lane, cbx, cby, cbz, cr2, nh, dummyT, dummyQ);
-                const int dst = o + __popcll(hm & ltMask);
-                P.cNode[dst] = nh.gnode;
-                P.cInfo[dst] = v.info;
-                P.cPair[dst] = p;
-                P.cPoint[dst] = nh.point;
-                P.cNormal[dst] = nh.normal;
-                P.cOverlap[dst] = nh.ov;
-              }
-              o += __popcll(hm);
-            }
-            chunkNext = first + total;
-          }
-        }
-      }
-      if (lane == 0) {
-        P.pairCoff[p] = first;
-        P.pairCcnt[p] = total;
-      }
-      statContacts += (lane == 0) ? (unsigned long long)total : 0ull;
-    }
+// FP32 map of A's unscaled node coordinates n into the partner's grid index space,
+// g = M n + t (g = (R_B^T (x_world - x_B) - origin_B) / spacing_B), with conservative acceptance
+// regions: every margin is far above the FP32 rounding of the map and of the interpolation.
+struct alignas(16) PairF32 {
+  float m[9], t[3];
+  float c[3], r2;     // pp: partner centre in g and squared bounding radius (g units); r2 < 0: no sphere test
+  float hi[3], lo;    // grid box in g: [lo, hi] per axis
+  float phiMargin;    // SDF prefilter margin for FP32 map rounding (unscaled SDF units)
+  float rhoScale;     // A's unscaled cluster radius -> g units
+  float lipH;         // Lipschitz bound of the partner's SDF per g unit
+  int nx, ny, nz, sdfOff;
+  int nodeOff, clusterOff, nClusters, pad0, pad1;  // A's shape
+};
+static_assert(sizeof(PairF32) == 128, "pair map record is 128 bytes");
+
+__device__ __forceinline__ void makeF32(const Params& P, const PairView& v, PairF32& f) {
+  const ShapeDev SB = P.shapes[v.shapeB];
+  const double is = v.invSpB;  // 1 / (scaled spacing)
+  const d3 c0 = rotInv(rot(v.qA, mk3(v.sA, 0., 0.)), v.qB) * is;
+  const d3 c1 = rotInv(rot(v.qA, mk3(0., v.sA, 0.)), v.qB) * is;
+  const d3 c2 = rotInv(rot(v.qA, mk3(0., 0., v.sA)), v.qB) * is;
+  const d3 t = (rotInv(v.xA - v.xB, v.qB) - v.goB) * is;
+  f.m[0] = float(c0.x); f.m[1] = float(c1.x); f.m[2] = float(c2.x);
+  f.m[3] = float(c0.y); f.m[4] = float(c1.y); f.m[5] = float(c2.y);
+  f.m[6] = float(c0.z); f.m[7] = float(c1.z); f.m[8] = float(c2.z);
+  f.t[0] = float(t.x); f.t[1] = float(t.y); f.t[2] = float(t.z);
+  // |g| of any node that can pass is below reach; FP32 error of g is ~1e-7 reach
+  const double cg = len(v.goB) * is;
+  const double reach = len(t) + 8.0 * v.sA * is + cg + double(SB.nx + SB.ny + SB.nz);
+  const double abs = 1e-5 * reach + 1e-6;
+  f.c[0] = float(-v.goB.x * is); f.c[1] = float(-v.goB.y * is); f.c[2] = float(-v.goB.z * is);
+  if (v.type == PT_PP) {
+    const double r = v.rB * is * (1.0 + 1e-5) + abs;
+    f.r2 = float(r * r);
+  } else {
+    f.r2 = -1.f;
   }
-  for (int o = chunkNext + lane; o < chunkEnd; o += 64) P.cInfo[o] = kInvalidInfo;
-  // statistics: wave sums spread over kStatSlots accumulators
-  const unsigned long long t = (unsigned long long)wsum(double(statTests));
-  const unsigned long long q = (unsigned long long)wsum(double(statQueries));
-  const unsigned long long cc = (unsigned long long)wsum(double(statContacts));
-  if (lane == 0) {
-    const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
-    atomicAdd(&P.statSlots[slot], t);
-    atomicAdd(&P.statSlots[kStatSlots + slot], q);
-    atomicAdd(&P.statSlots[2 * kStatSlots + slot], cc);
+  f.lo = float(-abs);
+  f.hi[0] = float(SB.nx - 1 + abs);
+  f.hi[1] = float(SB.ny - 1 + abs);
+  f.hi[2] = float(SB.nz - 1 + abs);
+  f.lipH = float(SB.lip * SB.h * (1.0 + 1e-5));
+  f.phiMargin = float(SB.lip * SB.h * abs * 2.0 + 1e-30);
+  f.rhoScale = float(v.sA * is * (1.0 + 1e-5));
+  f.nx = SB.nx; f.ny = SB.ny; f.nz = SB.nz;
+  f.sdfOff = int(SB.sdfOff);
+  f.nodeOff = v.nodeOff;
+  f.clusterOff = v.clusterOff;
+  f.nClusters = v.nClusters;
+  f.pad0 = 0;
+  f.pad1 = 0;
+}
+
+// FP32 pair maps, one thread per candidate pair (FP64 poses in, 128-byte record out).
+__global__ void k_pair_maps(Params P) {
+  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
+  PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nPairs; p += gridDim.x * blockDim.x) {
+    const PairView v = loadPair(P, p);
+    PairF32 f;
+    makeF32(P, v, f);
+    out[p] = f;
   }
 }
 
-// ------------------------------------------------------------------------------------------
-// Contact law with tangential history, one thread per contact slot
-// ------------------------------------------------------------------------------------------
-__global__ void k_contacts(Params P) {
+__device__ __forceinline__ void mapF32(const PairF32& f, float nx, float ny, float nz, float& gx, float& gy, float& gz) {
+  gx = f.m[0] * nx + f.m[1] * ny + f.m[2] * nz + f.t[0];
+  gy = f.m[3] * nx + f.m[4] * ny + f.m[5] * nz + f.t[1];
+  gz = f.m[6] * nx + f.m[7] * ny + f.m[8] * nz + f.t[2];
+}
+
+// FP32 trilinear SDF value (unscaled) at grid coordinates g, clamped into the grid; a prefilter
+// only.  Its rounding error is below 1e-6 of the largest |corner|, which the caller adds (kRelPhi).
+constexpr float kRelPhi = 4e-6f;
+__device__ __forceinline__ float phiF32(const float* __restrict__ sdf, const PairF32& f, float gx, float gy, float gz,
+                                        float& cmax) {
+  const int i0 = min(max(int(floorf(gx)), 0), f.nx - 2), j0 = min(max(int(floorf(gy)), 0), f.ny - 2),
+            k0 = min(max(int(floorf(gz)), 0), f.nz - 2);
+  const float x = fminf(fmaxf(gx - float(i0), 0.f), 1.f), y = fminf(fmaxf(gy - float(j0), 0.f), 1.f),
+              z = fminf(fmaxf(gz - float(k0), 0.f), 1.f);
+  const int sx = 1, sy = f.nx, sz = f.nx * f.ny;
+  const float* g = sdf + f.sdfOff + i0 + j0 * sy + k0 * sz;
+  const float c000 = g[0], c100 = g[sx], c010 = g[sy], c110 = g[sx + sy];
+  const float c001 = g[sz], c101 = g[sx + sz], c011 = g[sy + sz], c111 = g[sx + sy + sz];
+  cmax = fmaxf(fmaxf(fmaxf(fabsf(c000), fabsf(c100)), fmaxf(fabsf(c010), fabsf(c110))),
+               fmaxf(fmaxf(fabsf(c001), fabsf(c101)), fmaxf(fabsf(c011), fabsf(c111))));
+  const float a = (c000 * (1.f - x) + c100 * x) * (1.f - y) + (c010 * (1.f - x) + c110 * x) * y;
+  const float b = (c001 * (1.f - x) + c101 * x) * (1.f - y) + (c011 * (1.f - x) + c111 * x) * y;
+  return a * (1.f - z) + b * z;
+}
+
+// Cluster acceptance (conservative): the cluster's bounding sphere must reach the partner's
+// bounding sphere (particle partners) and grid box, and, when its centre lies in the box, the
+// partner's SDF at the centre must be within Lipschitz reach of zero.
+__device__ __forceinline__ bool clusterF32(const float* __restrict__ sdf, const PairF32& f, const ClusterF32& cl) {
+  float gx, gy, gz;
+  mapF32(f, cl.cx, cl.cy, cl.cz, gx, gy, gz);
+  const float rho = cl.rho * f.rhoScale;
+  if (f.r2 >= 0.f) {
+    const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
+    const float r = sqrtf(f.r2) + rho;
+    if (dx * dx + dy * dy + dz * dz > r * r * 1.0001f) return false;
+  }
+  const float dx = fmaxf(0.f, fmaxf(f.lo - gx, gx - f.hi[0]));
+  const float dy = fmaxf(0.f, fmaxf(f.lo - gy, gy - f.hi[1]));
+  const float dz = fmaxf(0.f, fmaxf(f.lo - gz, gz - f.hi[2]));
+  if (dx * dx + dy * dy + dz * dz > rho * rho * 1.0001f) return false;
+  if (dx == 0.f && dy == 0.f && dz == 0.f) {
+    float cmax;
+    const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
+    if (phi > f.lipH * rho * 1.001f + f.phiMargin + kRelPhi * cmax) return false;
+  }
+  return true;
+}
+// Node acceptance (conservative superset of the exact FP64 test).
+__device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const PairF32& f, float4 n) {
+  float gx, gy, gz;
+  mapF32(f, n.x, n.y, n.z, gx, gy, gz);
+  if (f.r2 >= 0.f) {
+    const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
+    if (dx * dx + dy * dy + dz * dz > f.r2) return false;
+  }
+  if (gx < f.lo || gy < f.lo || gz < f.lo || gx > f.hi[0] || gy > f.hi[1] || gz > f.hi[2]) return false;
+  float cmax;
+  const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
+  return phi <= f.phiMargin + kRelPhi * cmax;
+}
+
+constexpr int kBatch = 8;            // candidate pairs per wave batch
+constexpr int kCandUnroll = 4;       // entries whose node loads are in flight together
+
+// Per-wave LDS of k_candidates: pair maps, surviving (pair, cluster) entries, their node masks,
+// per-pair candidate counts.
+__host__ __device__ inline int candWaveBytes(int maxClusters) {
+  const int b = int(sizeof(PairF32)) * kBatch + 16 * kBatch * maxClusters + 8 * kBatch;
+  return (b + 15) & ~15;
+}
+size_t candidatesLdsBytes(int maxClusters) { return size_t(candWaveBytes(maxClusters)) * (kNarrowBlock / 64); }
+
+// Candidate nodes: one wavefront per batch of 8 pairs.  Stage 1 (lanes = pair x cluster) maps
+// each pair into the partner's grid in FP32 and culls clusters; the survivors become entries in
+// LDS, ordered by pair.  Stage 2 (lanes = nodes of an entry, several entries in flight) applies
+// the node prefilter.  The batch then reserves its exact candidate count with one atomic and
+// writes each pair's candidates to a contiguous range.
+__global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
+  extern __shared__ __align__(16) unsigned char candLds[];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int maxC = P.maxClusters;
+  const int entCap = kBatch * maxC;
+  unsigned char* wb = candLds + wl * candWaveBytes(maxC);
+  PairF32* sF = reinterpret_cast<PairF32*>(wb);
+  unsigned long long* sMask = reinterpret_cast<unsigned long long*>(wb + sizeof(PairF32) * kBatch);
+  int2* sEnt = reinterpret_cast<int2*>(sMask + entCap);
+  int* sCnt = reinterpret_cast<int*>(sEnt + entCap);
+  int* sOff = sCnt + kBatch;
+  const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long statCand = 0, statEnt = 0;
+  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
+  const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
+  const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
+  const int iters = (maxC + 7) >> 3;
+  const int j = lane >> 3, k = lane & 7;
+  for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
+    // ---- stage 1: cluster cull ----
+    const int p = base + j;
+    const bool own = p < nPairs;
+    PairF32 f;
+    int nodeOff = 0, clusterOff = 0, nCl = 0;
+    if (own) {
+      f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
+      nodeOff = f.nodeOff;
+      clusterOff = f.clusterOff;
+      nCl = f.nClusters;
+      if (k == 0) sF[j] = f;
+    }
+    if (lane < kBatch) sCnt[lane] = 0;
+    // entries in (cluster group, pair, cluster) order; a pair's entries keep cluster order
+    int E = 0;
+    for (int it = 0; it < iters; ++it) {
+      const int c = it * 8 + k;
+      bool keep = false;
+      int cFirst = 0, cCount = 0;
+      if (own && c < nCl) {
+        const ClusterF32 cl = P.clusters32[clusterOff + c];
+        keep = clusterF32(P.sdf, f, cl);
+        cFirst = nodeOff + cl.first;
+        cCount = cl.count;
+      }
+      const unsigned long long m = __ballot(keep);
+      if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(cFirst, cCount | (j << 16));
+      E += __popcll(m);
+    }
+    statEnt += (lane == 0) ? (unsigned long long)E : 0ull;
+    // ---- stage 2: node prefilter ----
+    int total = 0;
+    for (int e0 = 0; e0 < E; e0 += kCandUnroll) {
+      int2 ent[kCandUnroll];
+      float4 nd[kCandUnroll];
+#pragma unroll
+      for (int u = 0; u < kCandUnroll; ++u) ent[u] = (e0 + u < E) ? sEnt[e0 + u] : make_int2(0, 0);
+#pragma unroll
+      for (int u = 0; u < kCandUnroll; ++u)
+        nd[u] = (lane < (ent[u].y & 0xFFFF)) ? P.nodes32[ent[u].x + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < kCandUnroll; ++u) {
+        const bool cand = lane < (ent[u].y & 0xFFFF) && nodeF32(P.sdf, sF[ent[u].y >> 16], nd[u]);
+        const unsigned long long cm = __ballot(cand);
+        const int n = __popcll(cm);
+        total += n;
+        if (lane == 0 && e0 + u < E) {
+          sMask[e0 + u] = cm;
+          sCnt[ent[u].y >> 16] += n;
+        }
+      }
+    }
+    // ---- one reservation per batch, exact size ----
+    int first = 0;
+    if (total > 0) {
+      if (lane == 0) first = atomicAdd(&P.devInt[1], total);
+      first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
+      if ((long long)first + total > P.contactCap) {
+        if (lane == 0) atomicOr(&P.devInt[3], 2);
+        total = 0;
+      }
+    }
+    if (lane < kBatch && base + lane < nPairs) {
+      int off = first;
+      for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
+      P.pairCoff[base + lane] = off;
+      P.pairCcnt[base + lane] = total ? sCnt[lane] : 0;
+      sOff[lane] = off;  // running write offset of each pair
+    }
+    // ---- write candidate slots; a pair's candidates are contiguous ----
+    if (total) {
+      for (int e = 0; e < E; ++e) {
+        const unsigned long long cm = sMask[e];
+        if (!cm) continue;
+        const int2 en = sEnt[e];
+        const int jj = en.y >> 16;
+        const int o = sOff[jj];
+        if ((cm >> lane) & 1ull) {
+          const int dst = o + __popcll(cm & ltMask);
+          P.cPair[dst] = base + jj;
+          P.cNode[dst] = en.x + lane;  // cluster-ordered node index until the exact test
+        }
+        if (lane == 0) sOff[jj] = o + __popcll(cm);
+      }
+    }
+    statCand += (lane == 0) ? (unsigned long long)total : 0ull;
+  }
+  const unsigned long long t = (unsigned long long)wsum(double(statCand));
+  const unsigned long long te = (unsigned long long)wsum(double(statEnt));
+  if (lane == 0) {
+    const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
+    atomicAdd(&P.statSlots[slot], t);
+    atomicAdd(&P.statSlots[3 * kStatSlots + slot], te);
+  }
+}
+
+// Exact test and contact law, one thread per candidate slot: the reference's node-vs-SDF
+// membership in FP64, then (for contacts) tangential history and the linear spring-dashpot.
+__global__ void k_exact_contacts(Params P) {
   const int n = min(P.devInt[1], (int)P.contactCap);
+  unsigned long long statQueries = 0, statContacts = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const uint32_t info = P.cInfo[c];
-    if (info == kInvalidInfo) continue;
-    const uint32_t type = info >> 28;
-    const int j = int(info & kPartnerMask);
-    const int A = P.pairA[P.cPair[c]];
-    const d3 point = P.cPoint[c], normal = P.cNormal[c];
-    const double ov = P.cOverlap[c];
+    const int p = P.cPair[c];
+    if (p < 0) {
+      P.cInfo[c] = kInvalidInfo;
+      continue;
+    }
+    const PairView v = loadPair(P, p);
+    NodeHit h;
+    unsigned long long tests = 0;
+    const bool hit = exactTest(P, v, P.cNode[c], true, h, tests, statQueries);
+    if (!hit) {
+      P.cInfo[c] = kInvalidInfo;
+      continue;
+    }
+    statContacts++;
+    P.cInfo[c] = v.info;
+    P.cNode[c] = h.gnode;
+    P.cPoint[c] = h.point;
+    P.cNormal[c] = h.normal;
+    P.cOverlap[c] = h.ov;
+    const uint32_t type = v.type;
+    const int j = v.partner;
+    const int A = v.A;
     const d3 riA = P.pos[A];
     const d3 vA = P.vel[A], wA = P.angvel[A];
     const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A];
@@ -822,16 +933,24 @@ __global__ void k_contacts(Params P) {
       mu = fmin(P.mu[A], P.mu[j]);
     }
     // tangential history of the same (list, node, partner) from the previous step
-    d3 spring = histLookup(P, histKey(type, P.cNode[c], j));
-    const d3 vc = vA + cross(wA, point - riA) - (vB + cross(wB, point - xf));
+    d3 spring = histLookup(P, histKey(type, h.gnode, j));
+    const d3 vc = vA + cross(wA, h.point - riA) - (vB + cross(wB, h.point - xf));
     d3 F = mk3(0., 0., 0.);
-    linearContact(F, spring, vc, normal, ov, P.dt, kn, ks, mu, res, effM);
+    linearContact(F, spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
     P.cSpring[c] = spring;
-    P.cEn[c] = 0.5 * ov * ov * kn;
+    P.cEn[c] = 0.5 * h.ov * h.ov * kn;
     P.cEs[c] = 0.5 * len2(spring) * ks;
     P.cF[c] = F;
-    P.cTA[c] = cross(point - riA, F);
-    P.cTB[c] = cross(point - xf, -F);
+    P.cTA[c] = cross(h.point - riA, F);
+    P.cTB[c] = cross(h.point - xf, -F);
+  }
+  const int lane = threadIdx.x & 63;
+  const unsigned long long q = (unsigned long long)wsum(double(statQueries));
+  const unsigned long long cc = (unsigned long long)wsum(double(statContacts));
+  if (lane == 0) {
+    const int slot = (blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1);
+    atomicAdd(&P.statSlots[kStatSlots + slot], q);
+    atomicAdd(&P.statSlots[2 * kStatSlots + slot], cc);
   }
 }
 
@@ -905,6 +1024,7 @@ __global__ void k_reduce_kick(Params P) {
   for (int p = P.pairStart[b]; p < P.pairStart[b + 1] && p < cap; ++p) {
     const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
     for (int c = c0; c < c0 + n; ++c) {
+      if (P.cInfo[c] == kInvalidInfo) continue;
       F += P.cF[c];
       T += P.cTA[c];
     }
@@ -914,6 +1034,7 @@ __global__ void k_reduce_kick(Params P) {
     const int p = P.rcPair[q];
     const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
     for (int c = c0; c < c0 + n; ++c) {
+      if (P.cInfo[c] == kInvalidInfo) continue;
       F += -P.cF[c];
       T += P.cTB[c];
     }
@@ -935,8 +1056,10 @@ __global__ void k_reduce_kick(Params P) {
 // Step bookkeeping
 // ------------------------------------------------------------------------------------------
 __global__ void k_step_begin(Params P) {
-  if (threadIdx.x == 0) P.devInt[1] = 0;  // contact slots allocated
-  for (int k = threadIdx.x; k < 3 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
+  if (threadIdx.x == 0) {
+    P.devInt[1] = 0;  // contact slots allocated
+  }
+  for (int k = threadIdx.x; k < 4 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
 }
 __global__ void k_step_end(Params P) {
   if (threadIdx.x == 0) {
@@ -944,11 +1067,12 @@ __global__ void k_step_end(Params P) {
     if ((long long)np > P.pairCap) P.devInt[3] |= 1;
     P.devInt[4] = max(P.devInt[4], np);
     P.devInt[5] = max(P.devInt[5], P.devInt[1]);
-    unsigned long long t = 0, q = 0, c = 0;
+    unsigned long long t = 0, q = 0, c = 0, e = 0;
     for (int k = 0; k < kStatSlots; ++k) {
       t += P.statSlots[k];
       q += P.statSlots[kStatSlots + k];
       c += P.statSlots[2 * kStatSlots + k];
+      e += P.statSlots[3 * kStatSlots + k];
     }
     P.devStat[0] = t;
     P.devStat[1] = q;
@@ -958,6 +1082,7 @@ __global__ void k_step_end(Params P) {
     P.devStat[4] += t;
     P.devStat[5] += q;
     P.devStat[6] += c;
+    P.devStat[8] += e;
   }
 }
 

@@ -67,7 +67,8 @@ class Profile(C.Structure):
         ("node_tests", C.c_int64),
         ("sdf_queries", C.c_int64),
         ("contacts", C.c_int64),
-        ("reserved", C.c_int64 * 8),
+        ("cluster_tests", C.c_int64),
+        ("reserved", C.c_int64 * 7),
     ]
 
 
@@ -256,7 +257,7 @@ class Sim:
         p = Profile()
         self._call("profile_end", self.ctx, C.byref(p))
         return {"steps": p.steps, "ms": dict(zip(PHASES, list(p.ms))), "pairs": p.pairs, "node_tests": p.node_tests,
-                "sdf_queries": p.sdf_queries, "contacts": p.contacts}
+                "sdf_queries": p.sdf_queries, "contacts": p.contacts, "cluster_tests": p.cluster_tests}
 
     def counters(self) -> dict:
         c = Counters()
