@@ -167,6 +167,34 @@ int lsdem_profile_begin(lsdem_ctx* ctx, int32_t max_steps);
 /* Stop recording, wait for the work and return the totals. */
 int lsdem_profile_end(lsdem_ctx* ctx, lsdem_profile* out);
 
+/* Device arrays of the particles, as the reference's apps reach them through
+ * getLSParticle().force() / torque() / velocity() / angularVelocity() and num_device(),
+ * gridDim(), blockDim() (LSParticle.h:661-670): N x 3 doubles each (AoS, the layout of double3),
+ * on the context's device, in the caller's particle order. */
+typedef struct lsdem_device_view {
+  double* force;
+  double* torque;
+  double* velocity;
+  double* angular_velocity;
+  const double* position;
+  const double* quaternion;      /* N x 4, scalar first */
+  int64_t num;                   /* num_device() */
+  int64_t grid_dim, block_dim;   /* one thread per particle at the configured block size */
+  void* stream;                  /* hipStream_t the step runs on */
+} lsdem_device_view;
+
+/* External-force hook: replaces the virtual LSSolver::addExternalForceTorque(double time)
+ * (LSSolver.h:508; called at :1009,1042,1079,1117 after all contact and bond forces and before
+ * the second half-kick).  `time` is the simulation time at the start of the step, accumulated as
+ * the reference does (time += dt per step).  The hook enqueues its work on view->stream or the
+ * null stream; it must not synchronise the device.  fn = NULL removes the hook. */
+typedef void (*lsdem_external_force_fn)(void* user, double time, const lsdem_device_view* view);
+int lsdem_set_external_force(lsdem_ctx* ctx, lsdem_external_force_fn fn, void* user);
+/* The same views outside a step (after lsdem_initialize). */
+int lsdem_get_device_view(lsdem_ctx* ctx, lsdem_device_view* out);
+/* Simulation time (sum of dt over the steps taken). */
+int lsdem_get_time(lsdem_ctx* ctx, double* time);
+
 /* Mass, centroid (local frame), inertia and inverse inertia from an SDF grid with a smoothed
  * Heaviside of half-width 1.5 cells; host computation used by the first addLSParticle
  * overload (LSSolver.h:51-163).  inertia6/inv_inertia6 may be NULL. */

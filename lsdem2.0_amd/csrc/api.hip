@@ -21,9 +21,13 @@ __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, in
 __global__ void k_kick_drift(Params P);
 __global__ void k_walls(Params P);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank);
-__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out);
-__global__ void k_bucket_sort(const int* start, int nb, int* items);
-template <bool WRITE> __global__ void k_pairs(Params P);
+__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
+                                 int* bigCount);
+__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount);
+__global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
+                                  const int* bigCount);
+__global__ void k_pairs_find(Params P);
+__global__ void k_pairs_emit(Params P);
 __global__ void k_rc_count(Params P);
 __global__ void k_rc_scatter(Params P);
 __global__ void k_hash_insert(Params P);
@@ -35,7 +39,8 @@ __global__ void k_candidates(Params P);
 size_t candidatesLdsBytes(int maxClusters);
 __global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
-__global__ void k_reduce_kick(Params P);
+__global__ void k_reduce_kick(Params P, int kick);
+__global__ void k_kick(Params P);
 __global__ void k_step_begin(Params P);
 __global__ void k_step_end(Params P);
 }  // namespace lsd
@@ -177,6 +182,9 @@ struct DBuf {
 
 struct lsdem_ctx {
   lsdem_config cfg;
+  lsdem_external_force_fn hookFn = nullptr;
+  void* hookUser = nullptr;
+  double time = 0.0;
   hipStream_t stream = nullptr;
   bool initialized = false;
   int numCU = 256;
@@ -216,9 +224,10 @@ struct lsdem_ctx {
   DBuf<d3> dGPos[3];
   DBuf<int> dGKey[3], dGCell[3], dGCellStart[3], dGSorted[3], dGCount[3], dGRank[3];
   DBuf<int> dCellCount, dCellStart, dSorted, dRank;
-  DBuf<d3> dSPos, dGSPos[3];
-  DBuf<double> dSRad, dGSBox[3];
-  DBuf<int> dPairCount, dPairStart, dPairA, dPairCoff, dPairCcnt, dRcCount, dRcStart, dRcPair, dRcRank;
+  DBuf<float4> dSPR, dSPRw, dGSPR[3];
+  DBuf<uint32_t> dPairSlab;
+  DBuf<int> dPairCount, dPairStart, dPairA, dPairCoff, dPairCcnt, dRcCount, dRcStart, dRcPair, dRcRank, dRcTmp;
+  DBuf<int> dSortTmp, dSortBig;
   DBuf<uint32_t> dPairInfo;
   DBuf<int> dCNode[2], dCPair;
   DBuf<d3> dCF, dCTA, dCTB;
@@ -265,10 +274,11 @@ void lsdem_ctx::releaseAll() {
     dGSorted[s].release(); dGCount[s].release(); dGRank[s].release();
   }
   dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
-  dSPos.release(); dSRad.release(); dHashSlot.release(); dStatSlots.release();
-  for (int g = 0; g < 3; ++g) { dGSPos[g].release(); dGSBox[g].release(); }
+  dSPR.release(); dSPRw.release(); dPairSlab.release(); dHashSlot.release(); dStatSlots.release();
+  for (int g = 0; g < 3; ++g) dGSPR[g].release();
   dPairCount.release(); dPairStart.release(); dPairA.release(); dPairCoff.release(); dPairCcnt.release();
-  dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release();
+  dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release(); dRcTmp.release();
+  dSortTmp.release(); dSortBig.release();
   dPairInfo.release();
   for (int b = 0; b < 2; ++b) {
     dCNode[b].release(); dCInfo[b].release(); dCSpring[b].release(); dCPoint[b].release();
@@ -333,9 +343,13 @@ int bucketSort(lsdem_ctx* c, const int* bucket, int n, int nb, int* count, int* 
   int rc = scan(c, count, nb, start);
   if (rc) return rc;
   if (n > 0) {
+    int* bigCount = c->P.devInt + 2;
     hipLaunchKernelGGL(k_bucket_scatter, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n,
-                       (const int*)start, (const int*)rank, out);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, (const int*)start, nb, out);
+                       (const int*)start, (const int*)rank, c->dSortTmp.p, bigCount);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, (const int*)start, nb,
+                       (const int*)c->dSortTmp.p, out, c->dSortBig.p, bigCount);
+    hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, c->stream, (const int*)start,
+                       (const int*)c->dSortTmp.p, out, (const int*)c->dSortBig.p, (const int*)bigCount);
   }
   return LSDEM_OK;
 }
@@ -350,6 +364,20 @@ void setCurrentBuffers(lsdem_ctx* c) {
 }
 
 constexpr int kPhaseEvents = LSDEM_NUM_PHASES + 1;
+
+void deviceView(lsdem_ctx* c, lsdem_device_view* v) {
+  const Params& P = c->P;
+  v->force = reinterpret_cast<double*>(P.force);
+  v->torque = reinterpret_cast<double*>(P.torque);
+  v->velocity = reinterpret_cast<double*>(P.vel);
+  v->angular_velocity = reinterpret_cast<double*>(P.angvel);
+  v->position = reinterpret_cast<const double*>(P.pos);
+  v->quaternion = reinterpret_cast<const double*>(P.quat);
+  v->num = P.nP;
+  v->block_dim = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  v->grid_dim = (P.nP + v->block_dim - 1) / v->block_dim;
+  v->stream = c->stream;
+}
 
 int runStep(lsdem_ctx* c) {
   Params& P = c->P;
@@ -379,10 +407,10 @@ int runStep(lsdem_ctx* c) {
     hipLaunchKernelGGL(k_gather_images, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P, g);
   }
   phase();
-  hipLaunchKernelGGL(k_pairs<false>, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  hipLaunchKernelGGL(k_pairs_find, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
   rc = scan(c, P.pairCount, P.nP, P.pairStart);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pairs<true>, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  hipLaunchKernelGGL(k_pairs_emit, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
   phase();
   // reaction CSR
   hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P.rcCount, (long long)P.nP, 0);
@@ -392,13 +420,15 @@ int runStep(lsdem_ctx* c) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_rc_scatter, dim3(gsPairs), dim3(256), 0, s, P);
   hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, (const int*)P.rcStart, P.nP,
-                     P.rcPair);
+                     (const int*)P.rcTmp, P.rcPair, c->dSortBig.p, P.devInt + 2);
+  hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, s, (const int*)P.rcStart,
+                     (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2));
   phase();
   // narrow phase + contact law
   hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
   hipLaunchKernelGGL(k_candidates, dim3(unsigned(c->narrowBlocks)), dim3(256),
                      unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
-  hipLaunchKernelGGL(k_exact_contacts, dim3(unsigned(c->numCU * 16)), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_exact_contacts, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
   phase();
   // history of this step's contacts for the next step
   hipLaunchKernelGGL(k_hash_clear, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
@@ -406,12 +436,21 @@ int runStep(lsdem_ctx* c) {
   phase();
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
   phase();
-  hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  if (c->hookFn) {
+    hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 0);
+    lsdem_device_view v;
+    deviceView(c, &v);
+    c->hookFn(c->hookUser, c->time, &v);
+    hipLaunchKernelGGL(k_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  } else {
+    hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 1);
+  }
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, P);
   phase();
   if (prof) c->profSteps++;
   c->cur = 1 - c->cur;
   c->stepCount++;
+  c->time += P.dt;
   return LSDEM_OK;
 }
 
@@ -728,9 +767,12 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dCellCount.alloc(ncell));
   HIPCHK(c->dCellStart.alloc(ncell + 1));
   HIPCHK(c->dSorted.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSortTmp.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSortBig.alloc(std::max(ncell, nP) + 1));
   HIPCHK(c->dRank.alloc(std::max(nP, 1)));
-  HIPCHK(c->dSPos.alloc(std::max(nP, 1)));
-  HIPCHK(c->dSRad.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSPR.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSPRw.alloc(std::max(nP, 1)));
+  HIPCHK(c->dPairSlab.alloc(size_t(std::max(nP, 1)) * kPairSlabHost));
   const bool periodic = P.periodicX || P.periodicY;
   for (int g = 0; g < 3; ++g) {
     const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
@@ -742,8 +784,7 @@ int lsdem_initialize(lsdem_ctx* c) {
     HIPCHK(c->dGRank[g].alloc(n));
     HIPCHK(c->dGCount[g].alloc(on ? ncell : 1));
     HIPCHK(c->dGCellStart[g].alloc(on ? ncell + 1 : 1));
-    HIPCHK(c->dGSPos[g].alloc(n));
-    HIPCHK(c->dGSBox[g].alloc(n));
+    HIPCHK(c->dGSPR[g].alloc(n));
   }
 
   // ---- pairs and contacts ----
@@ -766,6 +807,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
+  HIPCHK(c->dRcTmp.alloc(pairCap));
   HIPCHK(c->dRcRank.alloc(pairCap));
   HIPCHK(c->dCPair.alloc(contactCap));
   HIPCHK(c->dCF.alloc(contactCap));
@@ -845,12 +887,13 @@ int lsdem_initialize(lsdem_ctx* c) {
     P.gCellStart[g] = c->dGCellStart[g].p; P.gSorted[g] = c->dGSorted[g].p; P.gCount[g] = c->dGCount[g].p;
   }
   P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
-  P.sPos = c->dSPos.p; P.sRad = c->dSRad.p;
-  for (int g = 0; g < 3; ++g) { P.gsPos[g] = c->dGSPos[g].p; P.gsBox[g] = c->dGSBox[g].p; }
+  P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
+  for (int g = 0; g < 3; ++g) P.gsPR[g] = c->dGSPR[g].p;
   P.hashSlot = c->dHashSlot.p; P.statSlots = c->dStatSlots.p;
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
   P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p;
   P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
+  P.rcTmp = c->dRcTmp.p;
   P.hashKey = c->dHashKey.p; P.hashVal = c->dHashVal.p;
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
   P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
@@ -1137,6 +1180,27 @@ int lsdem_mass_properties(const double* sdf, int32_t nx, int32_t ny, int32_t nz,
     inv_inertia6[0] = Ii.xx; inv_inertia6[1] = Ii.yy; inv_inertia6[2] = Ii.zz;
     inv_inertia6[3] = Ii.xy; inv_inertia6[4] = Ii.xz; inv_inertia6[5] = Ii.yz;
   }
+  return LSDEM_OK;
+}
+
+
+int lsdem_set_external_force(lsdem_ctx* c, lsdem_external_force_fn fn, void* user) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  c->hookFn = fn;
+  c->hookUser = user;
+  return LSDEM_OK;
+}
+
+int lsdem_get_device_view(lsdem_ctx* c, lsdem_device_view* out) {
+  if (!c || !out) return fail(LSDEM_ERR_ARG, "null argument");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  deviceView(c, out);
+  return LSDEM_OK;
+}
+
+int lsdem_get_time(lsdem_ctx* c, double* time) {
+  if (!c || !time) return fail(LSDEM_ERR_ARG, "null argument");
+  *time = c->time;
   return LSDEM_OK;
 }
 

@@ -115,10 +115,11 @@ struct Params {
   int* sorted;      // nP: particle ids in (cell, id) order
   int* rank;        // nP (scratch)
   // particle data gathered into sorted order (contiguous runs per search-grid row)
-  d3* sPos;         // posDet
-  double* sRad;     // bounding radius, negative when the particle has no reference key
-  d3* gsPos[3];     // ghost images of each set in their sorted order
-  double* gsBox[3]; // scaled grid-box radius of the image's source particle
+  float4* sPR;      // FP32 (posDet, bounding radius; radius -1 when the particle has no reference key)
+  float4* sPRw;     // FP32 (pos after the periodic wrap, bounding radius)
+  float4* gsPR[3];  // ghost images of each set in their sorted order: FP32 (centre, scaled grid-box radius)
+  int* rcTmp;       // reaction pairs before the per-partner sort
+  uint32_t* pairSlab;  // kPairSlab x nP: first candidate pairs of each master (column per master)
 
   // ---- candidate pairs (current step) ----
   int* pairCount;   // nP (+1) counts then exclusive prefix pairStart
@@ -178,6 +179,7 @@ struct Params {
 };
 
 constexpr int kStatSlots = 32;
+constexpr int kPairSlabHost = 16;  // slab depth of the broad phase (kPairSlab in the kernels)
 constexpr uint32_t kInvalidInfo = 0xFFFFFFFFu;  // unused contact slot
 
 }  // namespace lsd

@@ -4,7 +4,9 @@
 //   k_kick_drift      half-kick, drift, reference-grid key, search cell, periodic wrap + ghost images
 //   k_walls           wall drift + wall keys
 //   counting sort     particles (and ghost images) by search-grid cell, deterministic order
-//   k_pairs (x2)      bounding-sphere / grid-box candidate pairs per master particle (count, write)
+//   k_pairs_find      FP32 bounding-sphere / grid-box candidate pairs per master particle: count,
+//                     first 16 kept in a per-master slab
+//   k_pairs_emit      pair list in master order (slab copy; masters with more candidates search again)
 //   reaction CSR      pairs grouped by the partner that receives the reaction force
 //   k_candidates      one wavefront per batch of 8 candidate pairs: FP32 cluster cull and node
 //                     prefilter (conservative), survivors written to a contiguous range per pair
@@ -240,46 +242,65 @@ __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank) 
   const int b = bucket[i];
   rank[i] = b >= 0 ? atomicAdd(&count[b], 1) : -1;
 }
-__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out) {
+__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
+                                 int* bigCount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *bigCount = 0;  // list of large buckets of the following k_bucket_sort
   if (i >= n) return;
   const int b = bucket[i];
   if (b >= 0) out[start[b] + rank[i]] = i;
 }
-// Buckets of up to 16 items are sorted in registers (odd-even transposition network),
-// larger ones by insertion sort in memory.
-__global__ void k_bucket_sort(const int* start, int nb, int* items) {
+// Buckets of up to 16 items are sorted in registers (odd-even transposition network) from
+// `tmp` into `items`; larger ones are listed for k_bucket_sort_big.
+__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int s = start[b], e = start[b + 1];
   const int n = e - s;
-  if (n <= 1) return;
-  if (n <= 16) {
-    int v[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = k < n ? items[s + k] : 0x7fffffff;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-#pragma unroll
-      for (int k = (r & 1); k + 1 < 16; k += 2) {
-        const int a = v[k], c = v[k + 1];
-        v[k] = min(a, c);
-        v[k + 1] = max(a, c);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-      if (k < n) items[s + k] = v[k];
+  if (n == 0) return;
+  if (n == 1) {
+    items[s] = tmp[s];
     return;
   }
-  for (int k = s + 1; k < e; ++k) {
-    const int v = items[k];
-    int j = k - 1;
-    while (j >= s && items[j] > v) {
-      items[j + 1] = items[j];
-      --j;
+  if (n > 16) {
+    bigList[atomicAdd(bigCount, 1)] = b;
+    return;
+  }
+  int v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = k < n ? tmp[s + k] : 0x7fffffff;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+#pragma unroll
+    for (int k = (r & 1); k + 1 < 16; k += 2) {
+      const int a = v[k], c = v[k + 1];
+      v[k] = min(a, c);
+      v[k + 1] = max(a, c);
     }
-    items[j + 1] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (k < n) items[s + k] = v[k];
+}
+// One wavefront per listed bucket: each item's place is the number of smaller items (items are
+// distinct indices).
+__global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
+                                  const int* bigCount) {
+  const int lane = threadIdx.x & 63;
+  const int nBig = *bigCount;
+  const int nWaves = int(gridDim.x * (blockDim.x >> 6));
+  for (int q = int(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); q < nBig; q += nWaves) {
+    const int b = bigList[q];
+    const int s = start[b], e = start[b + 1];
+    for (int c0 = s; c0 < e; c0 += 64) {
+      const int v = c0 + lane < e ? tmp[c0 + lane] : 0x7fffffff;
+      int rank = 0;
+      for (int c1 = s; c1 < e; c1 += 64) {
+        const int u = c1 + lane < e ? tmp[c1 + lane] : 0x7fffffff;
+        for (int k = 0; k < 64; ++k) rank += __shfl(u, k, 64) < v ? 1 : 0;
+      }
+      if (c0 + lane < e) items[s + rank] = v;
+    }
   }
 }
 
@@ -289,15 +310,17 @@ __global__ void k_gather_particles(Params P) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.nP) return;
   const int i = P.sorted[k];
-  P.sPos[k] = P.posDet[i];
-  P.sRad[k] = P.keyRef[i] >= 0 ? P.radius[i] : -1.0;
+  const d3 x = P.posDet[i], w = P.pos[i];
+  const float r = float(P.radius[i]);
+  P.sPR[k] = make_float4(float(x.x), float(x.y), float(x.z), P.keyRef[i] >= 0 ? r : -1.f);
+  P.sPRw[k] = make_float4(float(w.x), float(w.y), float(w.z), r);
 }
 __global__ void k_gather_images(Params P, int set) {
   const int n = P.gCellStart[set][P.sg.ncell];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int j = P.gSorted[set][k];
-    P.gsPos[set][k] = P.gpos[set][j];
-    P.gsBox[set][k] = P.scale[j] * P.shapes[P.shape[j]].boxR;
+    const d3 x = P.gpos[set][j];
+    P.gsPR[set][k] = make_float4(float(x.x), float(x.y), float(x.z), float(P.scale[j] * P.shapes[P.shape[j]].boxR));
   }
 }
 
@@ -366,34 +389,45 @@ __device__ __forceinline__ double boxDist2(d3 loc, d3 o, double h, const ShapeDe
   return dx * dx + dy * dy + dz * dz;
 }
 
-template <bool WRITE>
-__global__ void k_pairs(Params P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.nP) return;
-  const int A = P.sorted[t];  // masters in spatial order: neighbouring lanes share cells
+// FP32 sphere-overlap test with a margin covering the FP32 rounding of both centres and radii:
+// accepts every pair whose FP64 bounding spheres overlap (a superset is harmless, every pair is
+// settled by the exact node tests).
+__device__ __forceinline__ bool spheresMayOverlap(float4 a, float4 b, float rb) {
+  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  const float ma = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z));
+  const float mb = fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fabsf(b.z));
+  const float rr = (a.w + rb) * (1.f + 1e-6f) + 3e-7f * (ma + mb) + 1e-30f;
+  return dx * dx + dy * dy + dz * dz <= rr * rr * (1.f + 1e-6f);
+}
+
+// Candidate pairs of the master in sorted position t (A = sorted[t]), in a fixed order:
+// particle partners, walls, then ghost images set by set.  DIRECT writes them to the pair list
+// at `out`; otherwise the first kPairSlab go to the master's slab column and all are counted.
+constexpr int kPairSlab = kPairSlabHost;
+template <bool DIRECT>
+__device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long out) {
   int cnt = 0;
-  long long out = WRITE ? P.pairStart[A] : 0;
   auto emit = [&](uint32_t type, int partner) {
-    if (WRITE) {
+    const uint32_t info = (type << 28) | uint32_t(partner);
+    if (DIRECT) {
       if (out < P.pairCap) {
         P.pairA[out] = A;
-        P.pairInfo[out] = (type << 28) | uint32_t(partner);
+        P.pairInfo[out] = info;
       }
       ++out;
+    } else if (cnt < kPairSlab) {
+      P.pairSlab[(long long)cnt * P.nP + t] = info;
     }
     ++cnt;
   };
-  const double rA = P.radius[A];
+  const float4 a = P.sPR[t];
   // particle-particle: B > A, B inside the domain, bounding spheres overlap
   {
-    const d3 xA = P.sPos[t];
-    const double reach = (rA + P.rMax) * (1.0 + 2e-9) + 2e-12;
-    visitNear(P.sg, P.cellStart, xA, reach, [&](int k) {
-      const double rB = P.sRad[k];
-      if (rB < 0.0) return;
-      const d3 d = xA - P.sPos[k];
-      const double rr = (rA + rB) * (1.0 + 1e-9) + 1e-12;
-      if (len2(d) > rr * rr) return;
+    const float ma = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z));
+    const double reach = (double(a.w) + P.rMax) * (1.0 + 2e-6) + 1e-6 * double(ma) + 1e-12;
+    visitNear(P.sg, P.cellStart, mk3(a.x, a.y, a.z), reach, [&](int k) {
+      const float4 b = P.sPR[k];
+      if (b.w < 0.f || !spheresMayOverlap(a, b, b.w)) return;
       const int B = P.sorted[k];
       if (B <= A) return;
       emit(PT_PP, B);
@@ -401,6 +435,7 @@ __global__ void k_pairs(Params P) {
   }
   // walls: sphere vs grid box, then a Lipschitz bound on the wall SDF at the centre
   if (P.nW > 0) {
+    const double rA = P.radius[A];
     const d3 xA = P.posDet[A];
     for (int w = 0; w < P.nW; ++w) {
       if (P.wkey[w] < 0) continue;
@@ -419,22 +454,47 @@ __global__ void k_pairs(Params P) {
   // periodic ghost images: any node inside the image's grid box is tested, so the reach is
   // the image's grid-box radius
   if (P.periodicX || P.periodicY) {
-    const d3 xA = P.pos[A];
-    const double reach = (rA + P.boxRMax) * (1.0 + 2e-9) + 2e-12;
+    const float4 g = P.sPRw[t];  // post-wrap centre (images are built from it)
+    const float mg = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
+    const double reach = (double(g.w) + P.boxRMax) * (1.0 + 2e-6) + 1e-6 * double(mg) + 1e-12;
     for (int s = 0; s < 3; ++s) {
       const bool on = (s == 0 && P.periodicX) || (s == 1 && P.periodicY) || (s == 2 && P.periodicX && P.periodicY);
       if (!on) continue;
-      const d3* gp = P.gsPos[s];
-      const double* gb = P.gsBox[s];
+      const float4* gp = P.gsPR[s];
       const int* gs = P.gSorted[s];
-      visitNear(P.sg, P.gCellStart[s], xA, reach, [&](int k) {
-        const double rr = (rA + gb[k]) * (1.0 + 1e-9) + 1e-12;
-        if (len2(xA - gp[k]) > rr * rr) return;
+      visitNear(P.sg, P.gCellStart[s], mk3(g.x, g.y, g.z), reach, [&](int k) {
+        const float4 b = gp[k];
+        if (!spheresMayOverlap(g, b, b.w)) return;
         emit(PT_GX + s, gs[k]);
       });
     }
   }
-  if (!WRITE) P.pairCount[A] = cnt;
+  return cnt;
+}
+
+// Broad phase, one thread per master in spatial order: count and keep the first candidates.
+__global__ void k_pairs_find(Params P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.nP) return;
+  const int A = P.sorted[t];
+  P.pairCount[A] = pairsOf<false>(P, t, A, 0);
+}
+// Pair list in master order: copy the slab, or search again for masters with more candidates.
+__global__ void k_pairs_emit(Params P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.nP) return;
+  const int A = P.sorted[t];
+  const int n = P.pairCount[A];
+  const long long out = P.pairStart[A];
+  if (n > kPairSlab) {
+    pairsOf<true>(P, t, A, out);
+    return;
+  }
+  for (int k = 0; k < n; ++k) {
+    if (out + k >= P.pairCap) break;
+    P.pairA[out + k] = A;
+    P.pairInfo[out + k] = P.pairSlab[(long long)k * P.nP + t];
+  }
 }
 
 // Pairs whose partner particle receives the reaction (particle-particle, ghosts with j != A).
@@ -451,11 +511,12 @@ __global__ void k_rc_count(Params P) {
 }
 __global__ void k_rc_scatter(Params P) {
   const int n = min(P.pairStart[P.nP], (int)P.pairCap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[2] = 0;  // large-bucket list of the following sort
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const int r = P.rcRank[p];
     if (r < 0) continue;
     const int partner = int(P.pairInfo[p] & kPartnerMask);
-    P.rcPair[P.rcStart[partner] + r] = p;
+    P.rcTmp[P.rcStart[partner] + r] = p;
   }
 }
 
@@ -876,7 +937,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   }
   const unsigned long long t = (unsigned long long)wsum(double(statCand));
   const unsigned long long te = (unsigned long long)wsum(double(statEnt));
-  if (lane == 0) {
+  if (lane == 0 && (t | te)) {
     const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
     atomicAdd(&P.statSlots[slot], t);
     atomicAdd(&P.statSlots[3 * kStatSlots + slot], te);
@@ -947,7 +1008,7 @@ __global__ void k_exact_contacts(Params P) {
   const int lane = threadIdx.x & 63;
   const unsigned long long q = (unsigned long long)wsum(double(statQueries));
   const unsigned long long cc = (unsigned long long)wsum(double(statContacts));
-  if (lane == 0) {
+  if (lane == 0 && (q | cc)) {
     const int slot = (blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1);
     atomicAdd(&P.statSlots[kStatSlots + slot], q);
     atomicAdd(&P.statSlots[2 * kStatSlots + slot], cc);
@@ -1015,7 +1076,7 @@ __global__ void k_bonds(Params P) {
 // ------------------------------------------------------------------------------------------
 // Per-body segmented reduction + second half-kick
 // ------------------------------------------------------------------------------------------
-__global__ void k_reduce_kick(Params P) {
+__global__ void k_reduce_kick(Params P, int kick) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= P.nP) return;
   d3 F = mk3(0., 0., 0.), T = mk3(0., 0., 0.);
@@ -1049,7 +1110,14 @@ __global__ void k_reduce_kick(Params P) {
   }
   P.force[b] = F;
   P.torque[b] = T;
-  halfKick(P, b, F, T);
+  if (kick) halfKick(P, b, F, T);
+}
+
+// Second half-kick from the stored sums (after an external-force hook has added to them).
+__global__ void k_kick(Params P) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.nP) return;
+  halfKick(P, b, P.force[b], P.torque[b]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1086,8 +1154,6 @@ __global__ void k_step_end(Params P) {
   }
 }
 
-template __global__ void k_pairs<false>(Params P);
 __global__ void k_gather_images(Params P, int set);
-template __global__ void k_pairs<true>(Params P);
 
 }  // namespace lsd

@@ -72,6 +72,26 @@ class Profile(C.Structure):
     ]
 
 
+class DeviceView(C.Structure):
+    """lsdem_device_view: N x 3 AoS arrays (device pointers on the GPU, host pointers in the oracle)."""
+    _fields_ = [
+        ("force", C.c_void_p),
+        ("torque", C.c_void_p),
+        ("velocity", C.c_void_p),
+        ("angular_velocity", C.c_void_p),
+        ("position", C.c_void_p),
+        ("quaternion", C.c_void_p),
+        ("num", C.c_int64),
+        ("grid_dim", C.c_int64),
+        ("block_dim", C.c_int64),
+        ("stream", C.c_void_p),
+    ]
+
+
+# void (*)(void* user, double time, const lsdem_device_view* view)
+EXTERNAL_FORCE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.POINTER(DeviceView))
+
+
 class LSDEMError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"lsdem error {code}: {msg}")
@@ -200,6 +220,26 @@ class Sim:
     # -- stepping and state --
     def step(self, n: int = 1):
         self._call("step", self.ctx, C.c_int32(n))
+
+    def set_external_force(self, fn):
+        """fn(time: float, view: DeviceView) runs after the contact and bond forces of every step and
+        before the second half-kick; None removes it."""
+        if fn is None:
+            self._hook = None
+            self._call("set_external_force", self.ctx, EXTERNAL_FORCE_FN(), None)
+            return
+        self._hook = EXTERNAL_FORCE_FN(lambda user, time, view: fn(time, view.contents))
+        self._call("set_external_force", self.ctx, self._hook, None)
+
+    def device_view(self) -> DeviceView:
+        v = DeviceView()
+        self._call("get_device_view", self.ctx, C.byref(v))
+        return v
+
+    def time(self) -> float:
+        t = C.c_double()
+        self._call("get_time", self.ctx, C.byref(t))
+        return t.value
 
     def get_particles(self) -> dict:
         n = self.n_particles

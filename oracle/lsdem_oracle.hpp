@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <unordered_map>
 #include <vector>
 
@@ -281,6 +282,10 @@ class Solver {
   Ghost ghost[3];
   Counters counters;
   int64_t stepCount = 0;
+  double time = 0.0;  // accumulated as LSSolver::compute does (time += timeStep, LSSolver.h:1013)
+  // LSSolver::addExternalForceTorque(time) (LSSolver.h:508), called after all contact and bond
+  // forces and before the second half-kick (:1009,1042,1079,1117)
+  std::function<void(double)> external;
 
   // ---- setup ----
   int addShape(const float* sdf, int nx, int ny, int nz, V3 origin, double spacing, const double* nodes,
@@ -844,8 +849,10 @@ class Solver {
       if (perX && perY) forcesGhost(lists[LIST_GXY], ghost[2]);
     }
     if (!bonds.empty()) forcesBonds();
+    if (external) external(time);
     kick(0.5 * dt);
     stepCount++;
+    time += dt;
   }
 };
 

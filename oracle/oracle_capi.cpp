@@ -311,4 +311,33 @@ int lsdem_oracle_interp(const double corners[8], double x, double y, double z, i
   return LSDEM_OK;
 }
 
+int lsdem_oracle_set_external_force(void* ctx, lsdem_external_force_fn fn, void* user) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!fn) {
+    c->s.external = nullptr;
+    return LSDEM_OK;
+  }
+  c->s.external = [c, fn, user](double time) {
+    Bodies& P = c->s.P;
+    lsdem_device_view v;
+    v.force = reinterpret_cast<double*>(P.force.data());
+    v.torque = reinterpret_cast<double*>(P.torque.data());
+    v.velocity = reinterpret_cast<double*>(P.vel.data());
+    v.angular_velocity = reinterpret_cast<double*>(P.angvel.data());
+    v.position = reinterpret_cast<const double*>(P.pos.data());
+    v.quaternion = reinterpret_cast<const double*>(P.q.data());
+    v.num = int64_t(P.n());
+    v.block_dim = 256;
+    v.grid_dim = (v.num + 255) / 256;
+    v.stream = nullptr;
+    fn(user, time, &v);
+  };
+  return LSDEM_OK;
+}
+
+int lsdem_oracle_get_time(void* ctx, double* time) {
+  *time = static_cast<Ctx*>(ctx)->s.time;
+  return LSDEM_OK;
+}
+
 }  // extern "C"

@@ -13,11 +13,18 @@ def declared(header: Path):
     return sorted(set(re.findall(r"\b(lsdem_[a-z0-9_]+)\s*\(", txt)))
 
 
+def extern_c(header: Path):
+    txt = header.read_text()
+    return sorted(set(re.findall(r'extern "C"[^;{]*?\b(\w+)\s*\(', txt)))
+
+
 def test_all_symbols_exported(L):
     lib = L.load_library()
     names = declared(ROOT / "include" / "lsdem.h")
     assert len(names) >= 15
-    missing = [n for n in names if not hasattr(lib, n)]
+    helpers = extern_c(ROOT / "include" / "globalDamping.h")
+    assert helpers == ["launchAddGlobalConstantForceTorque", "launchAddGlobalDampingForceTorque"]
+    missing = [n for n in names + helpers if not hasattr(lib, n)]
     assert not missing, missing
 
 

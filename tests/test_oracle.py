@@ -446,3 +446,92 @@ def test_bond_rejects_invalid(S):
         args.update(bad)
         with pytest.raises(L.LSDEMError):
             sim.add_bond(**args)
+
+
+# ----------------------------------------------------------------------------------------------
+# external-force hook (LSSolver::addExternalForceTorque, LSSolver.h:508,1009)
+# ----------------------------------------------------------------------------------------------
+def _view_array(ptr, n, k=3):
+    return np.ctypeslib.as_array((C.c_double * (k * n)).from_address(ptr)).reshape(n, k)
+
+
+def test_external_force_hook_time_and_placement(S):
+    """The hook sees the summed forces of the step, its additions drive the second half-kick of
+    that step (and the first of the next one), and time is the step's start time (sum of dt)."""
+    r = 5e-3
+    sh = sphere_shape(r, 12, r / 4, axis_nodes(r))
+    g = (0.0, 0.0, -9.81)
+    sc = one_particle_scene(S, sh, [(0, 0, 0.01)], [(0, 0, 0)], 1.0, 1e5, 0.5, g=g, dt=1e-4, walls=False)
+    sim = sc.load(oracle_sim(sc.cfg))
+    times = []
+
+    def hook(time, view):
+        times.append(time)
+        F = _view_array(view.force, view.num)
+        assert np.all(F == 0.0)  # nothing else acts on the particle
+        F[0] = (0.0, 0.0, 9.81)  # cancels gravity for a unit mass
+    sim.set_external_force(hook)
+    sim.step(50)
+    st = sim.get_particles()
+    # the first half-kick of step 1 still sees the initial zero force; every kick after that sees
+    # gravity cancelled, so v stays at -g dt/2 and x moves linearly
+    vz = 0.0 + (0.0 * 1.0 + -9.81) * (0.5 * 1e-4)
+    z = 0.01
+    for _ in range(50):
+        z += 1e-4 * vz
+    assert np.array_equal(st["vel"][0], [0.0, 0.0, vz])
+    assert np.array_equal(st["pos"][0], [0.0, 0.0, z])
+    t = 0.0
+    want = []
+    for _ in range(50):
+        want.append(t)
+        t += 1e-4
+    assert times == want
+    assert sim.time() == t
+    # without the hook the particle falls
+    sim.set_external_force(None)
+    sim.step(10)
+    assert sim.get_particles()["vel"][0][2] < 0
+
+
+def numpy_damping(view, c):
+    """FunDEM/apps/tutorial/globalDamping.cu:4-17 in numpy: F -= c|F| v/|v|, T -= c|T| w/|w|."""
+    n = view.num
+    for fp, vp in ((view.force, view.velocity), (view.torque, view.angular_velocity)):
+        F, v = _view_array(fp, n), _view_array(vp, n)
+        l2 = (v * v).sum(1)
+        nv = np.where(l2[:, None] > 0, v * (1.0 / np.sqrt(np.where(l2 > 0, l2, 1.0)))[:, None], 0.0)
+        F -= c * np.sqrt((F * F).sum(1))[:, None] * nv
+
+
+def _drop_on_wall(S, hook=None, e=0.8):
+    r, m, kn = 5e-3, 1.3e-3, 1e5
+    sh = sphere_shape(r, 24, r / 10, axis_nodes(r))
+    dt = 1e-6
+    v0 = 0.2
+    sc = one_particle_scene(S, sh, [(0, 0, r + 0.5 * v0 * dt)], [(0, 0, -v0)], m, kn, e, dt=dt)
+    sim = sc.load(oracle_sim(sc.cfg))
+    if hook is not None:
+        sim.set_external_force(hook)
+    seen = 0
+    for _ in range(2000):
+        sim.step(1)
+        if len(sim.contacts(1)["master"]):
+            seen += 1
+        elif seen:
+            break
+    return sim.get_particles()["vel"][0, 2] / v0
+
+
+def test_damping_hook_dissipates(S):
+    """The tutorial's global damping (force -= c |F| v/|v|) takes energy out of a wall bounce, and
+    the hook sees the contact force of the step (equal to the force reported afterwards)."""
+    forces = []
+
+    def record(t, view):
+        forces.append(_view_array(view.force, view.num).copy())
+    plain = _drop_on_wall(S, hook=record)
+    assert any(np.abs(f).max() > 0 for f in forces)
+    damped = _drop_on_wall(S, hook=lambda t, v: numpy_damping(v, 0.2))
+    assert plain == pytest.approx(0.8, rel=1e-2)
+    assert 0 < damped < plain - 0.05
