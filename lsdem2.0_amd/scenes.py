@@ -415,3 +415,134 @@ def remove_initial_overlaps(make_sim, sc: Scene, seed: int = 7, rounds: int = 12
         if log:
             log(f"[scene] relocated {moved} particles; {len(bad)} still overlap")
     return moved
+
+
+# ----------------------------------------------------------------------------------------------
+# Config 3: bonded agglomerate of 3 superellipsoid types, compressed between two walls
+# ----------------------------------------------------------------------------------------------
+def quat_rotate(q: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Rotate vectors v (..., 3) by unit quaternions q (..., 4), scalar first."""
+    w, u = q[..., :1], q[..., 1:]
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def sdf_trilinear(shape: dict, p: np.ndarray) -> np.ndarray:
+    """Trilinear value of a shape's grid at unscaled local points p (n, 3); +inf outside the grid."""
+    sdf = shape["sdf"].astype(np.float64)
+    nz, ny, nx = sdf.shape
+    g = (p - shape["origin"]) / shape["spacing"]
+    i0 = np.floor(g).astype(np.int64)
+    ok = np.all((i0 >= 0) & (i0 < np.array([nx - 1, ny - 1, nz - 1])), axis=1)
+    i0 = np.clip(i0, 0, np.array([nx - 2, ny - 2, nz - 2]))
+    f = g - i0
+    out = np.zeros(len(p))
+    for c in range(8):
+        dx, dy, dz = c & 1, (c >> 1) & 1, c >> 2
+        w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * (f[:, 2] if dz else 1 - f[:, 2])
+        out += w * sdf[i0[:, 2] + dz, i0[:, 1] + dy, i0[:, 0] + dx]
+    out[~ok] = np.inf
+    return out
+
+
+def surface_gap(sc: Scene, i: int, j: int):
+    """Smallest signed distance from particle i's boundary nodes to particle j's surface (world
+    units), and the node position (world) where it occurs."""
+    P, S = sc.particles, sc.shapes
+    si, sj = S[P["shape"][i]], S[P["shape"][j]]
+    xw = quat_rotate(P["quat"][i][None, :], si["nodes"] * P["scale"][i]) + P["pos"][i]
+    loc = quat_rotate(np.array([P["quat"][j][0], *(-P["quat"][j][1:])])[None, :], xw - P["pos"][j]) / P["scale"][j]
+    phi = sdf_trilinear(sj, loc) * P["scale"][j]
+    k = int(np.argmin(phi))
+    return float(phi[k]), xw[k]
+
+
+def bond_pairs_by_gap(sc: Scene, max_gap: float, radius_mult: float, youngs: float, poisson: float, rng,
+                      weibull_shape: float = 5.0, weibull_scale: float = 1e6, friction: float = 0.0):
+    """One parallel bond per particle pair whose surfaces are closer than max_gap (both directions
+    tested): bond point midway across the gap, normal from the slave's centre towards the master's,
+    length = centre distance, radius = radius_mult x the smaller equivalent radius, tensile strength
+    and cohesion Weibull(shape, scale)."""
+    P = sc.particles
+    n = sc.n_particles
+    rb = np.array([np.linalg.norm(s["nodes"], axis=1).max() if len(s["nodes"]) else 0.0 for s in sc.shapes])
+    rbound = rb[P["shape"]] * P["scale"]
+    x = P["pos"]
+    bonds = []
+    for i in range(n):
+        d = np.linalg.norm(x[i + 1:] - x[i], axis=1)
+        for jj in np.nonzero(d <= rbound[i] + rbound[i + 1:] + max_gap)[0]:
+            j = i + 1 + int(jj)
+            g1, p1 = surface_gap(sc, i, j)
+            g2, p2 = surface_gap(sc, j, i)
+            g, p = (g1, p1) if g1 <= g2 else (g2, p2)
+            if not (g < max_gap):
+                continue
+            nrm = x[i] - x[j]
+            L = float(np.linalg.norm(nrm))
+            nrm = nrm / L
+            sgn = 1.0 if g1 <= g2 else -1.0  # node of i: point moves towards j, node of j: towards i
+            point = p - sgn * 0.5 * max(g, 0.0) * nrm
+            r = radius_mult * min(P["scale"][i], P["scale"][j])
+            st, co = weibull_scale * rng.weibull(weibull_shape, 2)
+            bonds.append((i, j, tuple(point), tuple(nrm), r, L, youngs, poisson, float(st), float(co), friction))
+    return bonds
+
+
+def scene_c3(n: int = 100_000, seed: int = 20261205, n_grid: int = 40, n_nodes: int = 400,
+             exponents=(2.0, 3.0, 6.0), r_range=(2e-3, 3e-3), pitch: float | None = None, jitter: float = 0.05,
+             dt: float = 2e-6, kn: float = 1e5, restitution: float = 0.5, mu: float = 0.5, density: float = 2500.0,
+             gravity: float = 0.0, wall_speed: float = 0.05, bond_gap: float = 1e-4, radius_mult: float = 0.8,
+             youngs: float = 1e8, poisson: float = 0.3, weibull_shape: float = 5.0, weibull_scale: float = 1e6,
+             quirks: int = 7, bonds: bool = True) -> Scene:
+    """Lattice of volume-equivalent radii U(r_range) in a cube, random orientations; bonds between
+    pairs closer than bond_gap; top and bottom walls close at wall_speed each."""
+    rng = np.random.default_rng(seed)
+    shapes: list = []
+    rb = []
+    for p in exponents:
+        e = 2.0 / p
+        ax = np.ones(3)
+        g = np.linspace(-1, 1, 121)
+        Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+        f, _ = superellipsoid_value_grad(np.stack([X, Y, Z], -1), 1.0, 1.0, 1.0, e, e)
+        vol = (f <= 0).mean() * 8.0
+        ax = ax * (4.0 / 3.0 * np.pi / vol) ** (1.0 / 3.0)
+        h = 2.0 * ax.max() / (n_grid - 5)
+        sdf, org = superellipsoid_grid(*ax, e, e, n_grid, h)
+        nodes = superellipsoid_nodes(*ax, e, e, n_nodes)
+        sid = _add_shape(shapes, sdf, org, h, nodes, density)
+        rb.append(np.linalg.norm(shapes[sid]["nodes"], axis=1).max())
+    rb = np.asarray(rb)
+    P = _particles(n)
+    P["shape"] = rng.integers(0, len(exponents), n).astype(np.int32)
+    P["scale"] = rng.uniform(r_range[0], r_range[1], n)
+    if pitch is None:
+        pitch = 2.0 * r_range[1] * 1.005
+    m = int(np.ceil(n ** (1.0 / 3.0)))
+    k = np.arange(n)
+    idx = np.stack([k % m, (k // m) % m, k // (m * m)], 1).astype(float)
+    P["pos"] = (idx + 0.5) * pitch + rng.uniform(-jitter, jitter, (n, 3)) * pitch * 0.1
+    P["quat"] = random_quaternions(rng, n)
+    for t in range(len(exponents)):
+        sel = P["shape"] == t
+        Sh = shapes[t]
+        s = P["scale"][sel]
+        P["inv_mass"][sel] = 1.0 / (Sh["mass"] * s ** 3)
+        P["inv_inertia"][sel] = Sh["inv_inertia"][None, :] / (s ** 5)[:, None]
+    P["kn"][:] = kn
+    P["ks"][:] = 0.4 * kn
+    P["mu"][:] = mu
+    P["e"][:] = restitution
+    side = m * pitch
+    lo, hi = np.zeros(3), np.array([side, side, (np.max(idx[:, 2]) + 1) * pitch])
+    margin = 0.2 * side + 0.01
+    cfg = {"domain_min": tuple(lo - margin), "domain_max": tuple(hi + margin), "gravity": (0.0, 0.0, -gravity),
+           "dt": dt, "quirks": quirks}
+    sc = Scene(cfg=cfg, shapes=shapes, particles=P, meta={"name": "C3", "seed": seed})
+    sc.walls = box_walls(shapes, lo, hi, mu, restitution)
+    sc.walls["vel"][0] = (0.0, 0.0, wall_speed)    # bottom wall moves up
+    sc.walls["vel"][1] = (0.0, 0.0, -wall_speed)   # top wall moves down
+    if bonds:
+        sc.bonds = bond_pairs_by_gap(sc, bond_gap, radius_mult, youngs, poisson, rng, weibull_shape, weibull_scale)
+    return sc

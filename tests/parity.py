@@ -38,13 +38,28 @@ def compare_state(a: dict, b: dict, ftol: float = 1e-5):
     return out
 
 
-def step_and_compare(gpu, ref, steps: int, lists=(0, 1), ftol: float = 1e-5, log=None):
+def compare_bonds(a: dict, b: dict, name: str, tol: float = 1e-9):
+    """Identical active flags (every break event on the same step); energies, peak stresses and
+    bond points within tol relative to the largest value."""
+    assert np.array_equal(a["active"], b["active"]), f"{name}: bond states differ"
+    for k in ("energies", "sigma_max", "tau_max", "point"):
+        sc = np.abs(b[k]).max() if b[k].size else 0.0
+        if sc > 0:
+            e = np.abs(a[k] - b[k]).max() / sc
+            assert e <= tol, f"{name}: bond {k} rel err {e}"
+
+
+def step_and_compare(gpu, ref, steps: int, lists=(0, 1), ftol: float = 1e-5, log=None, bonds: bool = False):
     worst = {}
     counts = []
     for s in range(steps):
         gpu.step(1)
         ref.step(1)
         row = {}
+        if bonds:
+            ba, bb = gpu.bonds(), ref.bonds()
+            compare_bonds(ba, bb, f"step {s}")
+            row["broken"] = int((bb["active"] == 0).sum())
         for l in lists:
             ca, cb = gpu.contacts(l), ref.contacts(l)
             compare_contacts(ca, cb, f"step {s} list {LISTS[l]}")

@@ -54,3 +54,15 @@ def test_history_persists(L, S):
     step_and_compare(gpu, ref, 20, lists=(0,))
     a = gpu.contacts(0)
     assert np.abs(a["spring"]).max() > 0
+
+
+def test_c3_bonded_compression(L, S):
+    # bonded agglomerate squeezed between closing walls: bond forces every step and every break
+    # event on the same step as the oracle
+    sc = S.scene_c3(n=27, seed=5, exponents=(2.0,), r_range=(2.95e-3, 3e-3), pitch=6.02e-3, jitter=0.0,
+                    wall_speed=0.5, weibull_scale=2e4, n_grid=24, n_nodes=128)
+    assert len(sc.bonds) > 40
+    gpu, ref = _pair(sc, L)
+    counts, worst = step_and_compare(gpu, ref, 120, lists=(0, 1), log=print, bonds=True)
+    broken = [c["broken"] for c in counts]
+    assert broken[0] < broken[-1] and broken[-1] >= 40
