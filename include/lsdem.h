@@ -192,8 +192,10 @@ typedef void (*lsdem_external_force_fn)(void* user, double time, const lsdem_dev
 int lsdem_set_external_force(lsdem_ctx* ctx, lsdem_external_force_fn fn, void* user);
 /* The same views outside a step (after lsdem_initialize). */
 int lsdem_get_device_view(lsdem_ctx* ctx, lsdem_device_view* out);
-/* Simulation time (sum of dt over the steps taken). */
+/* Simulation time (sum of dt over the steps taken); set_time restarts the clock, as each
+ * LSSolver::solve phase does (compute(..., time = 0.), LSSolver.h:994-996). */
 int lsdem_get_time(lsdem_ctx* ctx, double* time);
+int lsdem_set_time(lsdem_ctx* ctx, double time);
 
 /* Mass, centroid (local frame), inertia and inverse inertia from an SDF grid with a smoothed
  * Heaviside of half-width 1.5 cells; host computation used by the first addLSParticle

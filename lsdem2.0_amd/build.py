@@ -16,13 +16,24 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> Path:
-    cmd = [HIPCC, *FLAGS, *map(str, SOURCES), "-o", str(OUT)]
+APPS = sorted((HERE / "apps").glob("*.cpp"))
+INCLUDE = HERE.parent / "include"
+
+
+def _run(cmd, verbose):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
     if verbose:
         print(" ".join(cmd))
+
+
+def build(verbose: bool = False) -> Path:
+    _run([HIPCC, *FLAGS, *map(str, SOURCES), "-o", str(OUT)], verbose)
+    # example apps written against the LSSolver drop-in (include/kernel/LSSolver.h)
+    for app in APPS:
+        _run([HIPCC, "-std=c++17", "-O2", f"-I{INCLUDE}", str(app), f"-L{HERE}", "-llsdem",
+              "-Wl,-rpath,$ORIGIN/..", "-o", str(app.with_suffix(""))], verbose)
     return OUT
 
 

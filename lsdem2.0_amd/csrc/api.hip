@@ -1204,4 +1204,10 @@ int lsdem_get_time(lsdem_ctx* c, double* time) {
   return LSDEM_OK;
 }
 
+int lsdem_set_time(lsdem_ctx* c, double time) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  c->time = time;
+  return LSDEM_OK;
+}
+
 }  // extern "C"

@@ -340,4 +340,9 @@ int lsdem_oracle_get_time(void* ctx, double* time) {
   return LSDEM_OK;
 }
 
+int lsdem_oracle_set_time(void* ctx, double time) {
+  static_cast<Ctx*>(ctx)->s.time = time;
+  return LSDEM_OK;
+}
+
 }  // extern "C"
