@@ -56,7 +56,9 @@ typedef struct lsdem_config {
   int64_t pair_capacity;  /* candidate pairs per step, 0 = automatic */
   int64_t contact_capacity; /* contacts per step, 0 = automatic */
   int32_t device;         /* HIP device ordinal */
-  int32_t reserved[7];
+  int32_t no_sdf_cull;    /* 1: cull node clusters by bounding spheres / grid boxes only (no SDF
+                             bound); for measuring what the SDF and surrogate culls save */
+  int32_t reserved[6];
 } lsdem_config;
 
 typedef struct lsdem_counters {
@@ -196,6 +198,56 @@ int lsdem_get_device_view(lsdem_ctx* ctx, lsdem_device_view* out);
  * LSSolver::solve phase does (compute(..., time = 0.), LSSolver.h:994-996). */
 int lsdem_get_time(lsdem_ctx* ctx, double* time);
 int lsdem_set_time(lsdem_ctx* ctx, double time);
+
+/* ---- Learned culling surrogate (no counterpart in the reference) ----
+ * A Fourier-feature MLP per shape family (every shape used by a particle): 16 sin/cos frequencies
+ * of the normalised local point (96 features) + a 16-d learned shape embedding -> 4 x 64 SiLU ->
+ * 1 distance.  BF16 weights, FP32 accumulation (MFMA).  When enabled, node clusters whose
+ * surrogate distance exceeds their Lipschitz reach plus the band (band_factor x the maximum
+ * validation error) are culled before any node is transformed; everything else takes the exact
+ * grid path.  Every audit_every steps the step runs the exact path and checks that each contact
+ * lies in a cluster the surrogate keeps (missed_contacts must stay 0). */
+typedef struct lsdem_surrogate_params {
+  int32_t iterations;      /* optimizer steps (3000) */
+  int32_t batch;           /* samples per step, a multiple of 64 (4096) */
+  int64_t samples;         /* training samples drawn from the grids (2,000,000) */
+  int64_t validation;      /* validation samples for the error bound (262,144) */
+  double lr, lr_final;     /* Adam step size, cosine decay (2e-3 -> 1e-4) */
+  double clamp;            /* |target| clamp in units of the family length scale (0.6) */
+  double band_factor;      /* band = band_factor x max validation error (2) */
+  int32_t audit_every;     /* exact audit period in steps, 0 = never (100) */
+  int32_t pad;
+  uint64_t seed;
+  int64_t reserved[8];
+} lsdem_surrogate_params;
+typedef struct lsdem_surrogate_report {
+  int32_t trained, enabled, n_shapes, n_params;
+  int64_t iterations;
+  double final_loss;       /* training MSE of the last step (normalised units) */
+  double max_error;        /* max |surrogate - clamped SDF| on the validation set (unscaled SDF units) */
+  double mean_error;
+  double band;             /* cull band (unscaled SDF units) */
+  double train_ms;         /* GPU time of the optimizer steps */
+  double length_scale;     /* family length scale (1 / input normalisation) */
+  int64_t audits, audited_contacts, missed_contacts;
+  int64_t reserved[8];
+} lsdem_surrogate_report;
+void lsdem_surrogate_params_default(lsdem_surrogate_params* p);
+/* Train on the GPU (after lsdem_initialize); the network is then available to the step. */
+int lsdem_surrogate_train(lsdem_ctx* ctx, const lsdem_surrogate_params* p);
+int lsdem_surrogate_enable(lsdem_ctx* ctx, int32_t on);
+int lsdem_surrogate_set_band(lsdem_ctx* ctx, double band, int32_t audit_every);
+int lsdem_surrogate_get_report(lsdem_ctx* ctx, lsdem_surrogate_report* r);
+/* FP32 parameters: W1 (112x64, row = input), b1, W2, b2, W3, b3, W4, b4 (64x64, 64), w5 (64),
+ * b5, then the embeddings (n_shapes x 16, family slots in shape-id order). */
+int lsdem_surrogate_get_weights(lsdem_ctx* ctx, float* w, int64_t count);
+int lsdem_surrogate_set_weights(lsdem_ctx* ctx, const float* w, int64_t count);
+/* Surrogate SDF (unscaled units) at unscaled local points of the given shapes. */
+int lsdem_surrogate_eval(lsdem_ctx* ctx, int64_t n, const float* points, const int32_t* shape_ids, float* out);
+/* Mean-squared-error loss against normalised targets and its gradient (same layout as the
+ * weights), computed by the training kernels; n a multiple of 64. */
+int lsdem_surrogate_gradient(lsdem_ctx* ctx, int64_t n, const float* points, const int32_t* shape_ids,
+                             const float* targets, float* grad, double* loss);
 
 /* Mass, centroid (local frame), inertia and inverse inertia from an SDF grid with a smoothed
  * Heaviside of half-width 1.5 cells; host computation used by the first addLSParticle

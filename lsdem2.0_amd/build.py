@@ -5,7 +5,8 @@ import subprocess
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "api.hip", HERE / "csrc" / "apps.hip"]
+SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "api.hip", HERE / "csrc" / "apps.hip",
+           HERE / "csrc" / "surrogate.hip"]
 OUT = HERE / "liblsdem.so"
 HIPCC = "/opt/rocm/bin/hipcc"
 FLAGS = [

@@ -10,6 +10,7 @@
 
 #include "../../include/lsdem.h"
 #include "device.h"
+#include "surrogate_train.h"
 
 namespace lsd {
 // kernels.hip
@@ -35,7 +36,8 @@ __global__ void k_hash_clear(Params P);
 __global__ void k_gather_particles(Params P);
 __global__ void k_gather_images(Params P, int set);
 __global__ void k_pair_maps(Params P);
-__global__ void k_candidates(Params P);
+template <bool SUR> __global__ void k_candidates(Params P);
+__global__ void k_sur_audit(Params P);
 size_t candidatesLdsBytes(int maxClusters);
 __global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
@@ -211,6 +213,28 @@ struct lsdem_ctx {
   DBuf<NodeDev> dNodes;
   DBuf<float4> dNodes32, dPairMap;
   DBuf<ClusterF32> dClusters32;
+  DBuf<int> dNodeCluster;
+  // learned culling surrogate
+  struct Sur {
+    bool trained = false, enabled = false;
+    int nShapes = 0;
+    std::vector<int> famShape;  // slot -> shape id
+    float invLen = 1.f, clamp = 0.6f, band = 0.f;
+    double maxErr = 0, meanErr = 0, loss = 0, trainMs = 0;
+    long long iterations = 0, audits = 0;
+    int auditEvery = 100;
+    DBuf<float> param, embF, m, v, partial, loss1, sx, st, vx, vt, out, errSum;
+    DBuf<int> sslot, vslot, famShapeD, slotOfShape;
+    DBuf<sur_v4s> wF, wB;
+    DBuf<uint16_t> emb;
+    DBuf<unsigned int> errBits;
+    void release() {
+      param.release(); embF.release(); m.release(); v.release(); partial.release(); loss1.release();
+      sx.release(); st.release(); vx.release(); vt.release(); out.release(); errSum.release();
+      sslot.release(); vslot.release(); famShapeD.release(); slotOfShape.release();
+      wF.release(); wB.release(); emb.release(); errBits.release();
+    }
+  } sur;
   DBuf<float> dSdf;
   DBuf<d3> dPos, dPosDet, dVel, dAng, dForce, dTorque, dGorig;
   DBuf<q4> dQuat;
@@ -289,6 +313,8 @@ void lsdem_ctx::releaseAll() {
   dBConst.release(); dBOut.release(); dBLocal.release(); dBPoint.release(); dBF.release(); dBMi.release();
   dBMj.release();
   dDevInt.release(); dScanBlocks.release(); dDevStat.release();
+  dNodeCluster.release();
+  sur.release();
 }
 
 namespace {
@@ -426,9 +452,20 @@ int runStep(lsdem_ctx* c) {
   phase();
   // narrow phase + contact law
   hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
-  hipLaunchKernelGGL(k_candidates, dim3(unsigned(c->narrowBlocks)), dim3(256),
-                     unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
+  const bool useSur = c->sur.enabled && c->sur.trained;
+  const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
+  if (useSur && !audit)
+    hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
+                       unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
+  else
+    hipLaunchKernelGGL(k_candidates<false>, dim3(unsigned(c->narrowBlocks)), dim3(256),
+                       unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
   hipLaunchKernelGGL(k_exact_contacts, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
+  if (audit) {
+    // the step ran the exact path; check every contact against the surrogate's decision
+    hipLaunchKernelGGL(k_sur_audit, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P);
+    c->sur.audits++;
+  }
   phase();
   // history of this step's contacts for the next step
   hipLaunchKernelGGL(k_hash_clear, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
@@ -652,6 +689,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.g = mk3(c->cfg.gravity[0], c->cfg.gravity[1], c->cfg.gravity[2]);
   P.dt = c->cfg.dt;
   P.quirks = c->cfg.quirks;
+  P.sdfCull = c->cfg.no_sdf_cull ? 0 : 1;
   P.periodicX = c->cfg.periodic_x != 0;
   P.periodicY = c->cfg.periodic_y != 0;
   const d3 dmn = mk3(c->cfg.domain_min[0], c->cfg.domain_min[1], c->cfg.domain_min[2]);
@@ -692,6 +730,17 @@ int lsdem_initialize(lsdem_ctx* c) {
   }
   int maxClusters = 1;
   for (int k = 0; k < nS; ++k) maxClusters = std::max(maxClusters, sd[k].nClusters);
+  {
+    // cluster of every node, indexed by the shape's node offset + the caller's node index
+    std::vector<int> nc(std::max<size_t>(nds.size(), 1), 0);
+    for (int k = 0; k < nS; ++k) {
+      const HostShape& S = c->shapes[k];
+      for (size_t cc = 0; cc < S.clusters.size(); ++cc)
+        for (int j = S.clusters[cc].first; j < S.clusters[cc].first + S.clusters[cc].count; ++j)
+          nc[sd[k].nodeOff + S.perm[j].orig] = int(cc);
+    }
+    if ((rc = upload(c->dNodeCluster, nc, s))) return rc;
+  }
   if ((rc = upload(c->dSdf, sdf, s))) return rc;
 
   // ---- particles ----
@@ -803,7 +852,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairInfo.alloc(pairCap));
   HIPCHK(c->dPairCoff.alloc(pairCap));
   HIPCHK(c->dPairCcnt.alloc(pairCap));
-  HIPCHK(c->dPairMap.alloc(pairCap * 8));
+  HIPCHK(c->dPairMap.alloc(pairCap * 10));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
@@ -862,10 +911,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
   HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
   HIPCHK(c->dDevInt.alloc(8));
-  HIPCHK(c->dDevStat.alloc(9));
+  HIPCHK(c->dDevStat.alloc(12));
   HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, 8 * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 9 * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 12 * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
@@ -873,7 +922,8 @@ int lsdem_initialize(lsdem_ctx* c) {
 
   // ---- pointer table ----
   P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
-  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.maxClusters = maxClusters;
+  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.nodeCluster = c->dNodeCluster.p;
+  P.surSlot = nullptr; P.surBand = 0.f; P.sur = SurNet{}; P.maxClusters = maxClusters;
   P.cPair = c->dCPair.p; P.cF = c->dCF.p; P.cTA = c->dCTA.p; P.cTB = c->dCTB.p;
   P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;
   P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
@@ -1207,6 +1257,362 @@ int lsdem_get_time(lsdem_ctx* c, double* time) {
 int lsdem_set_time(lsdem_ctx* c, double time) {
   if (!c) return fail(LSDEM_ERR_ARG, "null context");
   c->time = time;
+  return LSDEM_OK;
+}
+
+
+// ------------------------------------------------------------------------------------------
+// Learned culling surrogate
+// ------------------------------------------------------------------------------------------
+void lsdem_surrogate_params_default(lsdem_surrogate_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->iterations = 3000;
+  p->batch = 4096;
+  p->samples = 2000000;
+  p->validation = 262144;
+  p->lr = 2e-3;
+  p->lr_final = 1e-4;
+  p->clamp = 0.6;
+  p->band_factor = 2.0;
+  p->audit_every = 100;
+  p->seed = 20261206ull;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The family is every shape used by a particle; slots follow shape ids.
+int surEnsureFamily(lsdem_ctx* c) {
+  auto& S = c->sur;
+  if (S.nShapes > 0) return LSDEM_OK;
+  std::vector<int> used(c->shapes.size(), 0);
+  for (int sh : c->pShape) used[sh] = 1;
+  std::vector<int> slotOf(std::max<size_t>(c->shapes.size(), 1), -1);
+  double maxBox = 0;
+  for (size_t k = 0; k < c->shapes.size(); ++k)
+    if (used[k]) {
+      slotOf[k] = int(S.famShape.size());
+      S.famShape.push_back(int(k));
+      maxBox = std::max(maxBox, c->shapes[k].boxR);
+    }
+  if (S.famShape.empty()) return fail(LSDEM_ERR_STATE, "no particle shapes to learn");
+  S.nShapes = int(S.famShape.size());
+  S.invLen = float(1.0 / maxBox);
+  const int nAll = kSurNParam + S.nShapes * kSurEmb;
+  HIPCHK(S.param.alloc(kSurNParam));
+  HIPCHK(S.embF.alloc(size_t(S.nShapes) * kSurEmb));
+  HIPCHK(S.m.alloc(nAll));
+  HIPCHK(S.v.alloc(nAll));
+  HIPCHK(S.wF.alloc(kSurNTiles * 64));
+  HIPCHK(S.wB.alloc(kSurNTiles * 64));
+  HIPCHK(S.emb.alloc(size_t(S.nShapes) * kSurEmb));
+  HIPCHK(S.loss1.alloc(1));
+  HIPCHK(S.errBits.alloc(1));
+  HIPCHK(S.errSum.alloc(1));
+  HIPCHK(S.famShapeD.alloc(S.nShapes));
+  HIPCHK(S.slotOfShape.alloc(slotOf.size()));
+  HIPCHK(hipMemcpy(S.famShapeD.p, S.famShape.data(), S.nShapes * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(S.slotOfShape.p, slotOf.data(), slotOf.size() * sizeof(int), hipMemcpyHostToDevice));
+  return LSDEM_OK;
+}
+
+SurNet surNet(lsdem_ctx* c) {
+  auto& S = c->sur;
+  SurNet N;
+  N.wF = S.wF.p;
+  N.wB = S.wB.p;
+  N.param = S.param.p;
+  N.emb = S.emb.p;
+  N.invLen = S.invLen;
+  N.nShapes = S.nShapes;
+  return N;
+}
+
+// Network in use by the step (pair maps carry the family slot from now on).
+void surPublish(lsdem_ctx* c) {
+  c->P.sur = surNet(c);
+  c->P.surSlot = c->sur.slotOfShape.p;
+  c->P.surBand = c->sur.band;
+}
+
+int surUpload(lsdem_ctx* c, const float* all) {
+  auto& S = c->sur;
+  HIPCHK(hipMemcpy(S.param.p, all, kSurNParam * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(S.embF.p, all + kSurNParam, size_t(S.nShapes) * kSurEmb * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_sur_pack, dim3(80), dim3(256), 0, c->stream, (const float*)S.param.p, (const float*)S.embF.p,
+                     S.nShapes, S.wF.p, S.wB.p, S.emb.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return LSDEM_OK;
+}
+
+int surSamples(lsdem_ctx* c, DBuf<float>& x, DBuf<int>& slot, DBuf<float>& t, long long n, unsigned long long seed) {
+  auto& S = c->sur;
+  HIPCHK(x.alloc(size_t(n) * 3));
+  HIPCHK(slot.alloc(n));
+  HIPCHK(t.alloc(n));
+  SurFamily F{c->P.shapes, c->P.sdf, S.famShapeD.p, S.nShapes, S.invLen, S.clamp};
+  SurSamples D{x.p, slot.p, t.p, n};
+  hipLaunchKernelGGL(k_sur_sample, dim3(unsigned(c->numCU * 4)), dim3(256), 0, c->stream, F, D, seed);
+  HIPCHK(hipGetLastError());
+  return LSDEM_OK;
+}
+
+// Max and mean |out - target| over a sample set (normalised units).
+int surMeasure(lsdem_ctx* c, DBuf<float>& x, DBuf<int>& slot, DBuf<float>& t, long long n, double& maxErr,
+               double& meanErr) {
+  auto& S = c->sur;
+  HIPCHK(hipMemsetAsync(S.errBits.p, 0, sizeof(unsigned int), c->stream));
+  HIPCHK(hipMemsetAsync(S.errSum.p, 0, sizeof(float), c->stream));
+  SurSamples D{x.p, slot.p, t.p, n};
+  hipLaunchKernelGGL(k_sur_eval, dim3(unsigned(c->numCU * 2)), dim3(256), 0, c->stream, surNet(c), D, (float*)nullptr,
+                     S.errBits.p, S.errSum.p);
+  HIPCHK(hipGetLastError());
+  unsigned int bits = 0;
+  float sum = 0.f;
+  HIPCHK(hipMemcpy(&bits, S.errBits.p, sizeof bits, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&sum, S.errSum.p, sizeof sum, hipMemcpyDeviceToHost));
+  float m;
+  std::memcpy(&m, &bits, sizeof m);
+  maxErr = m;
+  meanErr = n > 0 ? double(sum) / double(n) : 0.0;
+  return LSDEM_OK;
+}
+
+long long coprimeStride(long long n) {
+  long long s = (2654435761ll % n) | 1;
+  auto gcd = [](long long a, long long b) {
+    while (b) { const long long t = a % b; a = b; b = t; }
+    return a;
+  };
+  while (gcd(s, n) != 1) s += 2;
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsdem_surrogate_set_weights(lsdem_ctx* c, const float* w, int64_t count) {
+  if (!c || !w) return fail(LSDEM_ERR_ARG, "null argument");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  int rc = surEnsureFamily(c);
+  if (rc) return rc;
+  if (count != kSurNParam + int64_t(c->sur.nShapes) * kSurEmb)
+    return fail(LSDEM_ERR_ARG, "weight count must be " + std::to_string(kSurNParam + c->sur.nShapes * kSurEmb));
+  if ((rc = surUpload(c, w))) return rc;
+  c->sur.trained = true;
+  surPublish(c);
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_get_weights(lsdem_ctx* c, float* w, int64_t count) {
+  if (!c || !w) return fail(LSDEM_ERR_ARG, "null argument");
+  if (c->sur.nShapes == 0) return fail(LSDEM_ERR_STATE, "no surrogate");
+  if (count != kSurNParam + int64_t(c->sur.nShapes) * kSurEmb) return fail(LSDEM_ERR_ARG, "bad weight count");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(w, c->sur.param.p, kSurNParam * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(w + kSurNParam, c->sur.embF.p, size_t(c->sur.nShapes) * kSurEmb * sizeof(float),
+                   hipMemcpyDeviceToHost));
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_train(lsdem_ctx* c, const lsdem_surrogate_params* prm) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  lsdem_surrogate_params p;
+  if (prm) p = *prm;
+  else lsdem_surrogate_params_default(&p);
+  if (p.iterations < 0 || p.batch <= 0 || p.batch % kSurRows || p.samples <= 0 || p.validation <= 0 || p.clamp <= 0)
+    return fail(LSDEM_ERR_ARG, "bad surrogate parameters (batch must be a positive multiple of 64)");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  int rc = surEnsureFamily(c);
+  if (rc) return rc;
+  auto& S = c->sur;
+  S.clamp = float(p.clamp);
+  S.auditEvery = p.audit_every;
+  const int nAll = kSurNParam + S.nShapes * kSurEmb;
+  // initial weights: He-normal (Box-Muller over a splitmix stream), zero biases
+  {
+    std::vector<float> w(nAll, 0.f);
+    unsigned long long st = p.seed;
+    auto next = [&]() {
+      st += 0x9e3779b97f4a7c15ull;
+      unsigned long long z = st;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      return double((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto normal = [&]() {
+      const double u1 = std::max(next(), 1e-300), u2 = next();
+      return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int k = 0; k < kSurIn * kSurHid; ++k) w[kSurOffW[0] + k] = float(normal() * std::sqrt(2.0 / kSurIn));
+    for (int l = 1; l < 4; ++l)
+      for (int k = 0; k < kSurHid * kSurHid; ++k) w[kSurOffW[l] + k] = float(normal() * std::sqrt(2.0 / kSurHid));
+    for (int k = 0; k < kSurHid; ++k) w[kSurOffW5 + k] = float(normal() * std::sqrt(1.0 / kSurHid));
+    for (int k = 0; k < S.nShapes * kSurEmb; ++k) w[kSurNParam + k] = float(normal() * 0.5);
+    if ((rc = surUpload(c, w.data()))) return rc;
+  }
+  HIPCHK(hipMemsetAsync(S.m.p, 0, nAll * sizeof(float), c->stream));
+  HIPCHK(hipMemsetAsync(S.v.p, 0, nAll * sizeof(float), c->stream));
+  if ((rc = surSamples(c, S.sx, S.sslot, S.st, p.samples, p.seed ^ 0x51ull))) return rc;
+  if ((rc = surSamples(c, S.vx, S.vslot, S.vt, p.validation, p.seed ^ 0xA7ull))) return rc;
+  const int blocks = p.batch / kSurRows;
+  HIPCHK(S.partial.alloc(size_t(blocks) * (nAll + 1)));
+  const long long stride = coprimeStride(p.samples);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  const double b1 = 0.9, b2 = 0.999;
+  for (int it = 0; it < p.iterations; ++it) {
+    const double lr = p.lr_final + 0.5 * (p.lr - p.lr_final) * (1.0 + std::cos(3.141592653589793 * it / p.iterations));
+    SurTrainArgs T{surNet(c), SurSamples{S.sx.p, S.sslot.p, S.st.p, p.samples},
+                   (long long)((unsigned long long)it * (unsigned long long)p.batch % (unsigned long long)p.samples),
+                   stride, p.batch, S.partial.p};
+    hipLaunchKernelGGL(k_sur_train, dim3(unsigned(blocks)), dim3(256), 0, c->stream, T);
+    SurAdamArgs Ad{S.param.p, S.embF.p, S.m.p, S.v.p, S.partial.p, blocks, S.nShapes, float(lr), float(b1), float(b2),
+                   1e-8f, float(1.0 / (1.0 - std::pow(b1, it + 1))), float(1.0 / (1.0 - std::pow(b2, it + 1))),
+                   S.wF.p, S.wB.p, S.emb.p, S.loss1.p};
+    hipLaunchKernelGGL(k_sur_adam, dim3(unsigned((nAll + 255) / 256)), dim3(256), 0, c->stream, Ad);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  float loss = 0.f;
+  HIPCHK(hipMemcpy(&loss, S.loss1.p, sizeof loss, hipMemcpyDeviceToHost));
+  S.loss = loss;
+  S.trainMs = ms;
+  S.iterations = p.iterations;
+  if ((rc = surMeasure(c, S.vx, S.vslot, S.vt, p.validation, S.maxErr, S.meanErr))) return rc;
+  S.band = float(p.band_factor * S.maxErr / S.invLen);
+  S.trained = true;
+  surPublish(c);
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_enable(lsdem_ctx* c, int32_t on) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (on && !c->sur.trained) return fail(LSDEM_ERR_STATE, "surrogate not trained");
+  c->sur.enabled = on != 0;
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_set_band(lsdem_ctx* c, double band, int32_t audit_every) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!(band >= 0.0)) return fail(LSDEM_ERR_ARG, "band must be >= 0");
+  c->sur.band = float(band);
+  c->sur.auditEvery = audit_every;
+  if (c->sur.trained) surPublish(c);
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_get_report(lsdem_ctx* c, lsdem_surrogate_report* r) {
+  if (!c || !r) return fail(LSDEM_ERR_ARG, "null argument");
+  std::memset(r, 0, sizeof(*r));
+  const auto& S = c->sur;
+  r->trained = S.trained;
+  r->enabled = S.enabled;
+  r->n_shapes = S.nShapes;
+  r->n_params = kSurNParam + S.nShapes * kSurEmb;
+  r->iterations = S.iterations;
+  r->final_loss = S.loss;
+  r->max_error = S.maxErr / S.invLen;
+  r->mean_error = S.meanErr / S.invLen;
+  r->band = S.band;
+  r->train_ms = S.trainMs;
+  r->length_scale = 1.0 / S.invLen;
+  r->audits = S.audits;
+  if (c->initialized) {
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long ds[12];
+    HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
+    r->audited_contacts = (int64_t)ds[10];
+    r->missed_contacts = (int64_t)ds[11];
+  }
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_eval(lsdem_ctx* c, int64_t n, const float* points, const int32_t* shape_ids, float* out) {
+  if (!c || (n > 0 && (!points || !shape_ids || !out))) return fail(LSDEM_ERR_ARG, "null argument");
+  if (c->sur.nShapes == 0) return fail(LSDEM_ERR_STATE, "no surrogate");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  auto& S = c->sur;
+  std::vector<int> slot(n);
+  std::vector<int> slotOf(c->shapes.size(), -1);
+  for (int k = 0; k < S.nShapes; ++k) slotOf[S.famShape[k]] = k;
+  for (int64_t i = 0; i < n; ++i) {
+    if (shape_ids[i] < 0 || shape_ids[i] >= int(slotOf.size()) || slotOf[shape_ids[i]] < 0)
+      return fail(LSDEM_ERR_ARG, "shape outside the surrogate family");
+    slot[i] = slotOf[shape_ids[i]];
+  }
+  DBuf<float> x, o;
+  DBuf<int> sl;
+  HIPCHK(x.alloc(size_t(n) * 3));
+  HIPCHK(o.alloc(n));
+  HIPCHK(sl.alloc(n));
+  HIPCHK(hipMemcpy(x.p, points, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sl.p, slot.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice));
+  SurSamples D{x.p, sl.p, nullptr, n};
+  hipLaunchKernelGGL(k_sur_eval, dim3(unsigned(c->numCU * 2)), dim3(256), 0, c->stream, surNet(c), D, o.p,
+                     (unsigned int*)nullptr, (float*)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<float> r(n);
+  HIPCHK(hipMemcpy(r.data(), o.p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; ++i) out[i] = r[i] / S.invLen;
+  return LSDEM_OK;
+}
+
+int lsdem_surrogate_gradient(lsdem_ctx* c, int64_t n, const float* points, const int32_t* shape_ids,
+                             const float* targets, float* grad, double* loss) {
+  if (!c || !points || !shape_ids || !targets || !grad) return fail(LSDEM_ERR_ARG, "null argument");
+  if (n <= 0 || n % kSurRows) return fail(LSDEM_ERR_ARG, "n must be a positive multiple of 64");
+  if (c->sur.nShapes == 0) return fail(LSDEM_ERR_STATE, "no surrogate");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  auto& S = c->sur;
+  std::vector<int> slotOf(c->shapes.size(), -1), slot(n);
+  for (int k = 0; k < S.nShapes; ++k) slotOf[S.famShape[k]] = k;
+  for (int64_t i = 0; i < n; ++i) {
+    if (shape_ids[i] < 0 || shape_ids[i] >= int(slotOf.size()) || slotOf[shape_ids[i]] < 0)
+      return fail(LSDEM_ERR_ARG, "shape outside the surrogate family");
+    slot[i] = slotOf[shape_ids[i]];
+  }
+  const int blocks = int(n / kSurRows);
+  const int nAll = kSurNParam + S.nShapes * kSurEmb;
+  DBuf<float> x, t, part;
+  DBuf<int> sl;
+  HIPCHK(x.alloc(size_t(n) * 3));
+  HIPCHK(t.alloc(n));
+  HIPCHK(sl.alloc(n));
+  HIPCHK(part.alloc(size_t(blocks) * (nAll + 1)));
+  HIPCHK(hipMemcpy(x.p, points, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(t.p, targets, size_t(n) * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sl.p, slot.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice));
+  SurTrainArgs T{surNet(c), SurSamples{x.p, sl.p, t.p, n}, 0, 1, int(n), part.p};
+  hipLaunchKernelGGL(k_sur_train, dim3(unsigned(blocks)), dim3(256), 0, c->stream, T);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<float> h(size_t(blocks) * (nAll + 1));
+  HIPCHK(hipMemcpy(h.data(), part.p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  double l = 0;
+  for (int k = 0; k < nAll; ++k) {
+    double g = 0;
+    for (int b = 0; b < blocks; ++b) g += h[size_t(b) * (nAll + 1) + k];
+    grad[k] = float(g);
+  }
+  for (int b = 0; b < blocks; ++b) l += h[size_t(b) * (nAll + 1) + nAll];
+  if (loss) *loss = l;
   return LSDEM_OK;
 }
 

@@ -2,6 +2,7 @@
 // value to every kernel.
 #pragma once
 #include "dmath.h"
+#include "surrogate.h"
 
 namespace lsd {
 
@@ -50,7 +51,7 @@ struct Params {
   int nP, nW, nShapes, nBonds;
   d3 g;
   double dt;
-  int quirks, periodicX, periodicY, pad0;
+  int quirks, periodicX, periodicY, sdfCull;
   GridGeom gridP, gridW;
   SearchGrid sg;
   double rMax;        // max particle bounding radius
@@ -173,9 +174,16 @@ struct Params {
   int* devInt;
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
-  //          (summed while profiling)
+  //          (summed while profiling); [9] audits, [10] audited contacts, [11] missed contacts
+  //          (cumulative)
   unsigned long long* devStat;
   unsigned long long* statSlots;  // kStatSlots x 4 spread accumulators of the narrow phase
+
+  // ---- learned culling surrogate ----
+  SurNet sur;              // network (valid when surSlot != nullptr)
+  const int* surSlot;      // family slot of each shape id, -1 outside the family; nullptr: untrained
+  float surBand;           // cull band, unscaled SDF units (2 x measured max error)
+  const int* nodeCluster;  // per shape node (caller order, at the shape's node offset): its cluster
 };
 
 constexpr int kStatSlots = 32;

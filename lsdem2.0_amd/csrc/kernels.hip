@@ -16,6 +16,7 @@
 //   k_bonds           parallel bonds
 //   k_reduce_kick     per-body segmented sum of contact and bond contributions + second half-kick
 #include "device.h"
+#include "surrogate.h"
 
 namespace lsd {
 
@@ -697,8 +698,11 @@ struct alignas(16) PairF32 {
   float lipH;         // Lipschitz bound of the partner's SDF per g unit
   int nx, ny, nz, sdfOff;
   int nodeOff, clusterOff, nClusters, pad0, pad1;  // A's shape
+  float o[3], h;      // partner's unscaled grid origin and spacing (g -> local point)
+  int slot;           // partner's surrogate family slot, -1 if none (walls)
+  int pad2, pad3, pad4;
 };
-static_assert(sizeof(PairF32) == 128, "pair map record is 128 bytes");
+static_assert(sizeof(PairF32) == 160, "pair map record is 160 bytes");
 
 __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, PairF32& f) {
   const ShapeDev SB = P.shapes[v.shapeB];
@@ -736,6 +740,10 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
   f.nClusters = v.nClusters;
   f.pad0 = 0;
   f.pad1 = 0;
+  f.o[0] = float(SB.ox); f.o[1] = float(SB.oy); f.o[2] = float(SB.oz);
+  f.h = float(SB.h);
+  f.slot = (P.surSlot && v.type != PT_PW) ? P.surSlot[v.shapeB] : -1;
+  f.pad2 = f.pad3 = f.pad4 = 0;
 }
 
 // FP32 pair maps, one thread per candidate pair (FP64 poses in, 128-byte record out).
@@ -779,10 +787,13 @@ __device__ __forceinline__ float phiF32(const float* __restrict__ sdf, const Pai
 // Cluster acceptance (conservative): the cluster's bounding sphere must reach the partner's
 // bounding sphere (particle partners) and grid box, and, when its centre lies in the box, the
 // partner's SDF at the centre must be within Lipschitz reach of zero.
-__device__ __forceinline__ bool clusterF32(const float* __restrict__ sdf, const PairF32& f, const ClusterF32& cl) {
-  float gx, gy, gz;
+// Geometric part of the cluster test: bounding spheres and grid box.  `inside` reports whether the
+// centre lies in the grid box (then an SDF bound at the centre can cull the cluster).
+__device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF32& cl, float& gx, float& gy, float& gz,
+                                               float& rho, bool& inside) {
   mapF32(f, cl.cx, cl.cy, cl.cz, gx, gy, gz);
-  const float rho = cl.rho * f.rhoScale;
+  rho = cl.rho * f.rhoScale;
+  inside = false;
   if (f.r2 >= 0.f) {
     const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
     const float r = sqrtf(f.r2) + rho;
@@ -792,12 +803,35 @@ __device__ __forceinline__ bool clusterF32(const float* __restrict__ sdf, const 
   const float dy = fmaxf(0.f, fmaxf(f.lo - gy, gy - f.hi[1]));
   const float dz = fmaxf(0.f, fmaxf(f.lo - gz, gz - f.hi[2]));
   if (dx * dx + dy * dy + dz * dz > rho * rho * 1.0001f) return false;
-  if (dx == 0.f && dy == 0.f && dz == 0.f) {
-    float cmax;
-    const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
-    if (phi > f.lipH * rho * 1.001f + f.phiMargin + kRelPhi * cmax) return false;
-  }
+  inside = dx == 0.f && dy == 0.f && dz == 0.f;
   return true;
+}
+// SDF part with the exact grid: phi(centre) beyond Lipschitz reach of zero culls the cluster.
+__device__ __forceinline__ bool clusterSdfF32(const float* __restrict__ sdf, const PairF32& f, float gx, float gy,
+                                              float gz, float rho) {
+  float cmax;
+  const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
+  return !(phi > f.lipH * rho * 1.001f + f.phiMargin + kRelPhi * cmax);
+}
+__device__ __forceinline__ bool clusterF32(const float* __restrict__ sdf, const PairF32& f, const ClusterF32& cl,
+                                           bool sdfCull) {
+  float gx, gy, gz, rho;
+  bool inside;
+  if (!clusterGeomF32(f, cl, gx, gy, gz, rho, inside)) return false;
+  return !inside || !sdfCull || clusterSdfF32(sdf, f, gx, gy, gz, rho);
+}
+// Surrogate part: the same bound with the learned distance, widened by the band (twice the
+// surrogate's measured maximum error).  `d` is the network output in normalised units.
+__device__ __forceinline__ bool clusterSurKeep(const Params& P, const PairF32& f, float d, float rho) {
+  const float phi = d * (1.0f / P.sur.invLen);
+  return !(phi - P.surBand > f.lipH * rho * 1.001f + f.phiMargin);
+}
+// Lane's surrogate query for a cluster centre at grid coordinates g (normalised local point).
+__device__ __forceinline__ void surQueryPoint(const Params& P, const PairF32& f, float gx, float gy, float gz,
+                                              float& ux, float& uy, float& uz) {
+  ux = (f.o[0] + gx * f.h) * P.sur.invLen;
+  uy = (f.o[1] + gy * f.h) * P.sur.invLen;
+  uz = (f.o[2] + gz * f.h) * P.sur.invLen;
 }
 // Node acceptance (conservative superset of the exact FP64 test).
 __device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const PairF32& f, float4 n) {
@@ -829,6 +863,7 @@ size_t candidatesLdsBytes(int maxClusters) { return size_t(candWaveBytes(maxClus
 // LDS, ordered by pair.  Stage 2 (lanes = nodes of an entry, several entries in flight) applies
 // the node prefilter.  The batch then reserves its exact candidate count with one atomic and
 // writes each pair's candidates to a contiguous range.
+template <bool SUR>
 __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   extern __shared__ __align__(16) unsigned char candLds[];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
@@ -867,11 +902,34 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       const int c = it * 8 + k;
       bool keep = false;
       int cFirst = 0, cCount = 0;
-      if (own && c < nCl) {
-        const ClusterF32 cl = P.clusters32[clusterOff + c];
-        keep = clusterF32(P.sdf, f, cl);
-        cFirst = nodeOff + cl.first;
-        cCount = cl.count;
+      if (!SUR) {
+        if (own && c < nCl) {
+          const ClusterF32 cl = P.clusters32[clusterOff + c];
+          keep = clusterF32(P.sdf, f, cl, P.sdfCull != 0);
+          cFirst = nodeOff + cl.first;
+          cCount = cl.count;
+        }
+      } else {
+        // the wave's 64 (pair, cluster) centres go through the network together
+        float gx = 0.f, gy = 0.f, gz = 0.f, rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+        bool inside = false;
+        int qslot = -1;
+        if (own && c < nCl) {
+          const ClusterF32 cl = P.clusters32[clusterOff + c];
+          keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
+          cFirst = nodeOff + cl.first;
+          cCount = cl.count;
+          if (keep && inside) {
+            if (f.slot >= 0) {
+              qslot = f.slot;
+              surQueryPoint(P, f, gx, gy, gz, ux, uy, uz);
+            } else {
+              keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho);
+            }
+          }
+        }
+        const float d = surInfer64(P.sur, ux, uy, uz, qslot, lane);
+        if (qslot >= 0) keep = clusterSurKeep(P, f, d, rho);
       }
       const unsigned long long m = __ballot(keep);
       if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(cFirst, cCount | (j << 16));
@@ -941,6 +999,45 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
     atomicAdd(&P.statSlots[slot], t);
     atomicAdd(&P.statSlots[3 * kStatSlots + slot], te);
+  }
+}
+
+// Exact audit of the surrogate: every contact of this (plain) step whose partner is in the
+// family must lie in a cluster that the surrogate keeps; a culled one is a missed contact.
+// One lane per contact slot, 64 per wave through the network.
+__global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
+  const int lane = threadIdx.x & 63;
+  const int n = min(P.devInt[1], (int)P.contactCap);
+  const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
+  unsigned long long checked = 0, missed = 0;
+  for (int base = int(blockIdx.x * (kNarrowBlock / 64) + (threadIdx.x >> 6)) * 64; base < n; base += nWaves * 64) {
+    const int c = base + lane;
+    bool q = false;
+    PairF32 f;
+    float gx = 0.f, gy = 0.f, gz = 0.f, rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+    if (c < n && P.cInfo[c] != kInvalidInfo) {
+      const int p = P.cPair[c];
+      f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
+      if (f.slot >= 0) {
+        const int A = P.pairA[p];
+        const int orig = P.cNode[c] - P.nodeStart[A];
+        const int cluster = P.nodeCluster[f.nodeOff + orig];
+        const ClusterF32 cl = P.clusters32[f.clusterOff + cluster];
+        bool inside;
+        const bool geom = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
+        q = geom && inside;  // the only clusters the surrogate decides on
+        if (q) surQueryPoint(P, f, gx, gy, gz, ux, uy, uz);
+        checked++;
+      }
+    }
+    const float d = surInfer64(P.sur, ux, uy, uz, q ? f.slot : -1, lane);
+    if (q && !clusterSurKeep(P, f, d, rho)) missed++;
+  }
+  const unsigned long long tc = (unsigned long long)wsum(double(checked));
+  const unsigned long long tm = (unsigned long long)wsum(double(missed));
+  if (lane == 0 && (tc | tm)) {
+    atomicAdd(&P.devStat[10], tc);
+    atomicAdd(&P.devStat[11], tm);
   }
 }
 
@@ -1155,5 +1252,8 @@ __global__ void k_step_end(Params P) {
 }
 
 __global__ void k_gather_images(Params P, int set);
+
+template __global__ void k_candidates<false>(Params P);
+template __global__ void k_candidates<true>(Params P);
 
 }  // namespace lsd

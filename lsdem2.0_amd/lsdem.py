@@ -38,8 +38,30 @@ class Config(C.Structure):
         ("pair_capacity", C.c_int64),
         ("contact_capacity", C.c_int64),
         ("device", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("no_sdf_cull", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
+
+
+class SurrogateParams(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("batch", C.c_int32), ("samples", C.c_int64), ("validation", C.c_int64),
+        ("lr", C.c_double), ("lr_final", C.c_double), ("clamp", C.c_double), ("band_factor", C.c_double),
+        ("audit_every", C.c_int32), ("pad", C.c_int32), ("seed", C.c_uint64), ("reserved", C.c_int64 * 8),
+    ]
+
+
+class SurrogateReport(C.Structure):
+    _fields_ = [
+        ("trained", C.c_int32), ("enabled", C.c_int32), ("n_shapes", C.c_int32), ("n_params", C.c_int32),
+        ("iterations", C.c_int64), ("final_loss", C.c_double), ("max_error", C.c_double),
+        ("mean_error", C.c_double), ("band", C.c_double), ("train_ms", C.c_double), ("length_scale", C.c_double),
+        ("audits", C.c_int64), ("audited_contacts", C.c_int64), ("missed_contacts", C.c_int64),
+        ("reserved", C.c_int64 * 8),
+    ]
+
+
+SURROGATE_N_PARAM = 19777
 
 
 class Counters(C.Structure):
@@ -140,6 +162,7 @@ class Sim:
         c.pair_capacity = int(cfg.get("pair_capacity", 0))
         c.contact_capacity = int(cfg.get("contact_capacity", 0))
         c.device = int(cfg.get("device", 0))
+        c.no_sdf_cull = int(cfg.get("no_sdf_cull", 0))
         self.cfg = c
         self.ctx = C.c_void_p()
         self._call("create", C.byref(c), C.byref(self.ctx))
@@ -235,6 +258,54 @@ class Sim:
         v = DeviceView()
         self._call("get_device_view", self.ctx, C.byref(v))
         return v
+
+    # -- learned culling surrogate (GPU library only) --
+    def surrogate_train(self, **kw) -> dict:
+        p = SurrogateParams()
+        self.lib.lsdem_surrogate_params_default(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._call("surrogate_train", self.ctx, C.byref(p))
+        return self.surrogate_report()
+
+    def surrogate_enable(self, on: bool = True):
+        self._call("surrogate_enable", self.ctx, C.c_int32(int(on)))
+
+    def surrogate_set_band(self, band: float, audit_every: int):
+        self._call("surrogate_set_band", self.ctx, C.c_double(band), C.c_int32(audit_every))
+
+    def surrogate_report(self) -> dict:
+        r = SurrogateReport()
+        self._call("surrogate_get_report", self.ctx, C.byref(r))
+        return {k: getattr(r, k) for k, _ in SurrogateReport._fields_ if k != "reserved"}
+
+    def surrogate_weights(self, n_shapes: int) -> np.ndarray:
+        w = np.zeros(SURROGATE_N_PARAM + 16 * n_shapes, np.float32)
+        self._call("surrogate_get_weights", self.ctx, w.ctypes.data_as(C.c_void_p), C.c_int64(w.size))
+        return w
+
+    def surrogate_set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        self._call("surrogate_set_weights", self.ctx, w.ctypes.data_as(C.c_void_p), C.c_int64(w.size))
+
+    def surrogate_eval(self, points, shape_ids) -> np.ndarray:
+        p = np.ascontiguousarray(points, np.float32)
+        s = np.ascontiguousarray(shape_ids, np.int32)
+        out = np.zeros(len(s), np.float32)
+        self._call("surrogate_eval", self.ctx, C.c_int64(len(s)), p.ctypes.data_as(C.c_void_p),
+                   s.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def surrogate_gradient(self, points, shape_ids, targets, n_shapes: int):
+        p = np.ascontiguousarray(points, np.float32)
+        s = np.ascontiguousarray(shape_ids, np.int32)
+        t = np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(SURROGATE_N_PARAM + 16 * n_shapes, np.float32)
+        loss = C.c_double()
+        self._call("surrogate_gradient", self.ctx, C.c_int64(len(s)), p.ctypes.data_as(C.c_void_p),
+                   s.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p), g.ctypes.data_as(C.c_void_p),
+                   C.byref(loss))
+        return loss.value, g
 
     def time(self) -> float:
         t = C.c_double()

@@ -1,0 +1,110 @@
+"""Learned culling surrogate on the GPU (run on an MI355X).
+
+- forward and gradient of the BF16 MFMA kernels against the FP64 restatement
+  (oracle/surrogate_ref.py, with the same BF16 rounding points; differences come from FP32 vs FP64
+  accumulation and from features landing on the other side of a BF16 rounding boundary);
+- training on the method's own SDF grids lowers the error;
+- with the surrogate culling node clusters, every step's contact set, forces and torques are
+  identical to the exact path (the audits find no missed contact), and the surrogate prunes
+  clusters that bounding spheres alone keep (the gain over the method without it)."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from parity import compare_contacts, compare_state
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import surrogate_ref as R  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def flatten(P):
+    return np.concatenate([P["W1"].ravel(), P["b1"], P["W2"].ravel(), P["b2"], P["W3"].ravel(), P["b3"],
+                           P["W4"].ravel(), P["b4"], P["w5"], P["b5"], P["emb"].ravel()]).astype(np.float32)
+
+
+def f32params(P):
+    return {k: v.astype(np.float32).astype(np.float64) for k, v in P.items()}
+
+
+def small_scene(S):
+    return S.scene_c2(n=600, n_types=3, n_grid=24, n_nodes=128, cell_xy=0.06, gap=-0.15)
+
+
+def _sim_with_weights(L, S, seed=3):
+    sc = small_scene(S)
+    sim = sc.load(L.Sim(sc.cfg))
+    P = f32params(R.init_params(np.random.default_rng(seed), 3))
+    sim.surrogate_set_weights(flatten(P))
+    inv_len = 1.0 / sim.surrogate_report()["length_scale"]
+    return sc, sim, P, inv_len
+
+
+def _points(sc, rng, n):
+    p = rng.uniform(-1.0, 1.0, (n, 3)) * np.array([s["spacing"] * (s["sdf"].shape[0] - 1) * 0.5
+                                                   for s in sc.shapes[:3]]).max()
+    shape = rng.integers(0, 3, n).astype(np.int32)
+    return p.astype(np.float32).astype(np.float64), shape
+
+
+def test_forward_matches_oracle(L, S):
+    sc, sim, P, inv_len = _sim_with_weights(L, S)
+    p, shape = _points(sc, np.random.default_rng(7), 1000)
+    gpu = sim.surrogate_eval(p, shape).astype(np.float64) * inv_len
+    ref, _ = R.forward(P, p, shape, np.float32(inv_len), bf=True)
+    err = np.abs(gpu - ref)
+    print("forward max err", err.max(), "scale", np.abs(ref).max())
+    assert err.max() <= 5e-3 * np.abs(ref).max() + 1e-4
+
+
+def test_gradient_matches_oracle(L, S):
+    sc, sim, P, inv_len = _sim_with_weights(L, S, seed=4)
+    rng = np.random.default_rng(8)
+    p, shape = _points(sc, rng, 256)
+    t = rng.normal(0, 0.3, len(p)).astype(np.float32).astype(np.float64)
+    loss_g, g = sim.surrogate_gradient(p, shape, t, 3)
+    loss_r, G, _ = R.loss_and_grads(P, p, shape, t, np.float32(inv_len), bf=True)
+    assert loss_g == pytest.approx(loss_r, rel=2e-3)
+    ref = flatten(G).astype(np.float64)
+    off = 0
+    for name, size in (("W1", 112 * 64), ("b1", 64), ("W2", 4096), ("b2", 64), ("W3", 4096), ("b3", 64),
+                       ("W4", 4096), ("b4", 64), ("w5", 64), ("b5", 1), ("emb", 48)):
+        a, b = g[off:off + size].astype(np.float64), ref[off:off + size]
+        e = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        print(name, "rel err", e)
+        assert e < 2e-2, name
+        off += size
+
+
+def test_training_and_exact_culling(L, S):
+    sc = small_scene(S)
+    ref = sc.load(L.Sim(sc.cfg))
+    sur = sc.load(L.Sim(sc.cfg))
+    none = sc.load(L.Sim({**sc.cfg, "no_sdf_cull": 1}))
+    rep = sur.surrogate_train(iterations=800, batch=2048, samples=300_000, validation=65_536, audit_every=10)
+    print("trained", rep)
+    assert rep["trained"] and np.isfinite(rep["max_error"])
+    assert rep["max_error"] < 0.25 * rep["length_scale"]
+    assert rep["final_loss"] < 0.01
+    sur.surrogate_enable(True)
+    work = {}
+    for name, sim in (("exact", ref), ("surrogate", sur), ("spheres", none)):
+        sim.profile_begin(40)
+    for s in range(40):
+        for sim in (ref, sur, none):
+            sim.step(1)
+        for l in (0, 1, 2, 3, 4):
+            ca = sur.contacts(l)
+            compare_contacts(ca, ref.contacts(l), f"step {s} list {l} (surrogate)")
+            compare_contacts(none.contacts(l), ref.contacts(l), f"step {s} list {l} (spheres only)")
+        compare_state(sur.get_particles(), ref.get_particles(), ftol=1e-12)
+    for name, sim in (("exact", ref), ("surrogate", sur), ("spheres", none)):
+        work[name] = sim.profile_end()["cluster_tests"]
+    r = sur.surrogate_report()
+    print("cluster entries", work, "audits", r["audits"], r["audited_contacts"], r["missed_contacts"])
+    assert r["audits"] >= 4 and r["audited_contacts"] > 0
+    assert r["missed_contacts"] == 0
+    assert work["surrogate"] < work["spheres"]
