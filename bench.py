@@ -1,20 +1,27 @@
-"""Benchmark: particle-steps/s of the LS-DEM timestep at 200k particles (config 2) on one MI355X.
+"""Benchmark: particle-steps/s of the LS-DEM timestep at 200k particles on one MI355X, plain path
+and surrogate-culled path on the same GPU.
 
-Workload: 200,000 superquadrics of 10 types (exponents U[2,8], axis ratios (1, U[.5,1],
-U[.5,1]), equivalent radius lognormal(median 3 mm, sigma 0.25)), FP32 SDF grids of 50^3 and 512
-nodes per type, periodic x/y on a 0.3 m cell with a floor wall, linear spring-dashpot
-(kn 1e5, kt 0.4 kn, e 0.5, mu 0.5), dt 5e-6.  Synthetic random-lattice start (seeded).
-W warm-up steps, then K timed steps; the host clock brackets the timed steps (lsdem_step
-returns after the stream is drained).  Per-phase GPU times come from HIP events recorded on
-the library's stream inside the timed region.
+Workload (200k particles, config 2 of the benchmark set): 200,000 superquadrics of 10 types
+(exponents U[2,8], axis ratios (1, U[.5,1], U[.5,1]), equivalent radius lognormal(median 3 mm,
+sigma 0.25)), FP32 SDF grids of 50^3 and 512 nodes per type, periodic x/y on a 0.3 m cell with a
+floor wall, linear spring-dashpot (kn 1e5, kt 0.4 kn, e 0.5, mu 0.5), dt 5e-6, seeded
+random-lattice start (layer pitch 0.88 x the layer's largest bounding diameter, orientations
+resampled until no two particles overlap).  W warm-up steps, then K timed steps; the host clock
+brackets the timed steps (lsdem_step returns after the stream is drained).  Per-phase GPU times
+come from HIP events recorded on the library's stream inside the timed region.
 
-The roofline entry is for the dominant kernel, the narrow phase (k_narrow).  Its algorithmic
-bytes per launch are counted from the step's own work counters:
-    per candidate pair  B_PAIR   bytes (pair record, both bodies' pose/velocity/material, sums)
-    per node tested     B_NODE   bytes (node record)
-    per SDF query       B_QUERY  bytes (8 FP32 corners)
-    per contact         B_CONTACT bytes (history probe + contact record write)
-The CPU baseline is the oracle (serial C++, 1 core) on a bounded sample of the same workload:
+Surrogate path: a second context from the same initial state trains the culling network on the
+GPU (not timed in the step rate; reported), then runs W + K steps with the network culling node
+clusters and an exact audit every 100 steps (audits are inside the timed steps).
+
+Roofline (the narrow gather phase: k_pair_maps + k_candidates + k_exact_contacts, one launch
+each per step).  Algorithmic bytes per step, 4 B per FP32/int32 scalar:
+    P x 18 x 4   candidate pairs (pair record, both poses, map)
+  + Q x 3 x 4    boundary nodes of the masters (a node-by-node scan reads every one)
+  + S x 8 x 4    SDF queries (8 corners)
+  + C x 16 x 4   contacts written and history probes
+with P, Q, S, C counted by the kernels.  achieved = bytes / narrow-phase time; peak 8 TB/s.
+The CPU baseline is the oracle (serial C++, FP64, 1 core) on a bounded sample of the same workload:
 the same generator at fewer particles with the periodic cell shrunk to keep the density.
 """
 from __future__ import annotations
@@ -32,11 +39,12 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-B_PAIR = 8 + 2 * 232 + 76
-B_NODE = 32
-B_QUERY = 32
-B_CONTACT = 36 + 104
+B_PAIR = 18 * 4
+B_NODE = 3 * 4
+B_QUERY = 8 * 4
+B_CONTACT = 16 * 4
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 
 
 def log(msg):
@@ -44,36 +52,68 @@ def log(msg):
 
 
 def narrow_bytes(prof: dict) -> float:
-    return (B_PAIR * prof["pairs"] + B_NODE * prof["node_tests"] + B_QUERY * prof["sdf_queries"] +
+    return (B_PAIR * prof["pairs"] + B_NODE * prof["master_nodes"] + B_QUERY * prof["sdf_queries"] +
             B_CONTACT * prof["contacts"])
 
 
-def cpu_baseline(S, n_full: int, n_sample: int, steps: int, seed: int):
+def cpu_baseline(S, n_full: int, n_sample: int, steps: int, seed: int, gap: float):
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_binding import oracle_sim  # the checker, timed here as the CPU baseline
     cell = 0.3 * float(np.sqrt(n_sample / n_full))
-    sc = S.scene_c2(n=n_sample, cell_xy=cell, seed=seed)
+    sc = S.scene_c2(n=n_sample, cell_xy=cell, seed=seed, gap=gap)
+    S.remove_initial_overlaps(lambda c: oracle_sim(c), sc, log=lambda m: None)
     sim = sc.load(oracle_sim(sc.cfg))
     sim.step(1)  # first step builds the history; time the following ones
     t0 = time.perf_counter()
     sim.step(steps)
     dt = time.perf_counter() - t0
     return {"value": n_sample * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++, FP64), C2 generator at {n_sample} particles, periodic cell "
-                      f"{cell:.4f} m (same density), {steps} steps after 1 untimed step, {dt:.1f} s",
+            "sample": f"oracle (serial C++, FP64, pair broad phase), same generator at {n_sample} particles in a "
+                      f"{cell:.4f} m periodic cell (same density), {steps} steps after 1 untimed step, {dt:.1f} s",
             "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+
+
+def run(L, sc, caps, warmup, steps, label, surrogate=None):
+    sim = sc.load(L.Sim({**sc.cfg, **caps}))
+    extra = {}
+    if surrogate is not None:
+        t = time.perf_counter()
+        rep = sim.surrogate_train(**surrogate)
+        extra["train"] = rep
+        log(f"[bench] {label}: trained in {time.perf_counter() - t:.1f}s (GPU {rep['train_ms']:.0f} ms): "
+            f"loss {rep['final_loss']:.2e} max err {rep['max_error']:.3e} band {rep['band']:.3e}")
+        sim.surrogate_enable(True)
+    done = 0
+    t = time.perf_counter()
+    while done < warmup:
+        k = min(1000, warmup - done)
+        sim.step(k)
+        done += k
+        c = sim.counters()
+        log(f"[bench] {label} warmup {done}/{warmup} ({time.perf_counter() - t:.1f}s) pairs={c['pairs']} "
+            f"contacts={c['contacts']} node_tests={c['node_tests']}")
+    sim.profile_begin(steps)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    t1 = time.perf_counter()
+    prof = sim.profile_end()
+    if surrogate is not None:
+        extra["report"] = sim.surrogate_report()
+    return sim, t1 - t0, prof, extra
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=5000)
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--cpu-sample", type=int, default=5_000)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-iters", type=int, default=3000)
     ap.add_argument("--gap", type=float, default=-0.12,
                     help="lattice pitch relative to the layer's largest bounding diameter, minus 1 "
                          "(negative: denser than bounding spheres; overlapping orientations are resampled)")
@@ -90,49 +130,56 @@ def main():
 
     t = time.perf_counter()
     sc = S.scene_c2(n=args.n, seed=args.seed, gap=args.gap)
-    log(f"[bench] scene C2 n={args.n} built in {time.perf_counter() - t:.1f}s; domain {sc.cfg['domain_max']}")
+    log(f"[bench] scene C2 n={args.n} built in {time.perf_counter() - t:.1f}s")
     caps = {"pair_capacity": 48 * args.n + 65536, "contact_capacity": 48 * args.n + 65536}
     if args.gap < 0:
         t = time.perf_counter()
         moved = S.remove_initial_overlaps(lambda c: L.Sim({**c, **caps}), sc, log=log)
         log(f"[bench] initial overlaps removed ({moved} relocated, {time.perf_counter() - t:.1f}s); "
-            f"domain {sc.cfg['domain_max']}")
-    cfg = {**sc.cfg, **caps}
-    sim = sc.load(L.Sim(cfg))
-    done = 0
-    t = time.perf_counter()
-    while done < args.warmup:
-        k = min(500, args.warmup - done)
-        sim.step(k)
-        done += k
-        c = sim.counters()
-        log(f"[bench] warmup {done}/{args.warmup} ({time.perf_counter() - t:.1f}s) pairs={c['pairs']} "
-            f"contacts={c['contacts']} node_tests={c['node_tests']}")
-    sim.profile_begin(args.steps)
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    t1 = time.perf_counter()
-    prof = sim.profile_end()
-    wall = t1 - t0
+            f"domain {tuple(float(v) for v in sc.cfg['domain_max'])}")
+
+    sim, wall, prof, _ = run(L, sc, caps, args.warmup, args.steps, "plain")
     ms_step = 1e3 * wall / args.steps
     value = args.n * args.steps / wall
     K = max(prof["steps"], 1)
     narrow_ms = prof["ms"]["narrow"] / K
-    per_launch_bytes = narrow_bytes(prof) / K
-    achieved = per_launch_bytes / (narrow_ms * 1e-3) / 1e9
+    per_step_bytes = narrow_bytes(prof) / K
+    achieved = per_step_bytes / (narrow_ms * 1e-3) / 1e9
     gpu_ms = sum(prof["ms"].values()) / K
-    log(f"[bench] timed {args.steps} steps in {wall:.2f}s ({ms_step:.3f} ms/step, GPU phases {gpu_ms:.3f} ms/step)")
+    log(f"[bench] plain: {args.steps} steps in {wall:.2f}s ({ms_step:.3f} ms/step, GPU phases {gpu_ms:.3f} ms/step)")
     log("[bench] phases ms/step: " + " ".join(f"{k}={v / K:.4f}" for k, v in prof["ms"].items()))
-    log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} cluster_tests={prof['cluster_tests'] / K:.0f} "
-        f"node_tests={prof['node_tests'] / K:.0f} "
+    log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} master_nodes={prof['master_nodes'] / K:.0f} "
+        f"cluster_tests={prof['cluster_tests'] / K:.0f} node_tests={prof['node_tests'] / K:.0f} "
         f"sdf_queries={prof['sdf_queries'] / K:.0f} contacts={prof['contacts'] / K:.0f}")
+    del sim
+
+    sur = None
+    if not args.no_surrogate:
+        ssim, swall, sprof, extra = run(L, sc, caps, args.warmup, args.steps, "surrogate",
+                                        surrogate={"iterations": args.sur_iters})
+        sK = max(sprof["steps"], 1)
+        rep = extra["report"]
+        sur = {"value": args.n * args.steps / swall, "ms_per_step": 1e3 * swall / args.steps,
+               "speedup_vs_plain": (args.n * args.steps / swall) / value,
+               "narrow_ms_per_step": sprof["ms"]["narrow"] / sK,
+               "cluster_tests_per_step": sprof["cluster_tests"] / sK,
+               "plain_cluster_tests_per_step": prof["cluster_tests"] / K,
+               "train_ms": rep["train_ms"], "train_iterations": rep["iterations"], "final_loss": rep["final_loss"],
+               "max_error": rep["max_error"], "band": rep["band"], "length_scale": rep["length_scale"],
+               "audits": rep["audits"], "audited_contacts": rep["audited_contacts"],
+               "missed_contacts": rep["missed_contacts"]}
+        log(f"[bench] surrogate: {sur['ms_per_step']:.3f} ms/step ({sur['speedup_vs_plain']:.2f}x plain), "
+            f"audits {rep['audits']} contacts {rep['audited_contacts']} missed {rep['missed_contacts']}")
+        log("[bench] surrogate phases ms/step: " + " ".join(f"{k}={v / sK:.4f}" for k, v in sprof["ms"].items()))
+        del ssim
+
     base = None
     if not args.no_cpu:
         t = time.perf_counter()
-        base = cpu_baseline(S, args.n, args.cpu_sample, args.cpu_steps, args.seed)
+        base = cpu_baseline(S, args.n, args.cpu_sample, args.cpu_steps, args.seed, args.gap)
         log(f"[bench] cpu baseline {base['value']:.1f} particle-steps/s ({time.perf_counter() - t:.1f}s)")
     out = {
-        "metric": "particle-steps/s at 200k LS-DEM particles (plain path)",
+        "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -143,18 +190,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random-lattice start of config 2)",
-        "config": {"workload": "C2: 200k superquadrics, 10 types, SDF 50^3 FP32, 512 nodes, periodic x/y 0.3 m "
-                               "+ floor, dt 5e-6", "particles": args.n, "seed": args.seed, "lattice_gap": args.gap},
-        "roofline": {"bound": "hbm", "kernel": "k_narrow", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": per_launch_bytes, "ms_per_launch": narrow_ms},
+        "data": "synthetic (seeded random-lattice start, 200k superquadrics of 10 types)",
+        "config": {"workload": "200k superquadrics (10 types, SDF 50^3 FP32, 512 nodes), periodic x/y 0.3 m "
+                               "+ floor, dt 5e-6; value = plain path", "particles": args.n, "seed": args.seed,
+                   "lattice_gap": args.gap},
+        "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_maps + k_candidates + k_exact_contacts)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": per_step_bytes, "ms_per_launch": narrow_ms},
         "cpu_baseline": base,
+        "surrogate": sur,
         "phases_ms_per_step": {k: v / K for k, v in prof["ms"].items()},
-        "work_per_step": {k: prof[k] / K for k in ("pairs", "cluster_tests", "node_tests", "sdf_queries",
-                                                    "contacts")},
+        "work_per_step": {k: prof[k] / K for k in ("pairs", "master_nodes", "cluster_tests", "node_tests",
+                                                    "sdf_queries", "contacts")},
         "node_sdf_queries_per_s": prof["sdf_queries"] / K / (ms_step * 1e-3),
-        "surrogate": None,
     }
     print(json.dumps(out))
 

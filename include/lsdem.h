@@ -162,7 +162,8 @@ typedef struct lsdem_profile {
   double ms[LSDEM_NUM_PHASES]; /* summed GPU time per phase */
   int64_t pairs, node_tests, sdf_queries, contacts; /* sums over the profiled steps */
   int64_t cluster_tests;       /* node clusters passed to the node prefilter, summed */
-  int64_t reserved[7];
+  int64_t master_nodes;        /* boundary nodes of the masters of all candidate pairs, summed */
+  int64_t reserved[6];
 } lsdem_profile;
 /* Start recording; at most max_steps steps are timed (events are preallocated). */
 int lsdem_profile_begin(lsdem_ctx* ctx, int32_t max_steps);

@@ -878,7 +878,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dHashKey.alloc(hs));
   HIPCHK(c->dHashVal.alloc(hs));
   HIPCHK(c->dHashSlot.alloc(contactCap));
-  HIPCHK(c->dStatSlots.alloc(4 * kStatSlots));
+  HIPCHK(c->dStatSlots.alloc(5 * kStatSlots));
   P.hashMask = (unsigned long long)(hs - 1);
 
   // ---- bonds (+ per-particle CSR of bond ends, in bond order) ----
@@ -911,10 +911,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
   HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
   HIPCHK(c->dDevInt.alloc(8));
-  HIPCHK(c->dDevStat.alloc(12));
-  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 4 * kStatSlots * sizeof(unsigned long long), s));
+  HIPCHK(c->dDevStat.alloc(13));
+  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 5 * kStatSlots * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, 8 * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 12 * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 13 * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
@@ -1144,6 +1144,7 @@ int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemset(c->dDevStat.p + 3, 0, 6 * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(c->dDevStat.p + 12, 0, sizeof(unsigned long long)));
   c->profMax = max_steps;
   c->profSteps = 0;
   c->profiling = true;
@@ -1165,9 +1166,10 @@ int lsdem_profile_end(lsdem_ctx* c, lsdem_profile* out) {
       out->ms[ph] += ms;
     }
   }
-  unsigned long long ds[9];
+  unsigned long long ds[13];
   HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
   out->cluster_tests = (int64_t)ds[8];
+  out->master_nodes = (int64_t)ds[12];
   out->pairs = (int64_t)ds[3];
   out->node_tests = (int64_t)ds[4];
   out->sdf_queries = (int64_t)ds[5];

@@ -175,9 +175,9 @@ struct Params {
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
   //          (summed while profiling); [9] audits, [10] audited contacts, [11] missed contacts
-  //          (cumulative)
+  //          (cumulative); [12] master nodes of the candidate pairs (summed while profiling)
   unsigned long long* devStat;
-  unsigned long long* statSlots;  // kStatSlots x 4 spread accumulators of the narrow phase
+  unsigned long long* statSlots;  // kStatSlots x 5 spread accumulators of the narrow phase
 
   // ---- learned culling surrogate ----
   SurNet sur;              // network (valid when surSlot != nullptr)

@@ -750,12 +750,16 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
 __global__ void k_pair_maps(Params P) {
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
   PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
+  unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nPairs; p += gridDim.x * blockDim.x) {
     const PairView v = loadPair(P, p);
     PairF32 f;
     makeF32(P, v, f);
     out[p] = f;
+    masterNodes += (unsigned long long)P.shapes[P.shape[v.A]].nNodes;
   }
+  const unsigned long long t = (unsigned long long)wsum(double(masterNodes));
+  if ((threadIdx.x & 63) == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
 }
 
 __device__ __forceinline__ void mapF32(const PairF32& f, float nx, float ny, float nz, float& gx, float& gy, float& gz) {
@@ -1224,7 +1228,7 @@ __global__ void k_step_begin(Params P) {
   if (threadIdx.x == 0) {
     P.devInt[1] = 0;  // contact slots allocated
   }
-  for (int k = threadIdx.x; k < 4 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
+  for (int k = threadIdx.x; k < 5 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
 }
 __global__ void k_step_end(Params P) {
   if (threadIdx.x == 0) {
@@ -1232,13 +1236,15 @@ __global__ void k_step_end(Params P) {
     if ((long long)np > P.pairCap) P.devInt[3] |= 1;
     P.devInt[4] = max(P.devInt[4], np);
     P.devInt[5] = max(P.devInt[5], P.devInt[1]);
-    unsigned long long t = 0, q = 0, c = 0, e = 0;
+    unsigned long long t = 0, q = 0, c = 0, e = 0, mn = 0;
     for (int k = 0; k < kStatSlots; ++k) {
       t += P.statSlots[k];
       q += P.statSlots[kStatSlots + k];
       c += P.statSlots[2 * kStatSlots + k];
       e += P.statSlots[3 * kStatSlots + k];
+      mn += P.statSlots[4 * kStatSlots + k];
     }
+    P.devStat[12] += mn;
     P.devStat[0] = t;
     P.devStat[1] = q;
     P.devStat[2] = c;

@@ -90,7 +90,8 @@ class Profile(C.Structure):
         ("sdf_queries", C.c_int64),
         ("contacts", C.c_int64),
         ("cluster_tests", C.c_int64),
-        ("reserved", C.c_int64 * 7),
+        ("master_nodes", C.c_int64),
+        ("reserved", C.c_int64 * 6),
     ]
 
 
@@ -368,7 +369,8 @@ class Sim:
         p = Profile()
         self._call("profile_end", self.ctx, C.byref(p))
         return {"steps": p.steps, "ms": dict(zip(PHASES, list(p.ms))), "pairs": p.pairs, "node_tests": p.node_tests,
-                "sdf_queries": p.sdf_queries, "contacts": p.contacts, "cluster_tests": p.cluster_tests}
+                "sdf_queries": p.sdf_queries, "contacts": p.contacts, "cluster_tests": p.cluster_tests,
+                "master_nodes": p.master_nodes}
 
     def counters(self) -> dict:
         c = Counters()
