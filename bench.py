@@ -114,10 +114,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sur-iters", type=int, default=3000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB ten times (HBM byte-counter calibration)")
     ap.add_argument("--gap", type=float, default=-0.12,
                     help="lattice pitch relative to the layer's largest bounding diameter, minus 1 "
                          "(negative: denser than bounding spheres; overlapping orientations are resampled)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.calibrate_copy} MiB x 10")
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))

@@ -38,6 +38,7 @@ __global__ void k_gather_images(Params P, int set);
 __global__ void k_pair_maps(Params P);
 template <bool SUR> __global__ void k_candidates(Params P);
 __global__ void k_sur_audit(Params P);
+__global__ void k_sur_masks(Params P);
 size_t candidatesLdsBytes(int maxClusters);
 __global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
@@ -212,6 +213,7 @@ struct lsdem_ctx {
   DBuf<ClusterDev> dClusters;
   DBuf<NodeDev> dNodes;
   DBuf<float4> dNodes32, dPairMap;
+  DBuf<unsigned long long> dSurMask;
   DBuf<ClusterF32> dClusters32;
   DBuf<int> dNodeCluster;
   // learned culling surrogate
@@ -285,7 +287,7 @@ struct lsdem_ctx {
 };
 
 void lsdem_ctx::releaseAll() {
-  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dPairMap.release(); dClusters32.release(); dSdf.release();
+  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dPairMap.release(); dSurMask.release(); dClusters32.release(); dSdf.release();
   dCPair.release(); dCF.release(); dCTA.release(); dCTB.release();
   dPos.release(); dPosDet.release(); dVel.release(); dAng.release(); dForce.release(); dTorque.release();
   dGorig.release(); dQuat.release(); dInvI.release();
@@ -454,6 +456,7 @@ int runStep(lsdem_ctx* c) {
   hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
   const bool useSur = c->sur.enabled && c->sur.trained;
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
+  if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
   if (useSur && !audit)
     hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
                        unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
@@ -532,7 +535,8 @@ int lsdem_create(const lsdem_config* cfg, lsdem_ctx** out) {
   HIPCHK(hipSetDevice(cfg->device));
   lsdem_ctx* c = new lsdem_ctx();
   c->cfg = *cfg;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  // a blocking stream: work an external-force hook puts on the null stream stays ordered with the step
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return fail(LSDEM_ERR_HIP, "hipStreamCreate failed");
   }
@@ -853,6 +857,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairCoff.alloc(pairCap));
   HIPCHK(c->dPairCcnt.alloc(pairCap));
   HIPCHK(c->dPairMap.alloc(pairCap * 10));
+  HIPCHK(c->dSurMask.alloc(pairCap));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
@@ -941,7 +946,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   for (int g = 0; g < 3; ++g) P.gsPR[g] = c->dGSPR[g].p;
   P.hashSlot = c->dHashSlot.p; P.statSlots = c->dStatSlots.p;
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
-  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p;
+  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p; P.surMask = c->dSurMask.p;
   P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
   P.rcTmp = c->dRcTmp.p;
   P.hashKey = c->dHashKey.p; P.hashVal = c->dHashVal.p;
@@ -1372,6 +1377,7 @@ int surMeasure(lsdem_ctx* c, DBuf<float>& x, DBuf<int>& slot, DBuf<float>& t, lo
   hipLaunchKernelGGL(k_sur_eval, dim3(unsigned(c->numCU * 2)), dim3(256), 0, c->stream, surNet(c), D, (float*)nullptr,
                      S.errBits.p, S.errSum.p);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
   unsigned int bits = 0;
   float sum = 0.f;
   HIPCHK(hipMemcpy(&bits, S.errBits.p, sizeof bits, hipMemcpyDeviceToHost));

@@ -184,6 +184,7 @@ struct Params {
   const int* surSlot;      // family slot of each shape id, -1 outside the family; nullptr: untrained
   float surBand;           // cull band, unscaled SDF units (2 x measured max error)
   const int* nodeCluster;  // per shape node (caller order, at the shape's node offset): its cluster
+  unsigned long long* surMask;  // per candidate pair: bit c = 0 when the network culls cluster c
 };
 
 constexpr int kStatSlots = 32;

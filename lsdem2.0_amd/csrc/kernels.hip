@@ -914,26 +914,19 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
           cCount = cl.count;
         }
       } else {
-        // the wave's 64 (pair, cluster) centres go through the network together
-        float gx = 0.f, gy = 0.f, gz = 0.f, rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
-        bool inside = false;
-        int qslot = -1;
+        // the network's decisions for this pair's clusters come from k_sur_masks
         if (own && c < nCl) {
           const ClusterF32 cl = P.clusters32[clusterOff + c];
+          float gx, gy, gz, rho;
+          bool inside;
           keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
           cFirst = nodeOff + cl.first;
           cCount = cl.count;
           if (keep && inside) {
-            if (f.slot >= 0) {
-              qslot = f.slot;
-              surQueryPoint(P, f, gx, gy, gz, ux, uy, uz);
-            } else {
-              keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho);
-            }
+            if (f.slot >= 0) keep = (P.surMask[p] >> c) & 1ull;
+            else keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho);
           }
         }
-        const float d = surInfer64(P.sur, ux, uy, uz, qslot, lane);
-        if (qslot >= 0) keep = clusterSurKeep(P, f, d, rho);
       }
       const unsigned long long m = __ballot(keep);
       if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(cFirst, cCount | (j << 16));
@@ -1006,10 +999,52 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   }
 }
 
+// Surrogate cluster decisions, one wavefront per batch of 8 candidate pairs (lanes = pair x
+// cluster, as stage 1 of k_candidates): bit c of surMask[p] is 0 when the network culls cluster c
+// of pair p.  Only clusters whose centre lies in a family partner's grid box are queried, packed
+// 16 per network pass.
+__global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
+  __shared__ int sQ[kNarrowBlock / 64][64];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
+  const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
+  const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
+  const int iters = (P.maxClusters + 7) >> 3;
+  const int j = lane >> 3, k = lane & 7;
+  for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
+    const int p = base + j;
+    const bool own = p < nPairs;
+    PairF32 f;
+    if (own) f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
+    unsigned long long mask = ~0ull;
+    for (int it = 0; it < iters; ++it) {
+      const int c = it * 8 + k;
+      float gx = 0.f, gy = 0.f, gz = 0.f, rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+      int qslot = -1;
+      if (own && c < f.nClusters && f.slot >= 0) {
+        const ClusterF32 cl = P.clusters32[f.clusterOff + c];
+        bool inside;
+        if (clusterGeomF32(f, cl, gx, gy, gz, rho, inside) && inside) {
+          qslot = f.slot;
+          surQueryPoint(P, f, gx, gy, gz, ux, uy, uz);
+        }
+      }
+      const float d = surInferPacked(P.sur, ux, uy, uz, qslot, lane, sQ[wl]);
+      const bool cull = qslot >= 0 && !clusterSurKeep(P, f, d, rho);
+      // collect this group's 8 decisions of pair j into its mask
+      const unsigned long long b = __ballot(cull);
+      const unsigned long long mine = (b >> (8 * j)) & 0xFFull;
+      mask &= ~(mine << (8 * it));
+    }
+    if (own && k == 0) P.surMask[p] = mask;
+  }
+}
+
 // Exact audit of the surrogate: every contact of this (plain) step whose partner is in the
 // family must lie in a cluster that the surrogate keeps; a culled one is a missed contact.
 // One lane per contact slot, 64 per wave through the network.
 __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
+  __shared__ int sQ[kNarrowBlock / 64][64];
   const int lane = threadIdx.x & 63;
   const int n = min(P.devInt[1], (int)P.contactCap);
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
@@ -1034,7 +1069,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
         checked++;
       }
     }
-    const float d = surInfer64(P.sur, ux, uy, uz, q ? f.slot : -1, lane);
+    const float d = surInferPacked(P.sur, ux, uy, uz, q ? f.slot : -1, lane, sQ[threadIdx.x >> 6]);
     if (q && !clusterSurKeep(P, f, d, rho)) missed++;
   }
   const unsigned long long tc = (unsigned long long)wsum(double(checked));

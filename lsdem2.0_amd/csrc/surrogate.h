@@ -229,4 +229,30 @@ __device__ __forceinline__ float surInfer64(const SurNet& N, float ux, float uy,
   return shape < 0 ? 0.f : mine;
 }
 
+
+// Inference for the lanes of the wave that hold a query (shape >= 0), packed 16 per pass: only
+// ceil(queries / 16) passes run.  `idx` is 64 ints of the wave's LDS.  Lanes without a query get 0.
+__device__ __forceinline__ float surInferPacked(const SurNet& N, float ux, float uy, float uz, int shape, int lane,
+                                                int* idx) {
+  const bool q = shape >= 0;
+  const unsigned long long m = __ballot(q);
+  const int n = __popcll(m);
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int rank = __popcll(m & lt);
+  if (q) idx[rank] = lane;
+  __builtin_amdgcn_wave_barrier();
+  float mine = 0.f;
+  for (int pass = 0; pass * 16 < n; ++pass) {
+    const int r = 16 * pass + (lane & 15);
+    const int src = r < n ? idx[r] : 0;
+    const float sx = __shfl(ux, src, 64), sy = __shfl(uy, src, 64), sz = __shfl(uz, src, 64);
+    const int ss = __shfl(shape, src, 64);
+    const float o = surForwardInfer(N, sx, sy, sz, (r < n && ss >= 0) ? ss : 0, lane);
+    const float got = __shfl(o, rank & 15, 64);
+    if (q && (rank >> 4) == pass) mine = got;
+  }
+  __builtin_amdgcn_wave_barrier();
+  return mine;
+}
+
 }  // namespace lsd
