@@ -44,6 +44,9 @@ B_NODE = 3 * 4
 B_QUERY = 8 * 4
 B_CONTACT = 16 * 4
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per step of the narrow phase, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a
+# 256 MiB copy (profiles/c2_200k_narrow_pmc_traffic.md); None when the kernels changed since.
+NARROW_TRAFFIC_BYTES = 698.1e6
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 
 
@@ -56,12 +59,13 @@ def narrow_bytes(prof: dict) -> float:
             B_CONTACT * prof["contacts"])
 
 
-def cpu_baseline(S, n_full: int, n_sample: int, steps: int, seed: int, gap: float):
+def cpu_baseline(L, S, n_full: int, n_sample: int, steps: int, seed: int, gap: float):
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_binding import oracle_sim  # the checker, timed here as the CPU baseline
     cell = 0.3 * float(np.sqrt(n_sample / n_full))
     sc = S.scene_c2(n=n_sample, cell_xy=cell, seed=seed, gap=gap)
-    S.remove_initial_overlaps(lambda c: oracle_sim(c), sc, log=lambda m: None)
+    if gap < 0:  # set-up on the GPU library (not timed); only the oracle's steps are timed
+        S.remove_initial_overlaps(lambda c: L.Sim(c), sc, log=lambda m: None)
     sim = sc.load(oracle_sim(sc.cfg))
     sim.step(1)  # first step builds the history; time the following ones
     t0 = time.perf_counter()
@@ -187,7 +191,7 @@ def main():
     base = None
     if not args.no_cpu:
         t = time.perf_counter()
-        base = cpu_baseline(S, args.n, args.cpu_sample, args.cpu_steps, args.seed, args.gap)
+        base = cpu_baseline(L, S, args.n, args.cpu_sample, args.cpu_steps, args.seed, args.gap)
         log(f"[bench] cpu baseline {base['value']:.1f} particle-steps/s ({time.perf_counter() - t:.1f}s)")
     out = {
         "metric": METRIC,
@@ -207,7 +211,8 @@ def main():
                    "lattice_gap": args.gap},
         "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_maps + k_candidates + k_exact_contacts)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": per_step_bytes, "ms_per_launch": narrow_ms},
+                     "traffic": NARROW_TRAFFIC_BYTES, "bytes_per_launch": per_step_bytes, "ms_per_launch": narrow_ms,
+                     "traffic_source": "profiles/c2_200k_narrow_pmc_traffic.md"},
         "cpu_baseline": base,
         "surrogate": sur,
         "phases_ms_per_step": {k: v / K for k, v in prof["ms"].items()},
