@@ -34,7 +34,6 @@ __global__ void k_rc_scatter(Params P);
 __global__ void k_hash_insert(Params P);
 __global__ void k_hash_clear(Params P);
 __global__ void k_gather_particles(Params P);
-__global__ void k_gather_images(Params P, int set);
 __global__ void k_pair_maps(Params P);
 template <bool SUR> __global__ void k_candidates(Params P);
 __global__ void k_sur_audit(Params P);
@@ -248,9 +247,10 @@ struct lsdem_ctx {
   DBuf<double> dWMu, dWRes;
   DBuf<int> dWShape, dWKey;
   DBuf<d3> dGPos[3];
-  DBuf<int> dGKey[3], dGCell[3], dGCellStart[3], dGSorted[3], dGCount[3], dGRank[3];
+  DBuf<int> dGKey[3];
   DBuf<int> dCellCount, dCellStart, dSorted, dRank;
-  DBuf<float4> dSPR, dSPRw, dGSPR[3];
+  DBuf<float4> dSPR, dSPRw;
+  DBuf<float> dSBoxR;
   DBuf<uint32_t> dPairSlab;
   DBuf<int> dPairCount, dPairStart, dPairA, dPairCoff, dPairCcnt, dRcCount, dRcStart, dRcPair, dRcRank, dRcTmp;
   DBuf<int> dSortTmp, dSortBig;
@@ -296,12 +296,11 @@ void lsdem_ctx::releaseAll() {
   dWPos.release(); dWVel.release(); dWAng.release(); dWQuat.release(); dWMu.release(); dWRes.release();
   dWShape.release(); dWKey.release();
   for (int s = 0; s < 3; ++s) {
-    dGPos[s].release(); dGKey[s].release(); dGCell[s].release(); dGCellStart[s].release();
-    dGSorted[s].release(); dGCount[s].release(); dGRank[s].release();
+    dGPos[s].release(); dGKey[s].release();
   }
   dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
   dSPR.release(); dSPRw.release(); dPairSlab.release(); dHashSlot.release(); dStatSlots.release();
-  for (int g = 0; g < 3; ++g) dGSPR[g].release();
+  dSBoxR.release();
   dPairCount.release(); dPairStart.release(); dPairA.release(); dPairCoff.release(); dPairCcnt.release();
   dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release(); dRcTmp.release();
   dSortTmp.release(); dSortBig.release();
@@ -427,13 +426,6 @@ int runStep(lsdem_ctx* c) {
   int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
-  for (int g = 0; g < 3; ++g) {
-    const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
-    if (!on) continue;
-    rc = bucketSort(c, P.gcell[g], P.nP, P.sg.ncell, P.gCount[g], P.gCellStart[g], c->dGRank[g].p, P.gSorted[g]);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_gather_images, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P, g);
-  }
   phase();
   hipLaunchKernelGGL(k_pairs_find, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
   rc = scan(c, P.pairCount, P.nP, P.pairStart);
@@ -825,6 +817,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dRank.alloc(std::max(nP, 1)));
   HIPCHK(c->dSPR.alloc(std::max(nP, 1)));
   HIPCHK(c->dSPRw.alloc(std::max(nP, 1)));
+  HIPCHK(c->dSBoxR.alloc(std::max(nP, 1)));
   HIPCHK(c->dPairSlab.alloc(size_t(std::max(nP, 1)) * kPairSlabHost));
   const bool periodic = P.periodicX || P.periodicY;
   for (int g = 0; g < 3; ++g) {
@@ -832,12 +825,6 @@ int lsdem_initialize(lsdem_ctx* c) {
     const size_t n = on ? size_t(std::max(nP, 1)) : 1;
     HIPCHK(c->dGPos[g].alloc(n));
     HIPCHK(c->dGKey[g].alloc(n));
-    HIPCHK(c->dGCell[g].alloc(n));
-    HIPCHK(c->dGSorted[g].alloc(n));
-    HIPCHK(c->dGRank[g].alloc(n));
-    HIPCHK(c->dGCount[g].alloc(on ? ncell : 1));
-    HIPCHK(c->dGCellStart[g].alloc(on ? ncell + 1 : 1));
-    HIPCHK(c->dGSPR[g].alloc(n));
   }
 
   // ---- pairs and contacts ----
@@ -938,12 +925,11 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
   for (int g = 0; g < 3; ++g) {
-    P.gpos[g] = c->dGPos[g].p; P.gkey[g] = c->dGKey[g].p; P.gcell[g] = c->dGCell[g].p;
-    P.gCellStart[g] = c->dGCellStart[g].p; P.gSorted[g] = c->dGSorted[g].p; P.gCount[g] = c->dGCount[g].p;
+    P.gpos[g] = c->dGPos[g].p; P.gkey[g] = c->dGKey[g].p;
   }
   P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
   P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
-  for (int g = 0; g < 3; ++g) P.gsPR[g] = c->dGSPR[g].p;
+  P.sBoxR = c->dSBoxR.p;
   P.hashSlot = c->dHashSlot.p; P.statSlots = c->dStatSlots.p;
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
   P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p; P.surMask = c->dSurMask.p;

@@ -90,7 +90,7 @@ struct Params {
   const int* shape;
   const int* nodeStart;  // N+1 prefix of boundary-node counts
   int* keyRef;           // reference-grid key of posDet (-1 = outside the domain)
-  int* scell;            // search-grid cell of posDet
+  int* scell;            // search-grid cell of pos (post-wrap)
 
   // ---- walls ----
   d3* wpos;
@@ -104,11 +104,7 @@ struct Params {
 
   // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
   d3* gpos[3];
-  int* gkey[3];
-  int* gcell[3];     // search-grid cell (or -1 when the image has no key)
-  int* gCellStart[3];  // search-grid table of the images of each set (ncell + 1)
-  int* gSorted[3];
-  int* gCount[3];
+  int* gkey[3];      // reference-grid key of the image, -1 when the image does not exist
 
   // ---- search grid of particles ----
   int* cellCount;   // ncell (scratch)
@@ -118,7 +114,7 @@ struct Params {
   // particle data gathered into sorted order (contiguous runs per search-grid row)
   float4* sPR;      // FP32 (posDet, bounding radius; radius -1 when the particle has no reference key)
   float4* sPRw;     // FP32 (pos after the periodic wrap, bounding radius)
-  float4* gsPR[3];  // ghost images of each set in their sorted order: FP32 (centre, scaled grid-box radius)
+  float* sBoxR;     // scaled grid-box radius (ghost-image reach)
   int* rcTmp;       // reaction pairs before the per-partner sort
   uint32_t* pairSlab;  // kPairSlab x nP: first candidate pairs of each master (column per master)
 

@@ -189,7 +189,7 @@ __global__ void k_kick_drift(Params P) {
   }
   P.posDet[i] = p;
   P.keyRef[i] = refKey(G, p);
-  P.scell[i] = lin3(searchCell(P.sg, p), P.sg.size);
+  int unshifted = 0;  // valid images at the particle's own position (particle outside the domain)
   if (periodic) {
     d3 img;
     if (P.periodicX) {
@@ -199,7 +199,7 @@ __global__ void k_kick_drift(Params P) {
       const int k = ghostKey(P, img);
       P.gpos[0][i] = img;
       P.gkey[0][i] = k;
-      P.gcell[0][i] = k >= 0 ? lin3(searchCell(P.sg, img), P.sg.size) : -1;
+      unshifted += (k >= 0 && refCell(G, p).x != 0);
     }
     if (P.periodicY) {
       if (wrapAfter) { if (p.y < G.mn.y) p += oy; else if (p.y >= G.mx.y) p -= oy; }
@@ -208,7 +208,7 @@ __global__ void k_kick_drift(Params P) {
       const int k = ghostKey(P, img);
       P.gpos[1][i] = img;
       P.gkey[1][i] = k;
-      P.gcell[1][i] = k >= 0 ? lin3(searchCell(P.sg, img), P.sg.size) : -1;
+      unshifted += (k >= 0 && refCell(G, p).y != 0);
     }
     if (P.periodicX && P.periodicY) {
       img = p;
@@ -217,10 +217,14 @@ __global__ void k_kick_drift(Params P) {
       const int k = ghostKey(P, img);
       P.gpos[2][i] = img;
       P.gkey[2][i] = k;
-      P.gcell[2][i] = k >= 0 ? lin3(searchCell(P.sg, img), P.sg.size) : -1;
+      unshifted += (k >= 0 && !(c.x == 0 && c.y == 0));
     }
   }
   P.pos[i] = p;
+  // search cell of the post-wrap centre: ghost images are searched through their sources, and a
+  // particle that wrapped this step is outside the domain before the wrap (never a pp partner)
+  P.scell[i] = lin3(searchCell(P.sg, p), P.sg.size);
+  if (unshifted) atomicAdd(&P.devInt[7], unshifted);
 }
 
 __global__ void k_walls(Params P) {
@@ -315,19 +319,21 @@ __global__ void k_gather_particles(Params P) {
   const float r = float(P.radius[i]);
   P.sPR[k] = make_float4(float(x.x), float(x.y), float(x.z), P.keyRef[i] >= 0 ? r : -1.f);
   P.sPRw[k] = make_float4(float(w.x), float(w.y), float(w.z), r);
-}
-__global__ void k_gather_images(Params P, int set) {
-  const int n = P.gCellStart[set][P.sg.ncell];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int j = P.gSorted[set][k];
-    const d3 x = P.gpos[set][j];
-    P.gsPR[set][k] = make_float4(float(x.x), float(x.y), float(x.z), float(P.scale[j] * P.shapes[P.shape[j]].boxR));
-  }
+  P.sBoxR[k] = float(P.scale[i] * P.shapes[P.shape[i]].boxR);
 }
 
 // ------------------------------------------------------------------------------------------
 // Candidate pairs
 // ------------------------------------------------------------------------------------------
+// Squared distance from x to the search grid's box (0 inside).
+__device__ __forceinline__ double searchBoxDist2(const SearchGrid& S, d3 x) {
+  const double ex = S.size.x * S.cell, ey = S.size.y * S.cell, ez = S.size.z * S.cell;
+  const double dx = fmax(0.0, fmax(S.mn.x - x.x, x.x - (S.mn.x + ex)));
+  const double dy = fmax(0.0, fmax(S.mn.y - x.y, x.y - (S.mn.y + ey)));
+  const double dz = fmax(0.0, fmax(S.mn.z - x.z, x.z - (S.mn.z + ez)));
+  return dx * dx + dy * dy + dz * dz;
+}
+
 // Visit every item of a search-grid table lying in cells that can hold a point within `reach`
 // of x.  Rows (y, z) are pruned by their distance to x; the x-run of cells of a row is
 // contiguous in the sorted order, so each row costs one range lookup.
@@ -453,21 +459,38 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
     }
   }
   // periodic ghost images: any node inside the image's grid box is tested, so the reach is
-  // the image's grid-box radius
+  // the image's grid-box radius.  Images are found through their sources in the particle grid:
+  // an image shifted by the set's offset lies near x_A - offset; images left at the source's own
+  // position (a source outside the domain) are searched around x_A only when such images exist.
   if (P.periodicX || P.periodicY) {
     const float4 g = P.sPRw[t];  // post-wrap centre (images are built from it)
     const float mg = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
     const double reach = (double(g.w) + P.boxRMax) * (1.0 + 2e-6) + 1e-6 * double(mg) + 1e-12;
+    const GridGeom& G = P.gridP;
+    const float lx = float(G.mx.x - G.mn.x), ly = float(G.mx.y - G.mn.y);
+    const bool anyUnshifted = P.devInt[7] > 0;
     for (int s = 0; s < 3; ++s) {
       const bool on = (s == 0 && P.periodicX) || (s == 1 && P.periodicY) || (s == 2 && P.periodicX && P.periodicY);
       if (!on) continue;
-      const float4* gp = P.gsPR[s];
-      const int* gs = P.gSorted[s];
-      visitNear(P.sg, P.gCellStart[s], mk3(g.x, g.y, g.z), reach, [&](int k) {
-        const float4 b = gp[k];
-        if (!spheresMayOverlap(g, b, b.w)) return;
-        emit(PT_GX + s, gs[k]);
-      });
+      const float offx = s != 1 ? lx : 0.f, offy = s != 0 ? ly : 0.f;
+      for (int pass = 0; pass < 2; ++pass) {
+        const bool shifted = pass == 0;
+        if (!shifted && !anyUnshifted) break;
+        const d3 c = shifted ? mk3(double(g.x) - double(offx), double(g.y) - double(offy), double(g.z))
+                             : mk3(g.x, g.y, g.z);
+        if (searchBoxDist2(P.sg, c) > reach * reach) continue;
+        visitNear(P.sg, P.cellStart, c, reach, [&](int k) {
+          const float4 b = P.sPRw[k];
+          const float4 im = shifted ? make_float4(b.x + offx, b.y + offy, b.z, 0.f) : b;
+          if (!spheresMayOverlap(g, im, P.sBoxR[k])) return;
+          const int j = P.sorted[k];
+          if (P.gkey[s][j] < 0) return;
+          const d3 gp = P.gpos[s][j], pj = P.pos[j];
+          const bool jShifted = gp.x != pj.x || gp.y != pj.y;
+          if (jShifted != shifted) return;
+          emit(PT_GX + s, j);
+        });
+      }
     }
   }
   return cnt;
@@ -1262,6 +1285,7 @@ __global__ void k_kick(Params P) {
 __global__ void k_step_begin(Params P) {
   if (threadIdx.x == 0) {
     P.devInt[1] = 0;  // contact slots allocated
+    P.devInt[7] = 0;  // valid ghost images at their source's own position
   }
   for (int k = threadIdx.x; k < 5 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
 }
@@ -1292,7 +1316,6 @@ __global__ void k_step_end(Params P) {
   }
 }
 
-__global__ void k_gather_images(Params P, int set);
 
 template __global__ void k_candidates<false>(Params P);
 template __global__ void k_candidates<true>(Params P);
