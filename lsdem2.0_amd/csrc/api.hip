@@ -14,19 +14,21 @@
 
 namespace lsd {
 // kernels.hip
-__global__ void k_fill_int(int* a, long long n, int v);
+__global__ void k_fill_int(int* a, long long n, int v, const int* gate);
 __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v);
-__global__ void k_scan_reduce(const int* in, long long n, int* blockSum);
-__global__ void k_scan_top(int* blockSum, int nb, int* totalOut);
-__global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out);
+__global__ void k_scan_reduce(const int* in, long long n, int* blockSum, const int* gate);
+__global__ void k_scan_top(int* blockSum, int nb, int* totalOut, const int* gate);
+__global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out, const int* gate);
 __global__ void k_kick_drift(Params P);
 __global__ void k_walls(Params P);
-__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank);
+__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
-                                 int* bigCount);
-__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount);
+                                 int* bigCount, const int* gate);
+__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
+                              const int* gate);
 __global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
-                                  const int* bigCount);
+                                  const int* bigCount, const int* gate);
+__global__ void k_mark_build(Params P);
 __global__ void k_pairs_find(Params P);
 __global__ void k_pairs_emit(Params P);
 __global__ void k_rc_count(Params P);
@@ -43,7 +45,7 @@ __global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P, int kick);
 __global__ void k_kick(Params P);
-__global__ void k_step_begin(Params P);
+__global__ void k_step_begin(Params P, int forceRebuild);
 __global__ void k_step_end(Params P);
 }  // namespace lsd
 
@@ -214,6 +216,9 @@ struct lsdem_ctx {
   DBuf<float4> dNodes32, dPairMap;
   DBuf<unsigned long long> dSurMask;
   DBuf<ClusterF32> dClusters32;
+  DBuf<d3> dXBuild, dXBuildW, dWBuildPos;
+  DBuf<q4> dWBuildQ;
+  bool forceRebuild = true;
   DBuf<int> dNodeCluster;
   // learned culling surrogate
   struct Sur {
@@ -315,6 +320,7 @@ void lsdem_ctx::releaseAll() {
   dBMj.release();
   dDevInt.release(); dScanBlocks.release(); dDevStat.release();
   dNodeCluster.release();
+  dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   sur.release();
 }
 
@@ -349,34 +355,37 @@ int upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
 
 inline unsigned blocksFor(long long n, int bs) { return unsigned(std::max<long long>(1, (n + bs - 1) / bs)); }
 
-int scan(lsdem_ctx* c, const int* in, long long n, int* out) {
+// Exclusive scan (out[n] = total).  `gate`: run only when *gate != 0 (nullptr: always).
+int scan(lsdem_ctx* c, const int* in, long long n, int* out, const int* gate) {
   if (n <= 0) {
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, out, 1ll, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, out, 1ll, 0, gate);
     return LSDEM_OK;
   }
   const int nb = int((n + 2047) / 2048);
   if (nb > c->scanBlocksCap) return fail(LSDEM_ERR_STATE, "scan buffer too small");
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->dScanBlocks.p);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, c->stream, c->dScanBlocks.p, nb, out + n);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, in, n, (const int*)c->dScanBlocks.p, out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->dScanBlocks.p, gate);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, c->stream, c->dScanBlocks.p, nb, out + n, gate);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, in, n, (const int*)c->dScanBlocks.p, out, gate);
   return LSDEM_OK;
 }
 
-// Deterministic counting sort of `n` items by bucket id (ids < 0 skipped).
-int bucketSort(lsdem_ctx* c, const int* bucket, int n, int nb, int* count, int* start, int* rank, int* out) {
+// Deterministic counting sort of `n` items by bucket id (ids < 0 skipped), gated like scan.
+int bucketSort(lsdem_ctx* c, const int* bucket, int n, int nb, int* count, int* start, int* rank, int* out,
+               const int* gate) {
   const int bs = 256;
-  hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, count, (long long)nb, 0);
-  if (n > 0) hipLaunchKernelGGL(k_bucket_count, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n, count, rank);
-  int rc = scan(c, count, nb, start);
+  hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, count, (long long)nb, 0, gate);
+  if (n > 0)
+    hipLaunchKernelGGL(k_bucket_count, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n, count, rank, gate);
+  int rc = scan(c, count, nb, start, gate);
   if (rc) return rc;
   if (n > 0) {
     int* bigCount = c->P.devInt + 2;
     hipLaunchKernelGGL(k_bucket_scatter, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n,
-                       (const int*)start, (const int*)rank, c->dSortTmp.p, bigCount);
+                       (const int*)start, (const int*)rank, c->dSortTmp.p, bigCount, gate);
     hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, (const int*)start, nb,
-                       (const int*)c->dSortTmp.p, out, c->dSortBig.p, bigCount);
+                       (const int*)c->dSortTmp.p, out, c->dSortBig.p, bigCount, gate);
     hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, c->stream, (const int*)start,
-                       (const int*)c->dSortTmp.p, out, (const int*)c->dSortBig.p, (const int*)bigCount);
+                       (const int*)c->dSortTmp.p, out, (const int*)c->dSortBig.p, (const int*)bigCount, gate);
   }
   return LSDEM_OK;
 }
@@ -419,30 +428,33 @@ int runStep(lsdem_ctx* c) {
     ++mark;
   };
   phase();
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, P);
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, P, c->forceRebuild ? 1 : 0);
+  c->forceRebuild = false;
+  const int* gate = P.devInt + 8;  // broad phase only on rebuild steps
   hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
   if (P.nW > 0) hipLaunchKernelGGL(k_walls, dim3(blocksFor(P.nW, bs)), dim3(bs), 0, s, P);
   phase();
-  int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted);
+  int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
   phase();
   hipLaunchKernelGGL(k_pairs_find, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
-  rc = scan(c, P.pairCount, P.nP, P.pairStart);
+  rc = scan(c, P.pairCount, P.nP, P.pairStart, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_pairs_emit, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  hipLaunchKernelGGL(k_mark_build, dim3(blocksFor(std::max(P.nP, P.nW), 256)), dim3(256), 0, s, P);
   phase();
   // reaction CSR
-  hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P.rcCount, (long long)P.nP, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P.rcCount, (long long)P.nP, 0, gate);
   const unsigned gsPairs = unsigned(c->numCU * 8);
   hipLaunchKernelGGL(k_rc_count, dim3(gsPairs), dim3(256), 0, s, P);
-  rc = scan(c, P.rcCount, P.nP, P.rcStart);
+  rc = scan(c, P.rcCount, P.nP, P.rcStart, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_rc_scatter, dim3(gsPairs), dim3(256), 0, s, P);
   hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, (const int*)P.rcStart, P.nP,
-                     (const int*)P.rcTmp, P.rcPair, c->dSortBig.p, P.devInt + 2);
+                     (const int*)P.rcTmp, P.rcPair, c->dSortBig.p, P.devInt + 2, gate);
   hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, s, (const int*)P.rcStart,
-                     (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2));
+                     (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2), gate);
   phase();
   // narrow phase + contact law
   hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
@@ -510,6 +522,7 @@ void lsdem_config_default(lsdem_config* cfg) {
   cfg->dt = 1e-5;
   cfg->quirks = LSDEM_QUIRK_ALL;
   cfg->block_size = 256;
+  cfg->skin = -1.0;
 }
 
 const char* lsdem_last_error(void) { return g_err.c_str(); }
@@ -773,6 +786,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   if ((rc = upload(c->dShape, c->pShape, s))) return rc;
   if ((rc = upload(c->dNodeStart, nodeStart, s))) return rc;
   HIPCHK(c->dKeyRef.alloc(std::max(nP, 1)));
+  HIPCHK(c->dXBuild.alloc(std::max(nP, 1)));
+  HIPCHK(c->dXBuildW.alloc(std::max(nP, 1)));
   HIPCHK(c->dScell.alloc(std::max(nP, 1)));
 
   // ---- walls ----
@@ -784,6 +799,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   if ((rc = upload(c->dWRes, c->wRes, s))) return rc;
   if ((rc = upload(c->dWShape, c->wShape, s))) return rc;
   HIPCHK(c->dWKey.alloc(std::max(nW, 1)));
+  HIPCHK(c->dWBuildPos.alloc(std::max(nW, 1)));
+  HIPCHK(c->dWBuildQ.alloc(std::max(nW, 1)));
 
   // ---- grids ----
   double cellW = 0;
@@ -792,9 +809,13 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.gridW = makeGrid(dmn, dmx, nW ? cellW * 2.0 : 0.);
   P.rMax = rMax;
   P.boxRMax = boxRMax;
+  // skin of the reused candidate pairs (rebuilt when a body has moved half of it)
+  const double skin = c->cfg.skin >= 0.0 ? c->cfg.skin : 0.02 * rMax;
+  P.halfSkin = 0.5 * skin;
+  c->forceRebuild = true;
   {
     const d3 ext = dmx - dmn;
-    double cell = rMax > 0 ? rMax : std::max(ext.x, std::max(ext.y, ext.z));
+    double cell = rMax > 0 ? rMax + 0.5 * skin : std::max(ext.x, std::max(ext.y, ext.z));
     for (;;) {
       const double nc = std::ceil(ext.x / cell) * std::ceil(ext.y / cell) * std::ceil(ext.z / cell);
       if (nc <= 4.0e6) break;
@@ -922,6 +943,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.radius = c->dRadius.p; P.kn = c->dKn.p; P.ks = c->dKs.p; P.mu = c->dMu.p; P.res = c->dRes.p;
   P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
   P.nodeStart = c->dNodeStart.p; P.keyRef = c->dKeyRef.p; P.scell = c->dScell.p;
+  P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
   for (int g = 0; g < 3; ++g) {
@@ -994,6 +1016,7 @@ int lsdem_set_particles(lsdem_ctx* c, const double* pos, const double* quat, con
   HIPCHK(hipStreamSynchronize(c->stream));
   const size_t n = size_t(c->P.nP);
   if (n == 0) return LSDEM_OK;
+  c->forceRebuild = true;  // poses changed outside the step: rebuild the candidate pairs
   if (pos) {
     HIPCHK(hipMemcpy(c->dPos.p, pos, n * sizeof(d3), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->dPosDet.p, pos, n * sizeof(d3), hipMemcpyHostToDevice));
@@ -1025,6 +1048,7 @@ int lsdem_set_walls(lsdem_ctx* c, const double* pos, const double* quat, const d
   HIPCHK(hipStreamSynchronize(c->stream));
   const size_t n = size_t(c->P.nW);
   if (n == 0) return LSDEM_OK;
+  c->forceRebuild = true;  // poses changed outside the step: rebuild the candidate pairs
   if (pos) HIPCHK(hipMemcpy(c->dWPos.p, pos, n * sizeof(d3), hipMemcpyHostToDevice));
   if (quat) HIPCHK(hipMemcpy(c->dWQuat.p, quat, n * sizeof(q4), hipMemcpyHostToDevice));
   if (vel) HIPCHK(hipMemcpy(c->dWVel.p, vel, n * sizeof(d3), hipMemcpyHostToDevice));

@@ -55,6 +55,7 @@ struct Params {
   GridGeom gridP, gridW;
   SearchGrid sg;
   double rMax;        // max particle bounding radius
+  double halfSkin;    // candidate pairs are reused until a body has moved this far since they were built
   double boxRMax;     // max scaled grid-box radius over particles (ghost search reach)
   long long pairCap, contactCap;
   unsigned long long hashMask;
@@ -91,6 +92,8 @@ struct Params {
   const int* nodeStart;  // N+1 prefix of boundary-node counts
   int* keyRef;           // reference-grid key of posDet (-1 = outside the domain)
   int* scell;            // search-grid cell of pos (post-wrap)
+  d3* xBuild;            // posDet when the candidate pairs were built
+  d3* xBuildW;           // pos when the candidate pairs were built
 
   // ---- walls ----
   d3* wpos;
@@ -101,6 +104,8 @@ struct Params {
   const double* wres;
   const int* wshape;
   int* wkey;
+  d3* wBuildPos;         // wall pose when the candidate pairs were built
+  q4* wBuildQ;
 
   // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
   d3* gpos[3];

@@ -32,6 +32,35 @@ __device__ __forceinline__ int refKey(const GridGeom& G, d3 p) {
   if (p.z < G.mn.z) return -1; else if (p.z >= G.mx.z) return -1;
   return lin3(refCell(G, p), G.size);
 }
+// Squared distance from p to the reference domain box (0 inside).
+__device__ __forceinline__ double domainDist2(const GridGeom& G, d3 p) {
+  const double dx = fmax(0.0, fmax(G.mn.x - p.x, p.x - G.mx.x));
+  const double dy = fmax(0.0, fmax(G.mn.y - p.y, p.y - G.mx.y));
+  const double dz = fmax(0.0, fmax(G.mn.z - p.z, p.z - G.mx.z));
+  return dx * dx + dy * dy + dz * dz;
+}
+// Ghost-image eligibility of a source at post-wrap position p, widened by hs (half the skin) so
+// that it holds for every position within hs of p.  Set s: 0 = +x, 1 = +y, 2 = +x+y.
+__device__ __forceinline__ bool potInColumn(const GridGeom& G, d3 p, int s, double hs) {
+  const bool cx = (p.x - G.mn.x) * G.inv.x < 1.0 + hs * G.inv.x + 1e-9;
+  const bool cy = (p.y - G.mn.y) * G.inv.y < 1.0 + hs * G.inv.y + 1e-9;
+  return s == 0 ? cx : (s == 1 ? cy : (cx && cy));
+}
+__device__ __forceinline__ bool potOutColumn(const GridGeom& G, d3 p, int s, double hs) {
+  const bool nx = (p.x - G.mn.x) * G.inv.x >= 1.0 - hs * G.inv.x - 1e-9;
+  const bool ny = (p.y - G.mn.y) * G.inv.y >= 1.0 - hs * G.inv.y - 1e-9;
+  return s == 0 ? nx : (s == 1 ? ny : (nx || ny));
+}
+// Can the image key of an unshifted image be valid (the source outside the ghost domain)?
+__device__ __forceinline__ bool potOutOfDomain(const Params& P, d3 p, double hs) {
+  const GridGeom& G = P.gridP;
+  const double zmax = (P.quirks & 2) ? G.mx.y : G.mx.z;
+  bool out = p.z < G.mn.z + hs || p.z >= zmax - hs;
+  if (!P.periodicX) out = out || p.x < G.mn.x + hs || p.x >= G.mx.x - hs;
+  if (!P.periodicY) out = out || p.y < G.mn.y + hs || p.y >= G.mx.y - hs;
+  return out;
+}
+
 // Clamped cell and +-1 window of a node (the reference's 27-cell traversal bounds).
 __device__ __forceinline__ void refWindow(const GridGeom& G, d3 p, i3& c, i3& lo, i3& hi) {
   c = refCell(G, p);
@@ -77,7 +106,13 @@ __device__ __forceinline__ int ghostKey(const Params& P, d3 p) {
 // ------------------------------------------------------------------------------------------
 // Small utilities
 // ------------------------------------------------------------------------------------------
-__global__ void k_fill_int(int* a, long long n, int v) {
+// Broad-phase kernels run only on steps that rebuild the candidate pairs: `gate` points at the
+// step's rebuild flag (nullptr = always run).
+#define LSD_GATE(g) \
+  if ((g) != nullptr && *(g) == 0) return
+
+__global__ void k_fill_int(int* a, long long n, int v, const int* gate) {
+  LSD_GATE(gate);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     a[i] = v;
 }
@@ -118,7 +153,9 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int* lds, int& total) {
   return r;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* in, long long n, int* blockSum) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* in, long long n, int* blockSum,
+                                                               const int* gate) {
+  LSD_GATE(gate);
   __shared__ int lds[kScanThreads / 64 + 1];
   const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
   int s = 0;
@@ -129,7 +166,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* in, lon
   blockExclusiveScan(s, lds, total);
   if (threadIdx.x == 0) blockSum[blockIdx.x] = total;
 }
-__global__ __launch_bounds__(kScanThreads) void k_scan_top(int* blockSum, int nb, int* totalOut) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(int* blockSum, int nb, int* totalOut, const int* gate) {
+  LSD_GATE(gate);
   __shared__ int lds[kScanThreads / 64 + 1];
   int carry = 0;
   for (int base = 0; base < nb; base += kScanThreads) {
@@ -142,7 +180,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(int* blockSum, int nb
   }
   if (threadIdx.x == 0) *totalOut = carry;
 }
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, long long n, const int* blockSum, int* out) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, long long n, const int* blockSum, int* out,
+                                                              const int* gate) {
+  LSD_GATE(gate);
   __shared__ int lds[kScanThreads / 64 + 1];
   const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
   int v[kScanItems];
@@ -224,7 +264,19 @@ __global__ void k_kick_drift(Params P) {
   // search cell of the post-wrap centre: ghost images are searched through their sources, and a
   // particle that wrapped this step is outside the domain before the wrap (never a pp partner)
   P.scell[i] = lin3(searchCell(P.sg, p), P.sg.size);
+  // sources that may hold an unshifted image before the next rebuild
+  if (periodic && P.halfSkin > 0.0) {
+    unshifted = 0;
+    if (potOutOfDomain(P, p, P.halfSkin))
+      for (int s = 0; s < 3; ++s) unshifted += potOutColumn(G, p, s, P.halfSkin);
+  }
   if (unshifted) atomicAdd(&P.devInt[7], unshifted);
+  // rebuild the candidate pairs when a body has moved half the skin since the last build (a
+  // periodic wrap always does)
+  const double hs2 = P.halfSkin * P.halfSkin;
+  const d3 dd = P.posDet[i] - P.xBuild[i], dw = p - P.xBuildW[i];
+  const bool moved = len2(dd) > hs2 || len2(dw) > hs2;
+  if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[8], 1);
 }
 
 __global__ void k_walls(Params P) {
@@ -233,22 +285,48 @@ __global__ void k_walls(Params P) {
   d3 p = P.wpos[w];
   p += P.dt * P.wvel[w];
   P.wpos[w] = p;
-  P.wquat[w] = qintegrate(P.wquat[w], P.wangvel[w], P.dt);
+  const q4 q = qintegrate(P.wquat[w], P.wangvel[w], P.dt);
+  P.wquat[w] = q;
   P.wkey[w] = refKey(P.gridW, p);
+  // displacement bound of any point of the wall's grid box since the last build: translation plus
+  // 2 R sin(angle / 2) for the rotation
+  const q4 b = P.wBuildQ[w];
+  const double vx = b.q0 * q.q1 - q.q0 * b.q1 - (b.q2 * q.q3 - b.q3 * q.q2);
+  const double vy = b.q0 * q.q2 - q.q0 * b.q2 - (b.q3 * q.q1 - b.q1 * q.q3);
+  const double vz = b.q0 * q.q3 - q.q0 * b.q3 - (b.q1 * q.q2 - b.q2 * q.q1);
+  const double R = P.shapes[P.wshape[w]].boxR;
+  const double disp = len(p - P.wBuildPos[w]) + 2.0 * R * sqrt(vx * vx + vy * vy + vz * vz);
+  if (disp > P.halfSkin) atomicOr(&P.devInt[8], 1);
+}
+
+// Record the poses the candidate pairs were built from (rebuild steps only).
+__global__ void k_mark_build(Params P) {
+  LSD_GATE(P.devInt + 8);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.nP) {
+    P.xBuild[i] = P.posDet[i];
+    P.xBuildW[i] = P.pos[i];
+  }
+  if (i < P.nW) {
+    P.wBuildPos[i] = P.wpos[i];
+    P.wBuildQ[i] = P.wquat[i];
+  }
 }
 
 // ------------------------------------------------------------------------------------------
 // Deterministic counting sort by bucket: count (atomic rank) -> scan -> scatter -> per-bucket
 // insertion sort by item index.
 // ------------------------------------------------------------------------------------------
-__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank) {
+__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate) {
+  LSD_GATE(gate);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = bucket[i];
   rank[i] = b >= 0 ? atomicAdd(&count[b], 1) : -1;
 }
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
-                                 int* bigCount) {
+                                 int* bigCount, const int* gate) {
+  LSD_GATE(gate);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *bigCount = 0;  // list of large buckets of the following k_bucket_sort
   if (i >= n) return;
@@ -257,7 +335,9 @@ __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, con
 }
 // Buckets of up to 16 items are sorted in registers (odd-even transposition network) from
 // `tmp` into `items`; larger ones are listed for k_bucket_sort_big.
-__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount) {
+__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
+                              const int* gate) {
+  LSD_GATE(gate);
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int s = start[b], e = start[b + 1];
@@ -290,7 +370,8 @@ __global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* ite
 // One wavefront per listed bucket: each item's place is the number of smaller items (items are
 // distinct indices).
 __global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
-                                  const int* bigCount) {
+                                  const int* bigCount, const int* gate) {
+  LSD_GATE(gate);
   const int lane = threadIdx.x & 63;
   const int nBig = *bigCount;
   const int nWaves = int(gridDim.x * (blockDim.x >> 6));
@@ -312,12 +393,16 @@ __global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, 
 // Gather particle (or ghost-image) data into the sorted order of the search-grid table so
 // that the candidate loops stream contiguous memory.
 __global__ void k_gather_particles(Params P) {
+  LSD_GATE(P.devInt + 8);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.nP) return;
   const int i = P.sorted[k];
   const d3 x = P.posDet[i], w = P.pos[i];
   const float r = float(P.radius[i]);
-  P.sPR[k] = make_float4(float(x.x), float(x.y), float(x.z), P.keyRef[i] >= 0 ? r : -1.f);
+  // a pp partner must be inside the domain when the pair is used; one that can get there before
+  // the next rebuild (within half the skin) stays a candidate
+  const bool eligible = P.keyRef[i] >= 0 || domainDist2(P.gridP, x) <= P.halfSkin * P.halfSkin;
+  P.sPR[k] = make_float4(float(x.x), float(x.y), float(x.z), eligible ? r : -1.f);
   P.sPRw[k] = make_float4(float(w.x), float(w.y), float(w.z), r);
   P.sBoxR[k] = float(P.scale[i] * P.shapes[P.shape[i]].boxR);
 }
@@ -428,13 +513,14 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
     ++cnt;
   };
   const float4 a = P.sPR[t];
-  // particle-particle: B > A, B inside the domain, bounding spheres overlap
+  const float skin = float(2.0 * P.halfSkin);
+  // particle-particle: B > A, B inside the domain, bounding spheres within the skin
   {
     const float ma = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z));
-    const double reach = (double(a.w) + P.rMax) * (1.0 + 2e-6) + 1e-6 * double(ma) + 1e-12;
+    const double reach = (double(a.w) + P.rMax + 2.0 * P.halfSkin) * (1.0 + 2e-6) + 1e-6 * double(ma) + 1e-12;
     visitNear(P.sg, P.cellStart, mk3(a.x, a.y, a.z), reach, [&](int k) {
       const float4 b = P.sPR[k];
-      if (b.w < 0.f || !spheresMayOverlap(a, b, b.w)) return;
+      if (b.w < 0.f || !spheresMayOverlap(a, b, b.w + skin)) return;
       const int B = P.sorted[k];
       if (B <= A) return;
       emit(PT_PP, B);
@@ -444,17 +530,17 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
   if (P.nW > 0) {
     const double rA = P.radius[A];
     const d3 xA = P.posDet[A];
-    for (int w = 0; w < P.nW; ++w) {
-      if (P.wkey[w] < 0) continue;
+    for (int w = 0; w < P.nW; ++w) {  // a wall's key is checked each step when the pair is used
       const ShapeDev S = P.shapes[P.wshape[w]];
       const d3 loc = rotInv(xA - P.wpos[w], P.wquat[w]);
       const d3 o = mk3(S.ox, S.oy, S.oz);
-      const double rr = rA * (1.0 + 1e-9) + 1e-12;
+      const double rs = rA + 2.0 * P.halfSkin;  // the wall and the particle each move < half the skin
+      const double rr = rs * (1.0 + 1e-9) + 1e-12;
       if (boxDist2(loc, o, S.h, S) > rr * rr) continue;
       bool inside;
       double cmax;
       const double phi = triAt(P, S, 1.0, loc, o, 1. / S.h, inside, cmax);
-      if (inside && phi > S.lip * rA * (1.0 + 1e-6) + 1e-9 * S.h + 1e-12 * cmax) continue;
+      if (inside && phi > S.lip * rs * (1.0 + 1e-6) + 1e-9 * S.h + 1e-12 * cmax) continue;
       emit(PT_PW, w);
     }
   }
@@ -465,8 +551,9 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
   if (P.periodicX || P.periodicY) {
     const float4 g = P.sPRw[t];  // post-wrap centre (images are built from it)
     const float mg = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
-    const double reach = (double(g.w) + P.boxRMax) * (1.0 + 2e-6) + 1e-6 * double(mg) + 1e-12;
+    const double reach = (double(g.w) + P.boxRMax + 2.0 * P.halfSkin) * (1.0 + 2e-6) + 1e-6 * double(mg) + 1e-12;
     const GridGeom& G = P.gridP;
+    const double hs = P.halfSkin;
     const float lx = float(G.mx.x - G.mn.x), ly = float(G.mx.y - G.mn.y);
     const bool anyUnshifted = P.devInt[7] > 0;
     for (int s = 0; s < 3; ++s) {
@@ -481,13 +568,19 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
         if (searchBoxDist2(P.sg, c) > reach * reach) continue;
         visitNear(P.sg, P.cellStart, c, reach, [&](int k) {
           const float4 b = P.sPRw[k];
+          const float rb = P.sBoxR[k] + skin;
           const float4 im = shifted ? make_float4(b.x + offx, b.y + offy, b.z, 0.f) : b;
-          if (!spheresMayOverlap(g, im, P.sBoxR[k])) return;
+          if (!spheresMayOverlap(g, im, rb)) return;
           const int j = P.sorted[k];
-          if (P.gkey[s][j] < 0) return;
-          const d3 gp = P.gpos[s][j], pj = P.pos[j];
-          const bool jShifted = gp.x != pj.x || gp.y != pj.y;
-          if (jShifted != shifted) return;
+          const d3 pj = P.pos[j];
+          if (shifted) {
+            if (!potInColumn(G, pj, s, hs)) return;
+          } else {
+            if (!potOutOfDomain(P, pj, hs) || !potOutColumn(G, pj, s, hs)) return;
+            // one pair per (A, image): the shifted pass already took it
+            const float4 sh = make_float4(b.x + offx, b.y + offy, b.z, 0.f);
+            if (potInColumn(G, pj, s, hs) && spheresMayOverlap(g, sh, rb)) return;
+          }
           emit(PT_GX + s, j);
         });
       }
@@ -498,6 +591,7 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
 
 // Broad phase, one thread per master in spatial order: count and keep the first candidates.
 __global__ void k_pairs_find(Params P) {
+  LSD_GATE(P.devInt + 8);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P.nP) return;
   const int A = P.sorted[t];
@@ -505,6 +599,7 @@ __global__ void k_pairs_find(Params P) {
 }
 // Pair list in master order: copy the slab, or search again for masters with more candidates.
 __global__ void k_pairs_emit(Params P) {
+  LSD_GATE(P.devInt + 8);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P.nP) return;
   const int A = P.sorted[t];
@@ -523,6 +618,7 @@ __global__ void k_pairs_emit(Params P) {
 
 // Pairs whose partner particle receives the reaction (particle-particle, ghosts with j != A).
 __global__ void k_rc_count(Params P) {
+  LSD_GATE(P.devInt + 8);
   const int n = min(P.pairStart[P.nP], (int)P.pairCap);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const uint32_t info = P.pairInfo[p];
@@ -534,6 +630,7 @@ __global__ void k_rc_count(Params P) {
   }
 }
 __global__ void k_rc_scatter(Params P) {
+  LSD_GATE(P.devInt + 8);
   const int n = min(P.pairStart[P.nP], (int)P.pairCap);
   if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[2] = 0;  // large-bucket list of the following sort
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
@@ -778,8 +875,14 @@ __global__ void k_pair_maps(Params P) {
     const PairView v = loadPair(P, p);
     PairF32 f;
     makeF32(P, v, f);
+    // the pair list may be older than this step: the partner must exist now (pp: inside the
+    // domain; wall: its key; ghost: the image)
+    const int j = v.partner;
+    const bool live = v.type == PT_PP ? P.keyRef[j] >= 0
+                                      : (v.type == PT_PW ? P.wkey[j] >= 0 : P.gkey[v.type - PT_GX][j] >= 0);
+    if (!live) f.nClusters = 0;
     out[p] = f;
-    masterNodes += (unsigned long long)P.shapes[P.shape[v.A]].nNodes;
+    if (live) masterNodes += (unsigned long long)P.shapes[P.shape[v.A]].nNodes;
   }
   const unsigned long long t = (unsigned long long)wsum(double(masterNodes));
   if ((threadIdx.x & 63) == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
@@ -1282,8 +1385,9 @@ __global__ void k_kick(Params P) {
 // ------------------------------------------------------------------------------------------
 // Step bookkeeping
 // ------------------------------------------------------------------------------------------
-__global__ void k_step_begin(Params P) {
+__global__ void k_step_begin(Params P, int forceRebuild) {
   if (threadIdx.x == 0) {
+    P.devInt[8] = forceRebuild;  // rebuild flag of this step (the drift kernels may raise it)
     P.devInt[1] = 0;  // contact slots allocated
     P.devInt[7] = 0;  // valid ghost images at their source's own position
   }

@@ -39,7 +39,8 @@ class Config(C.Structure):
         ("contact_capacity", C.c_int64),
         ("device", C.c_int32),
         ("no_sdf_cull", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("skin", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -164,6 +165,7 @@ class Sim:
         c.contact_capacity = int(cfg.get("contact_capacity", 0))
         c.device = int(cfg.get("device", 0))
         c.no_sdf_cull = int(cfg.get("no_sdf_cull", 0))
+        c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()
         self._call("create", C.byref(c), C.byref(self.ctx))

@@ -23,8 +23,9 @@ def _pair(sc, L):
 def test_c1_compressed_walls(L, S):
     # 64 superellipsoids on a lattice tighter than their bounding spheres: many contacts at step 1
     sc = S.scene_c1(n=64, lattice=9.5e-3, jitter=0.3e-3)
+    sc.cfg["skin"] = 1e-3  # candidate pairs reused across the steps
     gpu, ref = _pair(sc, L)
-    counts, worst = step_and_compare(gpu, ref, 5, lists=(0, 1), log=print)
+    counts, worst = step_and_compare(gpu, ref, 12, lists=(0, 1), log=print)
     assert sum(c["pp"] for c in counts) > 100
     print("worst", worst)
 
@@ -38,12 +39,28 @@ def test_c1_floor_contacts(L, S):
     assert sum(c["pw"] for c in counts) > 10
 
 
-def test_c2_periodic_small(L, S):
-    # periodic x/y with a floor: ghost lists +x, +y, +x+y
+@pytest.mark.parametrize("skin", [0.0, -1.0, 2e-3])
+def test_c2_periodic_small(L, S, skin):
+    # periodic x/y with a floor: ghost lists +x, +y, +x+y.  skin 0 rebuilds the candidate pairs
+    # every step, -1 is the default, 2 mm (a third of a radius) keeps them for many steps
     sc = S.scene_c2(n=600, n_types=3, n_grid=24, n_nodes=128, cell_xy=0.06, gap=-0.15)
+    sc.cfg["skin"] = skin
     gpu, ref = _pair(sc, L)
-    counts, worst = step_and_compare(gpu, ref, 4, lists=(0, 1, 2, 3, 4), log=print)
+    counts, worst = step_and_compare(gpu, ref, 4 if skin == 0.0 else 25, lists=(0, 1, 2, 3, 4), log=print)
     assert sum(c["gx"] + c["gy"] for c in counts) > 0
+
+
+def test_reused_pairs_through_wraps(L, S):
+    # fast particles crossing the periodic faces while the candidate pairs are reused (large skin)
+    sc = S.scene_c2(n=300, n_types=2, n_grid=24, n_nodes=128, cell_xy=0.05, gap=-0.1)
+    rng = np.random.default_rng(11)
+    sc.particles["vel"][:, :2] = rng.normal(0.0, 20.0, (sc.n_particles, 2))
+    sc.cfg["skin"] = 3e-3
+    gpu, ref = _pair(sc, L)
+    x0 = ref.get_particles()["pos"].copy()
+    step_and_compare(gpu, ref, 40, lists=(0, 1, 2, 3, 4))
+    x1 = ref.get_particles()["pos"]
+    assert np.any(np.abs(x1[:, 0] - x0[:, 0]) > 0.02)  # some particles wrapped around
 
 
 def test_history_persists(L, S):
