@@ -73,61 +73,81 @@ struct HostShape {
   std::vector<d3> nodes;
   std::vector<NodeDev> perm;        // nodes in cluster order
   std::vector<ClusterDev> clusters;
+  std::vector<ClusterDev> subs;     // sub-clusters of <= 16 nodes
   double boxR = 0, lip = 0;
 };
 
-// Split nodes into spatially compact clusters of <= 64 (longest-axis median split, sizes kept
-// at multiples of 64 where possible) and record each cluster's bounding sphere.
-void buildClusters(HostShape& S) {
-  std::vector<int> idx(S.nodes.size());
-  std::iota(idx.begin(), idx.end(), 0);
-  struct Seg { int first, count; };
-  std::vector<Seg> work{{0, int(idx.size())}}, leaves;
+// Longest-axis median splits of idx[first, first + count) into leaves of <= leaf items, left
+// parts kept at multiples of `leaf` where possible; leaves returned in index order.
+struct Seg { int first, count; };
+std::vector<Seg> splitLeaves(const std::vector<d3>& pts, std::vector<int>& idx, Seg root, int leaf) {
+  std::vector<Seg> work{root}, leaves;
   while (!work.empty()) {
     Seg s = work.back();
     work.pop_back();
-    if (s.count <= 64) { if (s.count > 0) leaves.push_back(s); continue; }
-    d3 lo = S.nodes[idx[s.first]], hi = lo;
+    if (s.count <= leaf) { if (s.count > 0) leaves.push_back(s); continue; }
+    d3 lo = pts[idx[s.first]], hi = lo;
     for (int k = s.first; k < s.first + s.count; ++k) {
-      const d3 p = S.nodes[idx[k]];
+      const d3 p = pts[idx[k]];
       lo = mk3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
       hi = mk3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
     }
     const d3 ext = hi - lo;
     const int axis = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
-    int left = ((s.count + 1) / 2 + 63) / 64 * 64;
+    int left = ((s.count + 1) / 2 + leaf - 1) / leaf * leaf;
     left = std::min(left, s.count - 1);
-    auto coord = [&](int i) { const d3 p = S.nodes[i]; return axis == 0 ? p.x : axis == 1 ? p.y : p.z; };
+    auto coord = [&](int i) { const d3 p = pts[i]; return axis == 0 ? p.x : axis == 1 ? p.y : p.z; };
     std::nth_element(idx.begin() + s.first, idx.begin() + s.first + left, idx.begin() + s.first + s.count,
-                     [&](int a, int b) { return coord(a) < coord(b) || (coord(a) == coord(b) && a < b); });
+                     [&](int a2, int b2) { return coord(a2) < coord(b2) || (coord(a2) == coord(b2) && a2 < b2); });
     work.push_back({s.first + left, s.count - left});
     work.push_back({s.first, left});
   }
-  std::sort(leaves.begin(), leaves.end(), [](const Seg& a, const Seg& b) { return a.first < b.first; });
+  std::sort(leaves.begin(), leaves.end(), [](const Seg& x, const Seg& y) { return x.first < y.first; });
+  return leaves;
+}
+
+// Bounding sphere (box centre, max distance) of the nodes idx[s].
+ClusterDev boundSeg(const std::vector<d3>& pts, const std::vector<int>& idx, Seg s) {
+  d3 lo = pts[idx[s.first]], hi = lo;
+  for (int k = s.first; k < s.first + s.count; ++k) {
+    const d3 p = pts[idx[k]];
+    lo = mk3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+    hi = mk3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+  }
+  const d3 c = 0.5 * (lo + hi);
+  double rho = 0;
+  for (int k = s.first; k < s.first + s.count; ++k) rho = std::max(rho, len(pts[idx[k]] - c));
+  ClusterDev cl{};
+  cl.cx = c.x; cl.cy = c.y; cl.cz = c.z;
+  cl.rho = rho * (1.0 + 1e-12);
+  cl.first = s.first;
+  cl.count = s.count;
+  return cl;
+}
+
+// Node hierarchy of a shape: clusters of <= 64 nodes, each split into sub-clusters of <= 16,
+// both spatially compact, with bounding spheres.  Nodes are stored in cluster (and sub-cluster)
+// order; a cluster's sub-clusters are consecutive (subFirst local to the shape, nSub).
+void buildClusters(HostShape& S) {
+  std::vector<int> idx(S.nodes.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  const std::vector<Seg> leaves = splitLeaves(S.nodes, idx, Seg{0, int(idx.size())}, 64);
   S.perm.clear();
   S.clusters.clear();
+  S.subs.clear();
   for (const Seg& s : leaves) {
-    d3 lo = S.nodes[idx[s.first]], hi = lo;
-    for (int k = s.first; k < s.first + s.count; ++k) {
-      const d3 p = S.nodes[idx[k]];
-      lo = mk3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
-      hi = mk3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
-    }
-    const d3 c = 0.5 * (lo + hi);
-    double rho = 0;
-    ClusterDev cl{};
-    cl.first = int(S.perm.size());
-    cl.count = s.count;
-    for (int k = s.first; k < s.first + s.count; ++k) {
-      const d3 p = S.nodes[idx[k]];
-      rho = std::max(rho, len(p - c));
-      NodeDev nd{};
-      nd.x = p.x; nd.y = p.y; nd.z = p.z; nd.orig = idx[k];
-      S.perm.push_back(nd);
-    }
-    cl.cx = c.x; cl.cy = c.y; cl.cz = c.z;
-    cl.rho = rho * (1.0 + 1e-12);
+    const std::vector<Seg> subs = splitLeaves(S.nodes, idx, s, 16);
+    ClusterDev cl = boundSeg(S.nodes, idx, s);
+    cl.subFirst = int(S.subs.size());
+    cl.nSub = int(subs.size());
+    for (const Seg& u : subs) S.subs.push_back(boundSeg(S.nodes, idx, u));
     S.clusters.push_back(cl);
+  }
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const d3 p = S.nodes[idx[k]];
+    NodeDev nd{};
+    nd.x = p.x; nd.y = p.y; nd.z = p.z; nd.orig = idx[k];
+    S.perm.push_back(nd);
   }
 }
 
@@ -215,7 +235,7 @@ struct lsdem_ctx {
   DBuf<NodeDev> dNodes;
   DBuf<float4> dNodes32, dPairMap;
   DBuf<unsigned long long> dSurMask;
-  DBuf<ClusterF32> dClusters32;
+  DBuf<ClusterF32> dClusters32, dSubs32;
   DBuf<d3> dXBuild, dXBuildW, dWBuildPos;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
@@ -292,7 +312,7 @@ struct lsdem_ctx {
 };
 
 void lsdem_ctx::releaseAll() {
-  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dPairMap.release(); dSurMask.release(); dClusters32.release(); dSdf.release();
+  dShapes.release(); dClusters.release(); dNodes.release(); dNodes32.release(); dPairMap.release(); dSurMask.release(); dClusters32.release(); dSubs32.release(); dSdf.release();
   dCPair.release(); dCF.release(); dCTA.release(); dCTB.release();
   dPos.release(); dPosDet.release(); dVel.release(); dAng.release(); dForce.release(); dTorque.release();
   dGorig.release(); dQuat.release(); dInvI.release();
@@ -731,11 +751,25 @@ int lsdem_initialize(lsdem_ctx* c) {
     std::vector<float4> n32(nds.size());
     for (size_t k = 0; k < nds.size(); ++k) n32[k] = make_float4(float(nds[k].x), float(nds[k].y), float(nds[k].z), 0.f);
     if ((rc = upload(c->dNodes32, n32, s))) return rc;
-    std::vector<ClusterF32> c32(cls.size());
-    for (size_t k = 0; k < cls.size(); ++k)
-      c32[k] = ClusterF32{float(cls[k].cx), float(cls[k].cy), float(cls[k].cz), float(cls[k].rho * (1.0 + 1e-6)),
-                          cls[k].first, cls[k].count, 0, 0};
+    // FP32 copies; sub-cluster ranges of a cluster become global indices into the sub-cluster array
+    std::vector<ClusterF32> c32(cls.size()), s32;
+    std::vector<int> subBase(nS + 1, 0);
+    for (int k = 0; k < nS; ++k) subBase[k + 1] = subBase[k] + int(c->shapes[k].subs.size());
+    for (int k = 0; k < nS; ++k) {
+      const HostShape& Sh = c->shapes[k];
+      for (size_t u = 0; u < Sh.subs.size(); ++u) {
+        const ClusterDev& d = Sh.subs[u];
+        s32.push_back(ClusterF32{float(d.cx), float(d.cy), float(d.cz), float(d.rho * (1.0 + 1e-6)), d.first, d.count,
+                                 0, 0});
+      }
+      for (size_t cc = 0; cc < Sh.clusters.size(); ++cc) {
+        const ClusterDev& d = Sh.clusters[cc];
+        c32[sd[k].clusterOff + cc] = ClusterF32{float(d.cx), float(d.cy), float(d.cz), float(d.rho * (1.0 + 1e-6)),
+                                                d.first, d.count, subBase[k] + d.subFirst, d.nSub};
+      }
+    }
     if ((rc = upload(c->dClusters32, c32, s))) return rc;
+    if ((rc = upload(c->dSubs32, s32, s))) return rc;
   }
   int maxClusters = 1;
   for (int k = 0; k < nS; ++k) maxClusters = std::max(maxClusters, sd[k].nClusters);
@@ -935,7 +969,7 @@ int lsdem_initialize(lsdem_ctx* c) {
 
   // ---- pointer table ----
   P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
-  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.nodeCluster = c->dNodeCluster.p;
+  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.subs32 = c->dSubs32.p; P.nodeCluster = c->dNodeCluster.p;
   P.surSlot = nullptr; P.surBand = 0.f; P.sur = SurNet{}; P.maxClusters = maxClusters;
   P.cPair = c->dCPair.p; P.cF = c->dCF.p; P.cTA = c->dCTA.p; P.cTB = c->dCTB.p;
   P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;

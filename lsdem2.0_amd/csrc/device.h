@@ -22,11 +22,13 @@ struct ShapeDev {
 };
 struct ClusterDev {
   double cx, cy, cz, rho;  // bounding sphere of the cluster's nodes (unscaled)
-  int first, count, pad0, pad1;
+  int first, count;        // node range in the shape's cluster-ordered nodes
+  int subFirst, nSub;      // sub-clusters (<= 16 nodes each)
 };
 struct ClusterF32 {
   float cx, cy, cz, rho;  // FP32 copy of the bounding sphere, rho rounded up
-  int first, count, pad0, pad1;
+  int first, count;
+  int subFirst, nSub;     // clusters: global index of the first sub-cluster and their number
 };
 struct NodeDev {
   double x, y, z;  // unscaled local position
@@ -65,6 +67,7 @@ struct Params {
   const ClusterDev* clusters;
   const NodeDev* nodes;
   const ClusterF32* clusters32;
+  const ClusterF32* subs32;  // sub-clusters of <= 16 nodes
   float4* pairMap;        // per candidate pair: 128-byte FP32 map record (8 float4)
   const float4* nodes32;  // same nodes in FP32 (x, y, z, 0) for the conservative prefilter
   const float* sdf;

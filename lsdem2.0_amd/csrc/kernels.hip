@@ -977,110 +977,130 @@ __device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const Pai
   return phi <= f.phiMargin + kRelPhi * cmax;
 }
 
-constexpr int kBatch = 8;            // candidate pairs per wave batch
-constexpr int kCandUnroll = 4;       // entries whose node loads are in flight together
+constexpr int kBatch = 8;  // candidate pairs per wave batch
+constexpr int kMaxSub = 4;  // sub-clusters per cluster (64 / 16)
 
-// Per-wave LDS of k_candidates: pair maps, surviving (pair, cluster) entries, their node masks,
-// per-pair candidate counts.
+// Per-wave LDS of k_candidates: pair maps, surviving (pair, cluster) entries, surviving
+// (pair, sub-cluster) entries with their 16-bit node masks, per-pair counts and offsets.
 __host__ __device__ inline int candWaveBytes(int maxClusters) {
-  const int b = int(sizeof(PairF32)) * kBatch + 16 * kBatch * maxClusters + 8 * kBatch;
+  const int nEnt = kBatch * maxClusters, nSub = kMaxSub * nEnt;
+  const int b = int(sizeof(PairF32)) * kBatch + 8 * nEnt + 8 * nSub + 4 * nSub + 8 * kBatch + 16;
   return (b + 15) & ~15;
 }
 size_t candidatesLdsBytes(int maxClusters) { return size_t(candWaveBytes(maxClusters)) * (kNarrowBlock / 64); }
 
-// Candidate nodes: one wavefront per batch of 8 pairs.  Stage 1 (lanes = pair x cluster) maps
-// each pair into the partner's grid in FP32 and culls clusters; the survivors become entries in
-// LDS, ordered by pair.  Stage 2 (lanes = nodes of an entry, several entries in flight) applies
-// the node prefilter.  The batch then reserves its exact candidate count with one atomic and
-// writes each pair's candidates to a contiguous range.
+// Candidate nodes: one wavefront per batch of 8 pairs, three levels of conservative FP32 culls.
+//   1. lanes = (pair, cluster of <= 64 nodes): bounding spheres, grid box, SDF Lipschitz bound at
+//      the cluster centre (or the surrogate's decision); survivors become entries;
+//   2. lanes = (entry, sub-cluster of <= 16 nodes), 16 entries per pass: the same tests with the
+//      sub-cluster's smaller sphere; survivors become sub-entries;
+//   3. lanes = (sub-entry, node), 4 sub-entries per pass: the node prefilter.
+// The batch reserves its exact candidate count with one atomic and writes each pair's candidates
+// to a contiguous range, in (cluster, sub-cluster, node) order.
 template <bool SUR>
 __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   extern __shared__ __align__(16) unsigned char candLds[];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int maxC = P.maxClusters;
-  const int entCap = kBatch * maxC;
+  const int entCap = kBatch * maxC, subCap = kMaxSub * entCap;
   unsigned char* wb = candLds + wl * candWaveBytes(maxC);
   PairF32* sF = reinterpret_cast<PairF32*>(wb);
-  unsigned long long* sMask = reinterpret_cast<unsigned long long*>(wb + sizeof(PairF32) * kBatch);
-  int2* sEnt = reinterpret_cast<int2*>(sMask + entCap);
-  int* sCnt = reinterpret_cast<int*>(sEnt + entCap);
+  int2* sEnt = reinterpret_cast<int2*>(wb + sizeof(PairF32) * kBatch);  // (cluster index, pair j)
+  int2* sSub = sEnt + entCap;                                            // (first node, count | j << 16)
+  unsigned int* sMask = reinterpret_cast<unsigned int*>(sSub + subCap);  // 16-bit node masks
+  int* sCnt = reinterpret_cast<int*>(sMask + subCap);
   int* sOff = sCnt + kBatch;
-  const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int* sBase4 = sOff + kBatch;  // write offsets of the 4 sub-entries of a pass
+  const int q = lane >> 4, b16 = lane & 15;
+  const unsigned int lt16 = (1u << b16) - 1u;
   unsigned long long statCand = 0, statEnt = 0;
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
   const int iters = (maxC + 7) >> 3;
   const int j = lane >> 3, k = lane & 7;
+  const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
-    // ---- stage 1: cluster cull ----
+    // ---- level 1: clusters ----
     const int p = base + j;
     const bool own = p < nPairs;
     PairF32 f;
-    int nodeOff = 0, clusterOff = 0, nCl = 0;
+    int clusterOff = 0, nCl = 0;
     if (own) {
       f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
-      nodeOff = f.nodeOff;
       clusterOff = f.clusterOff;
       nCl = f.nClusters;
       if (k == 0) sF[j] = f;
     }
     if (lane < kBatch) sCnt[lane] = 0;
-    // entries in (cluster group, pair, cluster) order; a pair's entries keep cluster order
     int E = 0;
     for (int it = 0; it < iters; ++it) {
       const int c = it * 8 + k;
       bool keep = false;
-      int cFirst = 0, cCount = 0;
-      if (!SUR) {
-        if (own && c < nCl) {
-          const ClusterF32 cl = P.clusters32[clusterOff + c];
-          keep = clusterF32(P.sdf, f, cl, P.sdfCull != 0);
-          cFirst = nodeOff + cl.first;
-          cCount = cl.count;
-        }
-      } else {
-        // the network's decisions for this pair's clusters come from k_sur_masks
-        if (own && c < nCl) {
-          const ClusterF32 cl = P.clusters32[clusterOff + c];
-          float gx, gy, gz, rho;
-          bool inside;
-          keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
-          cFirst = nodeOff + cl.first;
-          cCount = cl.count;
-          if (keep && inside) {
-            if (f.slot >= 0) keep = (P.surMask[p] >> c) & 1ull;
-            else keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho);
-          }
+      if (own && c < nCl) {
+        const ClusterF32 cl = P.clusters32[clusterOff + c];
+        float gx, gy, gz, rho;
+        bool inside;
+        keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
+        if (keep && inside) {
+          if (SUR && f.slot >= 0) keep = (P.surMask[p] >> c) & 1ull;
+          else if (P.sdfCull) keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho);
         }
       }
       const unsigned long long m = __ballot(keep);
-      if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(cFirst, cCount | (j << 16));
+      if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(clusterOff + c, j);
       E += __popcll(m);
     }
     statEnt += (lane == 0) ? (unsigned long long)E : 0ull;
-    // ---- stage 2: node prefilter ----
-    int total = 0;
-    for (int e0 = 0; e0 < E; e0 += kCandUnroll) {
-      int2 ent[kCandUnroll];
-      float4 nd[kCandUnroll];
-#pragma unroll
-      for (int u = 0; u < kCandUnroll; ++u) ent[u] = (e0 + u < E) ? sEnt[e0 + u] : make_int2(0, 0);
-#pragma unroll
-      for (int u = 0; u < kCandUnroll; ++u)
-        nd[u] = (lane < (ent[u].y & 0xFFFF)) ? P.nodes32[ent[u].x + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int u = 0; u < kCandUnroll; ++u) {
-        const bool cand = lane < (ent[u].y & 0xFFFF) && nodeF32(P.sdf, sF[ent[u].y >> 16], nd[u]);
-        const unsigned long long cm = __ballot(cand);
-        const int n = __popcll(cm);
-        total += n;
-        if (lane == 0 && e0 + u < E) {
-          sMask[e0 + u] = cm;
-          sCnt[ent[u].y >> 16] += n;
+    __builtin_amdgcn_wave_barrier();
+    // ---- level 2: sub-clusters, 16 entries x 4 per pass ----
+    int NS = 0;
+    for (int e0 = 0; e0 < E; e0 += 16) {
+      const int e = e0 + (lane >> 2), u = lane & 3;
+      bool keep = false;
+      int first = 0, count = 0, jj = 0;
+      if (e < E) {
+        const int2 en = sEnt[e];
+        const ClusterF32 cl = P.clusters32[en.x];
+        jj = en.y;
+        if (u < cl.nSub) {
+          const ClusterF32 sc = P.subs32[cl.subFirst + u];
+          const PairF32& fj = sF[jj];
+          float gx, gy, gz, rho;
+          bool inside;
+          keep = clusterGeomF32(fj, sc, gx, gy, gz, rho, inside);
+          // with the surrogate the network has decided on the whole cluster; the sub-cluster keeps
+          // only its geometric tests then
+          if (keep && inside && !(SUR && fj.slot >= 0) && P.sdfCull) keep = clusterSdfF32(P.sdf, fj, gx, gy, gz, rho);
+          first = fj.nodeOff + sc.first;
+          count = sc.count;
         }
       }
+      const unsigned long long m = __ballot(keep);
+      if (keep) sSub[NS + __popcll(m & ltMask)] = make_int2(first, count | (jj << 16));
+      NS += __popcll(m);
     }
+    __builtin_amdgcn_wave_barrier();
+    // ---- level 3: nodes, 4 sub-entries x 16 per pass ----
+    int total = 0;
+    for (int e0 = 0; e0 < NS; e0 += 4) {
+      const int e = e0 + q;
+      bool cand = false;
+      int jj = 0;
+      if (e < NS) {
+        const int2 se = sSub[e];
+        jj = se.y >> 16;
+        if (b16 < (se.y & 0xFFFF)) cand = nodeF32(P.sdf, sF[jj], P.nodes32[se.x + b16]);
+      }
+      const unsigned long long m = __ballot(cand);
+      const unsigned int mq = unsigned((m >> (16 * q)) & 0xFFFFull);
+      if (b16 == 0 && e < NS) {
+        sMask[e] = mq;
+        atomicAdd(&sCnt[jj], __popc(mq));  // LDS: 4 groups may share a pair
+      }
+      total += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
     // ---- one reservation per batch, exact size ----
     int first = 0;
     if (total > 0) {
@@ -1098,20 +1118,29 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       P.pairCcnt[base + lane] = total ? sCnt[lane] : 0;
       sOff[lane] = off;  // running write offset of each pair
     }
+    __builtin_amdgcn_wave_barrier();
     // ---- write candidate slots; a pair's candidates are contiguous ----
     if (total) {
-      for (int e = 0; e < E; ++e) {
-        const unsigned long long cm = sMask[e];
-        if (!cm) continue;
-        const int2 en = sEnt[e];
-        const int jj = en.y >> 16;
-        const int o = sOff[jj];
-        if ((cm >> lane) & 1ull) {
-          const int dst = o + __popcll(cm & ltMask);
-          P.cPair[dst] = base + jj;
-          P.cNode[dst] = en.x + lane;  // cluster-ordered node index until the exact test
+      for (int e0 = 0; e0 < NS; e0 += 4) {
+        if (lane == 0) {
+          for (int t = 0; t < 4 && e0 + t < NS; ++t) {
+            const int jj = sSub[e0 + t].y >> 16;
+            sBase4[t] = sOff[jj];
+            sOff[jj] += __popc(sMask[e0 + t]);
+          }
         }
-        if (lane == 0) sOff[jj] = o + __popcll(cm);
+        __builtin_amdgcn_wave_barrier();
+        const int e = e0 + q;
+        if (e < NS) {
+          const unsigned int mq = sMask[e];
+          if ((mq >> b16) & 1u) {
+            const int2 se = sSub[e];
+            const int dst = sBase4[q] + __popc(mq & lt16);
+            P.cPair[dst] = base + (se.y >> 16);
+            P.cNode[dst] = se.x + b16;  // cluster-ordered node index until the exact test
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
       }
     }
     statCand += (lane == 0) ? (unsigned long long)total : 0ull;
