@@ -1024,15 +1024,17 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     // ---- level 1: clusters ----
     const int p = base + j;
     const bool own = p < nPairs;
-    PairF32 f;
-    int clusterOff = 0, nCl = 0;
+    // the 8 lanes of a pair copy its 160-byte map record into LDS, 20 bytes each
     if (own) {
-      f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
-      clusterOff = f.clusterOff;
-      nCl = f.nClusters;
-      if (k == 0) sF[j] = f;
+      const unsigned int* src = reinterpret_cast<const unsigned int*>(P.pairMap) + (long long)p * 40 + 5 * k;
+      unsigned int* dst = reinterpret_cast<unsigned int*>(&sF[j]) + 5 * k;
+#pragma unroll
+      for (int w = 0; w < 5; ++w) dst[w] = src[w];
     }
     if (lane < kBatch) sCnt[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    const PairF32& f = sF[j];
+    const int clusterOff = own ? f.clusterOff : 0, nCl = own ? f.nClusters : 0;
     int E = 0;
     for (int it = 0; it < iters; ++it) {
       const int c = it * 8 + k;
