@@ -165,7 +165,8 @@ def main():
     log("[bench] phases ms/step: " + " ".join(f"{k}={v / K:.4f}" for k, v in prof["ms"].items()))
     log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} master_nodes={prof['master_nodes'] / K:.0f} "
         f"cluster_tests={prof['cluster_tests'] / K:.0f} node_tests={prof['node_tests'] / K:.0f} "
-        f"sdf_queries={prof['sdf_queries'] / K:.0f} contacts={prof['contacts'] / K:.0f}")
+        f"sdf_queries={prof['sdf_queries'] / K:.0f} contacts={prof['contacts'] / K:.0f} "
+        f"evaluated_pairs={prof['evaluated_pairs'] / K:.0f} rebuilds={prof['rebuilds']}")
     del sim
 
     sur = None
@@ -216,8 +217,8 @@ def main():
         "cpu_baseline": base,
         "surrogate": sur,
         "phases_ms_per_step": {k: v / K for k, v in prof["ms"].items()},
-        "work_per_step": {k: prof[k] / K for k in ("pairs", "master_nodes", "cluster_tests", "node_tests",
-                                                    "sdf_queries", "contacts")},
+        "work_per_step": {k: prof[k] / K for k in ("pairs", "evaluated_pairs", "master_nodes", "cluster_tests",
+                                                    "node_tests", "sdf_queries", "contacts", "rebuilds")},
         "node_sdf_queries_per_s": prof["sdf_queries"] / K / (ms_step * 1e-3),
     }
     print(json.dumps(out))

@@ -61,7 +61,9 @@ typedef struct lsdem_config {
   double skin;            /* candidate pairs are kept for reuse within this distance and rebuilt
                              when a body has moved half of it (< 0: 2 % of the largest bounding
                              radius; 0: rebuilt every step).  Contacts do not depend on it. */
-  int32_t reserved[4];
+  int32_t no_certificates; /* 1: evaluate every candidate pair every step (no separation
+                              certificates); contacts do not depend on it */
+  int32_t reserved[3];
 } lsdem_config;
 
 typedef struct lsdem_counters {
@@ -166,7 +168,10 @@ typedef struct lsdem_profile {
   int64_t pairs, node_tests, sdf_queries, contacts; /* sums over the profiled steps */
   int64_t cluster_tests;       /* node clusters passed to the node prefilter, summed */
   int64_t master_nodes;        /* boundary nodes of the masters of all candidate pairs, summed */
-  int64_t reserved[6];
+  int64_t evaluated_pairs;     /* pairs the narrow phase evaluated (the rest held a separation
+                                  certificate or had no partner), summed */
+  int64_t rebuilds;            /* steps that rebuilt the candidate pairs */
+  int64_t reserved[4];
 } lsdem_profile;
 /* Start recording; at most max_steps steps are timed (events are preallocated). */
 int lsdem_profile_begin(lsdem_ctx* ctx, int32_t max_steps);

@@ -36,6 +36,7 @@ __global__ void k_rc_scatter(Params P);
 __global__ void k_hash_insert(Params P);
 __global__ void k_hash_clear(Params P);
 __global__ void k_gather_particles(Params P);
+__global__ void k_pair_check(Params P, int full);
 __global__ void k_pair_maps(Params P);
 template <bool SUR> __global__ void k_candidates(Params P);
 __global__ void k_sur_audit(Params P);
@@ -237,6 +238,10 @@ struct lsdem_ctx {
   DBuf<unsigned long long> dSurMask;
   DBuf<ClusterF32> dClusters32, dSubs32;
   DBuf<d3> dXBuild, dXBuildW, dWBuildPos;
+  DBuf<q4> dQBuild;
+  DBuf<int> dImgBuild, dImgFlip, dActive, dPairRec;
+  DBuf<float4> dMotion, dWMotion, dCert;
+  DBuf<float> dSurSlack;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
   DBuf<int> dNodeCluster;
@@ -341,6 +346,8 @@ void lsdem_ctx::releaseAll() {
   dDevInt.release(); dScanBlocks.release(); dDevStat.release();
   dNodeCluster.release();
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
+  dQBuild.release(); dImgBuild.release(); dImgFlip.release(); dActive.release(); dPairRec.release();
+  dMotion.release(); dWMotion.release(); dCert.release(); dSurSlack.release();
   sur.release();
 }
 
@@ -476,10 +483,11 @@ int runStep(lsdem_ctx* c) {
   hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, s, (const int*)P.rcStart,
                      (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2), gate);
   phase();
-  // narrow phase + contact law
-  hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  // narrow phase + contact law (an audit step evaluates every live pair)
   const bool useSur = c->sur.enabled && c->sur.trained;
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
+  hipLaunchKernelGGL(k_pair_check, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P, audit ? 1 : 0);
+  hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
   if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
   if (useSur && !audit)
     hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
@@ -719,6 +727,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.dt = c->cfg.dt;
   P.quirks = c->cfg.quirks;
   P.sdfCull = c->cfg.no_sdf_cull ? 0 : 1;
+  P.useCert = c->cfg.no_certificates ? 0 : 1;
   P.periodicX = c->cfg.periodic_x != 0;
   P.periodicY = c->cfg.periodic_y != 0;
   const d3 dmn = mk3(c->cfg.domain_min[0], c->cfg.domain_min[1], c->cfg.domain_min[2]);
@@ -738,6 +747,8 @@ int lsdem_initialize(lsdem_ctx* c) {
     d.sdfOff = (long long)sdf.size();
     d.ox = S.origin.x; d.oy = S.origin.y; d.oz = S.origin.z; d.h = S.h;
     d.boxR = S.boxR; d.lip = S.lip;
+    for (const d3& n : S.nodes) d.nodeR = std::max(d.nodeR, len(n));
+    d.nodeR *= 1.0 + 1e-12;
     if (d.nClusters > 64) return fail(LSDEM_ERR_ARG, "shape has more than 4096 boundary nodes");
     nds.insert(nds.end(), S.perm.begin(), S.perm.end());
     cls.insert(cls.end(), S.clusters.begin(), S.clusters.end());
@@ -822,6 +833,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dKeyRef.alloc(std::max(nP, 1)));
   HIPCHK(c->dXBuild.alloc(std::max(nP, 1)));
   HIPCHK(c->dXBuildW.alloc(std::max(nP, 1)));
+  HIPCHK(c->dQBuild.alloc(std::max(nP, 1)));
+  HIPCHK(c->dImgBuild.alloc(std::max(nP, 1)));
+  HIPCHK(c->dImgFlip.alloc(std::max(nP, 1)));
+  HIPCHK(c->dMotion.alloc(std::max(nP, 1)));
   HIPCHK(c->dScell.alloc(std::max(nP, 1)));
 
   // ---- walls ----
@@ -835,6 +850,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dWKey.alloc(std::max(nW, 1)));
   HIPCHK(c->dWBuildPos.alloc(std::max(nW, 1)));
   HIPCHK(c->dWBuildQ.alloc(std::max(nW, 1)));
+  HIPCHK(c->dWMotion.alloc(std::max(nW, 1)));
 
   // ---- grids ----
   double cellW = 0;
@@ -900,6 +916,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairCcnt.alloc(pairCap));
   HIPCHK(c->dPairMap.alloc(pairCap * 10));
   HIPCHK(c->dSurMask.alloc(pairCap));
+  HIPCHK(c->dSurSlack.alloc(pairCap));
+  HIPCHK(c->dCert.alloc(pairCap));
+  HIPCHK(c->dActive.alloc(pairCap));
+  HIPCHK(c->dPairRec.alloc(pairCap));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
@@ -957,11 +977,11 @@ int lsdem_initialize(lsdem_ctx* c) {
   long long maxScan = std::max<long long>({(long long)nP, (long long)ncell, 1ll});
   c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
   HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
-  HIPCHK(c->dDevInt.alloc(8));
-  HIPCHK(c->dDevStat.alloc(13));
+  HIPCHK(c->dDevInt.alloc(kDevInts));
+  HIPCHK(c->dDevStat.alloc(kDevStats));
   HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 5 * kStatSlots * sizeof(unsigned long long), s));
-  HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, 8 * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, 13 * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, kDevInts * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, kDevStats * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
@@ -978,6 +998,9 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
   P.nodeStart = c->dNodeStart.p; P.keyRef = c->dKeyRef.p; P.scell = c->dScell.p;
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
+  P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.imgFlip = c->dImgFlip.p; P.motion = c->dMotion.p;
+  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.active = c->dActive.p; P.pairRec = c->dPairRec.p;
+  P.surSlack = c->dSurSlack.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
   for (int g = 0; g < 3; ++g) {
@@ -1193,7 +1216,7 @@ int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemset(c->dDevStat.p + 3, 0, 6 * sizeof(unsigned long long)));
-  HIPCHK(hipMemset(c->dDevStat.p + 12, 0, sizeof(unsigned long long)));
+  HIPCHK(hipMemset(c->dDevStat.p + 12, 0, 3 * sizeof(unsigned long long)));
   c->profMax = max_steps;
   c->profSteps = 0;
   c->profiling = true;
@@ -1215,10 +1238,12 @@ int lsdem_profile_end(lsdem_ctx* c, lsdem_profile* out) {
       out->ms[ph] += ms;
     }
   }
-  unsigned long long ds[13];
+  unsigned long long ds[kDevStats];
   HIPCHK(hipMemcpy(ds, c->dDevStat.p, sizeof ds, hipMemcpyDeviceToHost));
   out->cluster_tests = (int64_t)ds[8];
   out->master_nodes = (int64_t)ds[12];
+  out->evaluated_pairs = (int64_t)ds[13];
+  out->rebuilds = (int64_t)ds[14];
   out->pairs = (int64_t)ds[3];
   out->node_tests = (int64_t)ds[4];
   out->sdf_queries = (int64_t)ds[5];

@@ -19,6 +19,7 @@ struct ShapeDev {
   double boxR;           // max |corner| of the grid box (unscaled), used for fixed-partner culls
   double lip;            // Lipschitz bound of the trilinear interpolant over cells with a positive
                          // corner (scale invariant)
+  double nodeR;          // max |node| (unscaled): how far a node lies from the body origin
 };
 struct ClusterDev {
   double cx, cy, cz, rho;  // bounding sphere of the cluster's nodes (unscaled)
@@ -68,7 +69,7 @@ struct Params {
   const NodeDev* nodes;
   const ClusterF32* clusters32;
   const ClusterF32* subs32;  // sub-clusters of <= 16 nodes
-  float4* pairMap;        // per candidate pair: 128-byte FP32 map record (8 float4)
+  float4* pairMap;        // per evaluated pair: 160-byte FP32 map record (10 float4)
   const float4* nodes32;  // same nodes in FP32 (x, y, z, 0) for the conservative prefilter
   const float* sdf;
   int maxClusters;        // largest cluster count of any shape
@@ -97,6 +98,11 @@ struct Params {
   int* scell;            // search-grid cell of pos (post-wrap)
   d3* xBuild;            // posDet when the candidate pairs were built
   d3* xBuildW;           // pos when the candidate pairs were built
+  q4* qBuild;            // quat when the candidate pairs were built
+  int* imgBuild;         // image shift states at the build (bit g: image g lies a period away)
+  float4* motion;        // displacement of posDet since the build (x, y, z) and the chord
+                         // 2 sin(angle / 2) of the rotation since the build (w)
+  int* imgFlip;          // bit g: image g's shift state differs from the build's
 
   // ---- walls ----
   d3* wpos;
@@ -109,6 +115,7 @@ struct Params {
   int* wkey;
   d3* wBuildPos;         // wall pose when the candidate pairs were built
   q4* wBuildQ;
+  float4* wMotion;       // as motion, for walls
 
   // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
   d3* gpos[3];
@@ -133,6 +140,12 @@ struct Params {
   uint32_t* pairInfo;  // (type << 28) | partner
   int* pairCoff;    // first contact slot of the pair (its contacts are contiguous)
   int* pairCcnt;    // contacts of the pair
+  // separation certificates: a pair whose nodes were all culled with slack s (world units, less
+  // the motion bound at that step) is skipped while the motion bound since the build stays below s
+  float4* cert;     // pairCap: (s or -1, cB, cA, 0); bound = |dA - dB| + chordB cB + chordA cA
+  int* active;      // pairs the narrow phase evaluates this step (devInt[9] of them)
+  int* pairRec;     // pair -> index of its map record this step
+  int useCert;
   // reaction CSR (pairs whose partner is a particle that receives the reaction)
   int* rcCount;     // nP
   int* rcStart;     // nP + 1
@@ -174,12 +187,15 @@ struct Params {
 
   // ---- device counters ----
   // devInt: [1] contact slots allocated this step, [3] overflow flags (1 pairs, 2 contacts),
-  //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert
+  //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert,
+  //         [7] images at their source's own position, [8] rebuild flag of the step,
+  //         [9] pairs the narrow phase evaluates this step
   int* devInt;
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
   //          (summed while profiling); [9] audits, [10] audited contacts, [11] missed contacts
-  //          (cumulative); [12] master nodes of the candidate pairs (summed while profiling)
+  //          (cumulative); [12] master nodes of the candidate pairs, [13] evaluated pairs,
+  //          [14] rebuild steps (summed while profiling)
   unsigned long long* devStat;
   unsigned long long* statSlots;  // kStatSlots x 5 spread accumulators of the narrow phase
 
@@ -188,10 +204,13 @@ struct Params {
   const int* surSlot;      // family slot of each shape id, -1 outside the family; nullptr: untrained
   float surBand;           // cull band, unscaled SDF units (2 x measured max error)
   const int* nodeCluster;  // per shape node (caller order, at the shape's node offset): its cluster
-  unsigned long long* surMask;  // per candidate pair: bit c = 0 when the network culls cluster c
+  unsigned long long* surMask;  // per map record: bit c = 0 when the network culls cluster c
+  float* surSlack;              // per map record: smallest slack of the clusters the network culled
 };
 
 constexpr int kStatSlots = 32;
+constexpr int kDevInts = 16;   // devInt entries
+constexpr int kDevStats = 16;  // devStat entries
 constexpr int kPairSlabHost = 16;  // slab depth of the broad phase (kPairSlab in the kernels)
 constexpr uint32_t kInvalidInfo = 0xFFFFFFFFu;  // unused contact slot
 

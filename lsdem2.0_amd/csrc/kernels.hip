@@ -204,6 +204,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, long
 // ------------------------------------------------------------------------------------------
 // Integration: first half-kick + drift (+ keys, wrap, ghost images)
 // ------------------------------------------------------------------------------------------
+// Chord 2 sin(angle / 2) of the rotation from unit quaternion b to unit quaternion q: the largest
+// displacement of a unit vector between the two orientations (|R(q) - R(b)| in the 2-norm).
+__device__ __forceinline__ double relChord(q4 b, q4 q) {
+  const double vx = b.q0 * q.q1 - q.q0 * b.q1 - (b.q2 * q.q3 - b.q3 * q.q2);
+  const double vy = b.q0 * q.q2 - q.q0 * b.q2 - (b.q3 * q.q1 - b.q1 * q.q3);
+  const double vz = b.q0 * q.q3 - q.q0 * b.q3 - (b.q1 * q.q2 - b.q2 * q.q1);
+  return 2.0 * sqrt(vx * vx + vy * vy + vz * vz);
+}
+
+// Which ghost images of a particle at (wrapped) position p lie a period away from it: bit 0 the
+// +x image (first reference-grid column), bit 1 the +y image, bit 2 the corner image.
+__device__ __forceinline__ int imgShiftMask(const Params& P, d3 p) {
+  const i3 c = refCell(P.gridP, p);
+  return (c.x == 0 ? 1 : 0) | (c.y == 0 ? 2 : 0) | (c.x == 0 && c.y == 0 ? 4 : 0);
+}
+
 __device__ __forceinline__ void halfKick(const Params& P, int i, d3 F, d3 T) {
   const double invM = P.invM[i];
   if (isZero(invM)) return;
@@ -277,6 +293,10 @@ __global__ void k_kick_drift(Params P) {
   const d3 dd = P.posDet[i] - P.xBuild[i], dw = p - P.xBuildW[i];
   const bool moved = len2(dd) > hs2 || len2(dw) > hs2;
   if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[8], 1);
+  // motion since the build for the separation certificates (a step that wraps rebuilds, so posDet
+  // and pos move alike on every step that reads this)
+  P.motion[i] = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(P.qBuild[i], P.quat[i])));
+  if (periodic) P.imgFlip[i] = imgShiftMask(P, p) ^ P.imgBuild[i];
 }
 
 __global__ void k_walls(Params P) {
@@ -290,13 +310,12 @@ __global__ void k_walls(Params P) {
   P.wkey[w] = refKey(P.gridW, p);
   // displacement bound of any point of the wall's grid box since the last build: translation plus
   // 2 R sin(angle / 2) for the rotation
-  const q4 b = P.wBuildQ[w];
-  const double vx = b.q0 * q.q1 - q.q0 * b.q1 - (b.q2 * q.q3 - b.q3 * q.q2);
-  const double vy = b.q0 * q.q2 - q.q0 * b.q2 - (b.q3 * q.q1 - b.q1 * q.q3);
-  const double vz = b.q0 * q.q3 - q.q0 * b.q3 - (b.q1 * q.q2 - b.q2 * q.q1);
+  const double chord = relChord(P.wBuildQ[w], q);
   const double R = P.shapes[P.wshape[w]].boxR;
-  const double disp = len(p - P.wBuildPos[w]) + 2.0 * R * sqrt(vx * vx + vy * vy + vz * vz);
+  const d3 dp = p - P.wBuildPos[w];
+  const double disp = len(dp) + R * chord;
   if (disp > P.halfSkin) atomicOr(&P.devInt[8], 1);
+  P.wMotion[w] = make_float4(float(dp.x), float(dp.y), float(dp.z), float(chord));
 }
 
 // Record the poses the candidate pairs were built from (rebuild steps only).
@@ -306,10 +325,17 @@ __global__ void k_mark_build(Params P) {
   if (i < P.nP) {
     P.xBuild[i] = P.posDet[i];
     P.xBuildW[i] = P.pos[i];
+    P.qBuild[i] = P.quat[i];
+    P.motion[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (P.periodicX || P.periodicY) {
+      P.imgBuild[i] = imgShiftMask(P, P.pos[i]);
+      P.imgFlip[i] = 0;
+    }
   }
   if (i < P.nW) {
     P.wBuildPos[i] = P.wpos[i];
     P.wBuildQ[i] = P.wquat[i];
+    P.wMotion[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -709,11 +735,12 @@ __device__ __forceinline__ double wsum(double v) {
 
 // Detection-side view of a pair (everything the node tests need).
 struct PairView {
-  int A, partner, nodeBase, valid;
+  int pair, A, partner, nodeBase, valid;
   uint32_t type, info;
   d3 xA;        // master position used for detection
   q4 qA;
   double sA;
+  double rA;    // largest distance of a node of A from A's origin (scaled)
   int clusterOff, nClusters, nodeOff, shapeB;
   d3 xB;        // partner pose used for detection
   q4 qB;
@@ -725,6 +752,7 @@ struct PairView {
 
 __device__ __forceinline__ PairView loadPair(const Params& P, int p) {
   PairView v;
+  v.pair = p;
   v.info = P.pairInfo[p];
   v.type = v.info >> 28;
   v.partner = int(v.info & kPartnerMask);
@@ -737,6 +765,7 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p) {
   v.clusterOff = SA.clusterOff;
   v.nClusters = SA.nClusters;
   v.nodeOff = SA.nodeOff;
+  v.rA = v.sA * SA.nodeR;
   v.nodeBase = P.nodeStart[v.A];
   const int j = v.partner;
   if (v.type == PT_PW) {
@@ -817,10 +846,13 @@ struct alignas(16) PairF32 {
   float rhoScale;     // A's unscaled cluster radius -> g units
   float lipH;         // Lipschitz bound of the partner's SDF per g unit
   int nx, ny, nz, sdfOff;
-  int nodeOff, clusterOff, nClusters, pad0, pad1;  // A's shape
+  int nodeOff, clusterOff, nClusters;  // A's shape
+  int pair;           // candidate pair index
+  float gToWorld;     // world length of one g unit (the partner's scaled spacing)
   float o[3], h;      // partner's unscaled grid origin and spacing (g -> local point)
   int slot;           // partner's surrogate family slot, -1 if none (walls)
-  int pad2, pad3, pad4;
+  float certD0;       // motion bound of the pair since the build, at this step (inf: none)
+  float certCB, certCA;  // certificate coefficients of the partner's and the master's rotation
 };
 static_assert(sizeof(PairF32) == 160, "pair map record is 160 bytes");
 
@@ -858,34 +890,105 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
   f.nodeOff = v.nodeOff;
   f.clusterOff = v.clusterOff;
   f.nClusters = v.nClusters;
-  f.pad0 = 0;
-  f.pad1 = 0;
+  f.pair = v.pair;
+  f.gToWorld = float(1.0 / is);
   f.o[0] = float(SB.ox); f.o[1] = float(SB.oy); f.o[2] = float(SB.oz);
   f.h = float(SB.h);
   f.slot = (P.surSlot && v.type != PT_PW) ? P.surSlot[v.shapeB] : -1;
-  f.pad2 = f.pad3 = f.pad4 = 0;
+  // Separation-certificate coefficients.  A node u of A (|u| <= rA) sits at
+  // g = R_B^T (d + R_A u) / spacing, d = x_A - x_B.  Between the build and now it moved at most
+  //   |dA - dB| + chord_B |d_build| + (chord_A + chord_B) rA,   |d_build| <= |d| + |dA - dB|,
+  // with dA, dB the displacements and chord the rotation chords since the build.
+  const float4 mA = P.motion[v.A];
+  float4 mB;
+  bool flip = false;  // an image that changed its shift state has no motion bound
+  if (v.type == PT_PW) mB = P.wMotion[v.partner];
+  else {
+    mB = P.motion[v.partner];
+    if (v.type >= PT_GX) flip = (P.imgFlip[v.partner] >> (v.type - PT_GX)) & 1;
+  }
+  const double ex = double(mA.x) - double(mB.x), ey = double(mA.y) - double(mB.y), ez = double(mA.z) - double(mB.z);
+  const double del = sqrt(ex * ex + ey * ey + ez * ez);
+  const double cA = v.rA * (1.0 + 1e-6);
+  const double cB = (len(v.xA - v.xB) + del + cA) * (1.0 + 1e-6);
+  const double d0 = del + double(mB.w) * cB + double(mA.w) * cA;
+  f.certD0 = flip ? __builtin_huge_valf() : float(d0 * (1.0 + 1e-5));
+  f.certCB = float(cB * (1.0 + 1e-6));
+  f.certCA = float(cA * (1.0 + 1e-6));
 }
 
-// FP32 pair maps, one thread per candidate pair (FP64 poses in, 128-byte record out).
-__global__ void k_pair_maps(Params P) {
+// Which pairs the narrow phase evaluates this step, one thread per candidate pair.  A pair is
+// skipped when its partner does not exist now (pp: outside the domain; wall: its key; ghost: the
+// image) or when its separation certificate still holds: the motion bound of its nodes since the
+// build stays below the slack recorded at its last evaluation (all its nodes were culled then
+// with that much room).  Skipped pairs have no contacts.  full != 0 evaluates every live pair.
+__global__ void k_pair_check(Params P, int full) {
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
-  PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
+  const bool rebuilt = P.devInt[8] != 0;  // a new pair list: no certificate exists yet
+  const bool fresh = full || rebuilt || !P.useCert;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nPairs; p += gridDim.x * blockDim.x) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int p0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); p0 < nPairs; p0 += stride) {
+    const int p = p0 + lane;
+    bool act = false;
+    if (p < nPairs) {
+      const uint32_t info = P.pairInfo[p];
+      const uint32_t type = info >> 28;
+      const int j = int(info & kPartnerMask);
+      const int A = P.pairA[p];
+      const bool live = type == PT_PP ? P.keyRef[j] >= 0
+                                      : (type == PT_PW ? P.wkey[j] >= 0 : P.gkey[type - PT_GX][j] >= 0);
+      if (live) {
+        masterNodes += (unsigned long long)P.shapes[P.shape[A]].nNodes;
+        act = true;
+        if (!fresh) {
+          const float4 c = P.cert[p];
+          if (c.x > 0.f) {
+            const float4 mA = P.motion[A];
+            float4 mB;
+            bool flip = false;
+            if (type == PT_PW) mB = P.wMotion[j];
+            else {
+              mB = P.motion[j];
+              if (type >= PT_GX) flip = (P.imgFlip[j] >> (type - PT_GX)) & 1;
+            }
+            if (!flip) {
+              const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
+              const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * c.z) * (1.f + 1e-5f);
+              act = !(bound < c.x);
+            }
+          }
+        }
+      } else if (rebuilt) {
+        P.cert[p] = make_float4(-1.f, 0.f, 0.f, 0.f);
+      }
+      if (!act) P.pairCcnt[p] = 0;
+    }
+    const unsigned long long m = __ballot(act);
+    int first = 0;
+    if (lane == 0 && m) first = atomicAdd(&P.devInt[9], __popcll(m));
+    first = __shfl(first, 0, 64);
+    if (act) P.active[first + __popcll(m & ltMask)] = p;
+  }
+  const unsigned long long t = (unsigned long long)wsum(double(masterNodes));
+  if (lane == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
+}
+
+// FP32 pair maps of the pairs evaluated this step, one thread per pair (FP64 poses in, one
+// 160-byte record out, in evaluation order).
+__global__ void k_pair_maps(Params P) {
+  const int nAct = P.devInt[9];
+  PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nAct; a += gridDim.x * blockDim.x) {
+    const int p = P.active[a];
     const PairView v = loadPair(P, p);
     PairF32 f;
     makeF32(P, v, f);
-    // the pair list may be older than this step: the partner must exist now (pp: inside the
-    // domain; wall: its key; ghost: the image)
-    const int j = v.partner;
-    const bool live = v.type == PT_PP ? P.keyRef[j] >= 0
-                                      : (v.type == PT_PW ? P.wkey[j] >= 0 : P.gkey[v.type - PT_GX][j] >= 0);
-    if (!live) f.nClusters = 0;
-    out[p] = f;
-    if (live) masterNodes += (unsigned long long)P.shapes[P.shape[v.A]].nNodes;
+    out[a] = f;
+    P.pairRec[p] = a;
   }
-  const unsigned long long t = (unsigned long long)wsum(double(masterNodes));
-  if ((threadIdx.x & 63) == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
 }
 
 __device__ __forceinline__ void mapF32(const PairF32& f, float nx, float ny, float nz, float& gx, float& gy, float& gz) {
@@ -917,44 +1020,72 @@ __device__ __forceinline__ float phiF32(const float* __restrict__ sdf, const Pai
 // Cluster acceptance (conservative): the cluster's bounding sphere must reach the partner's
 // bounding sphere (particle partners) and grid box, and, when its centre lies in the box, the
 // partner's SDF at the centre must be within Lipschitz reach of zero.
-// Geometric part of the cluster test: bounding spheres and grid box.  `inside` reports whether the
-// centre lies in the grid box (then an SDF bound at the centre can cull the cluster).
+//
+// Every test below is a function of the g coordinates of A's nodes alone (the partner's sphere,
+// box and SDF are fixed in its frame).  When a test culls, `slack` (g units) is how far the
+// cluster's radius could grow with the same test still culling it: while no node has moved
+// further than that in the partner's frame, the test still rules out every node of the cluster.
+// Slacks are rounded down; a cull that holds by less than the rounding gives a slack <= 0.
+
+// Geometric part: bounding spheres and grid box.  `inside` reports whether the centre lies in the
+// grid box (then an SDF bound at the centre can cull the cluster).
 __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF32& cl, float& gx, float& gy, float& gz,
-                                               float& rho, bool& inside) {
+                                               float& rho, bool& inside, float& slack) {
   mapF32(f, cl.cx, cl.cy, cl.cz, gx, gy, gz);
   rho = cl.rho * f.rhoScale;
   inside = false;
+  slack = 0.f;
   if (f.r2 >= 0.f) {
     const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
+    const float d2 = dx * dx + dy * dy + dz * dz;
     const float r = sqrtf(f.r2) + rho;
-    if (dx * dx + dy * dy + dz * dz > r * r * 1.0001f) return false;
+    if (d2 > r * r * 1.0001f) {
+      slack = sqrtf(d2) * 0.9999f - r;
+      return false;
+    }
   }
   const float dx = fmaxf(0.f, fmaxf(f.lo - gx, gx - f.hi[0]));
   const float dy = fmaxf(0.f, fmaxf(f.lo - gy, gy - f.hi[1]));
   const float dz = fmaxf(0.f, fmaxf(f.lo - gz, gz - f.hi[2]));
-  if (dx * dx + dy * dy + dz * dz > rho * rho * 1.0001f) return false;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 > rho * rho * 1.0001f) {
+    slack = sqrtf(d2) * 0.9999f - rho;
+    return false;
+  }
   inside = dx == 0.f && dy == 0.f && dz == 0.f;
   return true;
 }
+// Radius growth that a Lipschitz bound phi > lipH * 1.001 * radius + lim still allows.
+__device__ __forceinline__ float lipSlack(const PairF32& f, float phi, float lim, float rho) {
+  return f.lipH > 0.f ? (phi - lim) / (f.lipH * 1.001f) * 0.9999f - rho : 0.f;
+}
 // SDF part with the exact grid: phi(centre) beyond Lipschitz reach of zero culls the cluster.
 __device__ __forceinline__ bool clusterSdfF32(const float* __restrict__ sdf, const PairF32& f, float gx, float gy,
-                                              float gz, float rho) {
+                                              float gz, float rho, float& slack) {
   float cmax;
   const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
-  return !(phi > f.lipH * rho * 1.001f + f.phiMargin + kRelPhi * cmax);
+  const float lim = f.phiMargin + kRelPhi * cmax;
+  if (phi > f.lipH * rho * 1.001f + lim) {
+    slack = lipSlack(f, phi, lim, rho);
+    return false;
+  }
+  return true;
 }
 __device__ __forceinline__ bool clusterF32(const float* __restrict__ sdf, const PairF32& f, const ClusterF32& cl,
                                            bool sdfCull) {
-  float gx, gy, gz, rho;
+  float gx, gy, gz, rho, slack;
   bool inside;
-  if (!clusterGeomF32(f, cl, gx, gy, gz, rho, inside)) return false;
-  return !inside || !sdfCull || clusterSdfF32(sdf, f, gx, gy, gz, rho);
+  if (!clusterGeomF32(f, cl, gx, gy, gz, rho, inside, slack)) return false;
+  return !inside || !sdfCull || clusterSdfF32(sdf, f, gx, gy, gz, rho, slack);
 }
 // Surrogate part: the same bound with the learned distance, widened by the band (twice the
 // surrogate's measured maximum error).  `d` is the network output in normalised units.
 __device__ __forceinline__ bool clusterSurKeep(const Params& P, const PairF32& f, float d, float rho) {
   const float phi = d * (1.0f / P.sur.invLen);
   return !(phi - P.surBand > f.lipH * rho * 1.001f + f.phiMargin);
+}
+__device__ __forceinline__ float clusterSurSlack(const Params& P, const PairF32& f, float d, float rho) {
+  return lipSlack(f, d * (1.0f / P.sur.invLen), P.surBand + f.phiMargin, rho);
 }
 // Lane's surrogate query for a cluster centre at grid coordinates g (normalised local point).
 __device__ __forceinline__ void surQueryPoint(const Params& P, const PairF32& f, float gx, float gy, float gz,
@@ -963,40 +1094,66 @@ __device__ __forceinline__ void surQueryPoint(const Params& P, const PairF32& f,
   uy = (f.o[1] + gy * f.h) * P.sur.invLen;
   uz = (f.o[2] + gz * f.h) * P.sur.invLen;
 }
-// Node acceptance (conservative superset of the exact FP64 test).
-__device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const PairF32& f, float4 n) {
+// Node acceptance (conservative superset of the exact FP64 test); slack as for clusters, with
+// the node as a cluster of radius 0.
+__device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const PairF32& f, float4 n, float& slack) {
   float gx, gy, gz;
   mapF32(f, n.x, n.y, n.z, gx, gy, gz);
+  slack = 0.f;
   if (f.r2 >= 0.f) {
     const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
-    if (dx * dx + dy * dy + dz * dz > f.r2) return false;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > f.r2) {
+      slack = sqrtf(d2) * 0.9999f - sqrtf(f.r2);
+      return false;
+    }
   }
-  if (gx < f.lo || gy < f.lo || gz < f.lo || gx > f.hi[0] || gy > f.hi[1] || gz > f.hi[2]) return false;
+  const float out = fmaxf(fmaxf(fmaxf(f.lo - gx, gx - f.hi[0]), fmaxf(f.lo - gy, gy - f.hi[1])),
+                          fmaxf(f.lo - gz, gz - f.hi[2]));
+  if (out > 0.f) {  // outside the grid box on some axis, by `out`
+    slack = out * 0.9999f;
+    return false;
+  }
   float cmax;
   const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
-  return phi <= f.phiMargin + kRelPhi * cmax;
+  const float lim = f.phiMargin + kRelPhi * cmax;
+  if (phi <= lim) return true;
+  slack = lipSlack(f, phi, lim, 0.f);
+  return false;
 }
 
 constexpr int kBatch = 8;  // candidate pairs per wave batch
 constexpr int kMaxSub = 4;  // sub-clusters per cluster (64 / 16)
 
 // Per-wave LDS of k_candidates: pair maps, surviving (pair, cluster) entries, surviving
-// (pair, sub-cluster) entries with their 16-bit node masks, per-pair counts and offsets.
+// (pair, sub-cluster) entries with their 16-bit node masks, per-pair counts, offsets and slacks.
 __host__ __device__ inline int candWaveBytes(int maxClusters) {
   const int nEnt = kBatch * maxClusters, nSub = kMaxSub * nEnt;
-  const int b = int(sizeof(PairF32)) * kBatch + 8 * nEnt + 8 * nSub + 4 * nSub + 8 * kBatch + 16;
+  const int b = int(sizeof(PairF32)) * kBatch + 8 * nEnt + 8 * nSub + 4 * nSub + 12 * kBatch + 16;
   return (b + 15) & ~15;
 }
 size_t candidatesLdsBytes(int maxClusters) { return size_t(candWaveBytes(maxClusters)) * (kNarrowBlock / 64); }
 
-// Candidate nodes: one wavefront per batch of 8 pairs, three levels of conservative FP32 culls.
+// Minimum over aligned groups of W lanes (W a power of two <= 64).
+template <int W>
+__device__ __forceinline__ float groupMin(float v) {
+#pragma unroll
+  for (int o = 1; o < W; o <<= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Candidate nodes: one wavefront per batch of 8 evaluated pairs, three levels of conservative
+// FP32 culls.
 //   1. lanes = (pair, cluster of <= 64 nodes): bounding spheres, grid box, SDF Lipschitz bound at
 //      the cluster centre (or the surrogate's decision); survivors become entries;
 //   2. lanes = (entry, sub-cluster of <= 16 nodes), 16 entries per pass: the same tests with the
 //      sub-cluster's smaller sphere; survivors become sub-entries;
 //   3. lanes = (sub-entry, node), 4 sub-entries per pass: the node prefilter.
 // The batch reserves its exact candidate count with one atomic and writes each pair's candidates
-// to a contiguous range, in (cluster, sub-cluster, node) order.
+// to a contiguous range, in (cluster, sub-cluster, node) order.  Every node of a pair is ruled
+// out by exactly one cull (cluster, sub-cluster or node) or becomes a candidate; the smallest
+// slack of those culls, less the rounding of the map, is the pair's separation certificate
+// (none when a node is a candidate).
 template <bool SUR>
 __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   extern __shared__ __align__(16) unsigned char candLds[];
@@ -1010,23 +1167,25 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   unsigned int* sMask = reinterpret_cast<unsigned int*>(sSub + subCap);  // 16-bit node masks
   int* sCnt = reinterpret_cast<int*>(sMask + subCap);
   int* sOff = sCnt + kBatch;
-  int* sBase4 = sOff + kBatch;  // write offsets of the 4 sub-entries of a pass
+  unsigned int* sSlack = reinterpret_cast<unsigned int*>(sOff + kBatch);  // bits of a float >= 0
+  int* sBase4 = reinterpret_cast<int*>(sSlack + kBatch);  // write offsets of the 4 sub-entries of a pass
   const int q = lane >> 4, b16 = lane & 15;
   const unsigned int lt16 = (1u << b16) - 1u;
+  const float kInf = __builtin_huge_valf();
   unsigned long long statCand = 0, statEnt = 0;
-  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
+  const int nAct = P.devInt[9];
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
   const int iters = (maxC + 7) >> 3;
   const int j = lane >> 3, k = lane & 7;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
+  for (int base = wave * kBatch; base < nAct; base += nWaves * kBatch) {
     // ---- level 1: clusters ----
-    const int p = base + j;
-    const bool own = p < nPairs;
+    const int a = base + j;  // map record
+    const bool own = a < nAct;
     // the 8 lanes of a pair copy its 160-byte map record into LDS, 20 bytes each
     if (own) {
-      const unsigned int* src = reinterpret_cast<const unsigned int*>(P.pairMap) + (long long)p * 40 + 5 * k;
+      const unsigned int* src = reinterpret_cast<const unsigned int*>(P.pairMap) + (long long)a * 40 + 5 * k;
       unsigned int* dst = reinterpret_cast<unsigned int*>(&sF[j]) + 5 * k;
 #pragma unroll
       for (int w = 0; w < 5; ++w) dst[w] = src[w];
@@ -1036,23 +1195,28 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     const PairF32& f = sF[j];
     const int clusterOff = own ? f.clusterOff : 0, nCl = own ? f.nClusters : 0;
     int E = 0;
+    float lmin = kInf;  // smallest slack of this lane's culled clusters
     for (int it = 0; it < iters; ++it) {
       const int c = it * 8 + k;
       bool keep = false;
       if (own && c < nCl) {
         const ClusterF32 cl = P.clusters32[clusterOff + c];
-        float gx, gy, gz, rho;
+        float gx, gy, gz, rho, slack;
         bool inside;
-        keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
+        keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside, slack);
         if (keep && inside) {
-          if (SUR && f.slot >= 0) keep = (P.surMask[p] >> c) & 1ull;
-          else if (P.sdfCull) keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho);
+          if (SUR && f.slot >= 0) keep = (P.surMask[a] >> c) & 1ull;  // slack: surSlack
+          else if (P.sdfCull) keep = clusterSdfF32(P.sdf, f, gx, gy, gz, rho, slack);
+          if (!keep && SUR && f.slot >= 0) slack = kInf;
         }
+        if (!keep) lmin = fminf(lmin, fmaxf(slack, 0.f));
       }
       const unsigned long long m = __ballot(keep);
       if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(clusterOff + c, j);
       E += __popcll(m);
     }
+    lmin = groupMin<8>(lmin);
+    if (k == 0) sSlack[j] = __float_as_uint(lmin);
     statEnt += (lane == 0) ? (unsigned long long)E : 0ull;
     __builtin_amdgcn_wave_barrier();
     // ---- level 2: sub-clusters, 16 entries x 4 per pass ----
@@ -1060,6 +1224,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     for (int e0 = 0; e0 < E; e0 += 16) {
       const int e = e0 + (lane >> 2), u = lane & 3;
       bool keep = false;
+      float smin = kInf;
       int first = 0, count = 0, jj = 0;
       if (e < E) {
         const int2 en = sEnt[e];
@@ -1068,16 +1233,20 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         if (u < cl.nSub) {
           const ClusterF32 sc = P.subs32[cl.subFirst + u];
           const PairF32& fj = sF[jj];
-          float gx, gy, gz, rho;
+          float gx, gy, gz, rho, slack;
           bool inside;
-          keep = clusterGeomF32(fj, sc, gx, gy, gz, rho, inside);
+          keep = clusterGeomF32(fj, sc, gx, gy, gz, rho, inside, slack);
           // with the surrogate the network has decided on the whole cluster; the sub-cluster keeps
           // only its geometric tests then
-          if (keep && inside && !(SUR && fj.slot >= 0) && P.sdfCull) keep = clusterSdfF32(P.sdf, fj, gx, gy, gz, rho);
+          if (keep && inside && !(SUR && fj.slot >= 0) && P.sdfCull)
+            keep = clusterSdfF32(P.sdf, fj, gx, gy, gz, rho, slack);
+          if (!keep) smin = fmaxf(slack, 0.f);
           first = fj.nodeOff + sc.first;
           count = sc.count;
         }
       }
+      smin = groupMin<4>(smin);  // the 4 lanes of an entry share its pair
+      if (e < E && (lane & 3) == 0 && smin < kInf) atomicMin(&sSlack[jj], __float_as_uint(smin));
       const unsigned long long m = __ballot(keep);
       if (keep) sSub[NS + __popcll(m & ltMask)] = make_int2(first, count | (jj << 16));
       NS += __popcll(m);
@@ -1088,17 +1257,24 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     for (int e0 = 0; e0 < NS; e0 += 4) {
       const int e = e0 + q;
       bool cand = false;
+      float smin = kInf;
       int jj = 0;
       if (e < NS) {
         const int2 se = sSub[e];
         jj = se.y >> 16;
-        if (b16 < (se.y & 0xFFFF)) cand = nodeF32(P.sdf, sF[jj], P.nodes32[se.x + b16]);
+        if (b16 < (se.y & 0xFFFF)) {
+          float slack;
+          cand = nodeF32(P.sdf, sF[jj], P.nodes32[se.x + b16], slack);
+          smin = cand ? 0.f : fmaxf(slack, 0.f);
+        }
       }
+      smin = groupMin<16>(smin);  // the 16 lanes of a sub-entry share its pair
       const unsigned long long m = __ballot(cand);
       const unsigned int mq = unsigned((m >> (16 * q)) & 0xFFFFull);
       if (b16 == 0 && e < NS) {
         sMask[e] = mq;
         atomicAdd(&sCnt[jj], __popc(mq));  // LDS: 4 groups may share a pair
+        if (smin < kInf) atomicMin(&sSlack[jj], __float_as_uint(smin));
       }
       total += __popcll(m);
     }
@@ -1113,12 +1289,20 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         total = 0;
       }
     }
-    if (lane < kBatch && base + lane < nPairs) {
+    if (lane < kBatch && base + lane < nAct) {
+      const PairF32& fl = sF[lane];
       int off = first;
       for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
-      P.pairCoff[base + lane] = off;
-      P.pairCcnt[base + lane] = total ? sCnt[lane] : 0;
+      P.pairCoff[fl.pair] = off;
+      P.pairCcnt[fl.pair] = total ? sCnt[lane] : 0;
       sOff[lane] = off;  // running write offset of each pair
+      // separation certificate in world units, net of the motion since the build
+      float sg = __uint_as_float(sSlack[lane]);
+      if (SUR) sg = fminf(sg, P.surSlack[base + lane]);
+      float sw = -1.f;
+      const float sgr = sg * (1.f - 1e-4f) + 2.f * fl.lo;  // lo = -(map rounding bound)
+      if (sgr > 0.f && P.useCert) sw = (sgr * fl.gToWorld - fl.certD0) * (1.f - 1e-5f);
+      P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, fl.certCA, 0.f);
     }
     __builtin_amdgcn_wave_barrier();
     // ---- write candidate slots; a pair's candidates are contiguous ----
@@ -1138,7 +1322,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
           if ((mq >> b16) & 1u) {
             const int2 se = sSub[e];
             const int dst = sBase4[q] + __popc(mq & lt16);
-            P.cPair[dst] = base + (se.y >> 16);
+            P.cPair[dst] = sF[se.y >> 16].pair;
             P.cNode[dst] = se.x + b16;  // cluster-ordered node index until the exact test
           }
         }
@@ -1163,17 +1347,18 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
 __global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
   __shared__ int sQ[kNarrowBlock / 64][64];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
-  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
+  const int nAct = P.devInt[9];
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
   const int iters = (P.maxClusters + 7) >> 3;
   const int j = lane >> 3, k = lane & 7;
-  for (int base = wave * kBatch; base < nPairs; base += nWaves * kBatch) {
-    const int p = base + j;
-    const bool own = p < nPairs;
+  for (int base = wave * kBatch; base < nAct; base += nWaves * kBatch) {
+    const int a = base + j;  // map record
+    const bool own = a < nAct;
     PairF32 f;
-    if (own) f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
+    if (own) f = reinterpret_cast<const PairF32*>(P.pairMap)[a];
     unsigned long long mask = ~0ull;
+    float smin = __builtin_huge_valf();  // smallest slack of the clusters the network culls
     for (int it = 0; it < iters; ++it) {
       const int c = it * 8 + k;
       float gx = 0.f, gy = 0.f, gz = 0.f, rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
@@ -1181,19 +1366,25 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
       if (own && c < f.nClusters && f.slot >= 0) {
         const ClusterF32 cl = P.clusters32[f.clusterOff + c];
         bool inside;
-        if (clusterGeomF32(f, cl, gx, gy, gz, rho, inside) && inside) {
+        float slack;
+        if (clusterGeomF32(f, cl, gx, gy, gz, rho, inside, slack) && inside) {
           qslot = f.slot;
           surQueryPoint(P, f, gx, gy, gz, ux, uy, uz);
         }
       }
       const float d = surInferPacked(P.sur, ux, uy, uz, qslot, lane, sQ[wl]);
       const bool cull = qslot >= 0 && !clusterSurKeep(P, f, d, rho);
+      if (cull) smin = fminf(smin, fmaxf(clusterSurSlack(P, f, d, rho), 0.f));
       // collect this group's 8 decisions of pair j into its mask
       const unsigned long long b = __ballot(cull);
       const unsigned long long mine = (b >> (8 * j)) & 0xFFull;
       mask &= ~(mine << (8 * it));
     }
-    if (own && k == 0) P.surMask[p] = mask;
+    smin = groupMin<8>(smin);
+    if (own && k == 0) {
+      P.surMask[a] = mask;
+      P.surSlack[a] = smin;
+    }
   }
 }
 
@@ -1213,14 +1404,15 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
     float gx = 0.f, gy = 0.f, gz = 0.f, rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
     if (c < n && P.cInfo[c] != kInvalidInfo) {
       const int p = P.cPair[c];
-      f = reinterpret_cast<const PairF32*>(P.pairMap)[p];
+      f = reinterpret_cast<const PairF32*>(P.pairMap)[P.pairRec[p]];
       if (f.slot >= 0) {
         const int A = P.pairA[p];
         const int orig = P.cNode[c] - P.nodeStart[A];
         const int cluster = P.nodeCluster[f.nodeOff + orig];
         const ClusterF32 cl = P.clusters32[f.clusterOff + cluster];
         bool inside;
-        const bool geom = clusterGeomF32(f, cl, gx, gy, gz, rho, inside);
+        float slack;
+        const bool geom = clusterGeomF32(f, cl, gx, gy, gz, rho, inside, slack);
         q = geom && inside;  // the only clusters the surrogate decides on
         if (q) surQueryPoint(P, f, gx, gy, gz, ux, uy, uz);
         checked++;
@@ -1421,6 +1613,7 @@ __global__ void k_step_begin(Params P, int forceRebuild) {
     P.devInt[8] = forceRebuild;  // rebuild flag of this step (the drift kernels may raise it)
     P.devInt[1] = 0;  // contact slots allocated
     P.devInt[7] = 0;  // valid ghost images at their source's own position
+    P.devInt[9] = 0;  // pairs the narrow phase evaluates
   }
   for (int k = threadIdx.x; k < 5 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
 }
@@ -1439,6 +1632,8 @@ __global__ void k_step_end(Params P) {
       mn += P.statSlots[4 * kStatSlots + k];
     }
     P.devStat[12] += mn;
+    P.devStat[13] += (unsigned long long)P.devInt[9];
+    P.devStat[14] += P.devInt[8] != 0 ? 1ull : 0ull;
     P.devStat[0] = t;
     P.devStat[1] = q;
     P.devStat[2] = c;

@@ -40,7 +40,8 @@ class Config(C.Structure):
         ("device", C.c_int32),
         ("no_sdf_cull", C.c_int32),
         ("skin", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("no_certificates", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -92,7 +93,9 @@ class Profile(C.Structure):
         ("contacts", C.c_int64),
         ("cluster_tests", C.c_int64),
         ("master_nodes", C.c_int64),
-        ("reserved", C.c_int64 * 6),
+        ("evaluated_pairs", C.c_int64),
+        ("rebuilds", C.c_int64),
+        ("reserved", C.c_int64 * 4),
     ]
 
 
@@ -165,6 +168,7 @@ class Sim:
         c.contact_capacity = int(cfg.get("contact_capacity", 0))
         c.device = int(cfg.get("device", 0))
         c.no_sdf_cull = int(cfg.get("no_sdf_cull", 0))
+        c.no_certificates = int(cfg.get("no_certificates", 0))
         c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()
@@ -372,7 +376,7 @@ class Sim:
         self._call("profile_end", self.ctx, C.byref(p))
         return {"steps": p.steps, "ms": dict(zip(PHASES, list(p.ms))), "pairs": p.pairs, "node_tests": p.node_tests,
                 "sdf_queries": p.sdf_queries, "contacts": p.contacts, "cluster_tests": p.cluster_tests,
-                "master_nodes": p.master_nodes}
+                "master_nodes": p.master_nodes, "evaluated_pairs": p.evaluated_pairs, "rebuilds": p.rebuilds}
 
     def counters(self) -> dict:
         c = Counters()

@@ -49,7 +49,8 @@ def compare_bonds(a: dict, b: dict, name: str, tol: float = 1e-9):
             assert e <= tol, f"{name}: bond {k} rel err {e}"
 
 
-def step_and_compare(gpu, ref, steps: int, lists=(0, 1), ftol: float = 1e-5, log=None, bonds: bool = False):
+def step_and_compare(gpu, ref, steps: int, lists=(0, 1), ftol: float = 1e-5, log=None, bonds: bool = False,
+                     ctol: float = 1e-12):
     worst = {}
     counts = []
     for s in range(steps):
@@ -62,7 +63,7 @@ def step_and_compare(gpu, ref, steps: int, lists=(0, 1), ftol: float = 1e-5, log
             row["broken"] = int((bb["active"] == 0).sum())
         for l in lists:
             ca, cb = gpu.contacts(l), ref.contacts(l)
-            compare_contacts(ca, cb, f"step {s} list {LISTS[l]}")
+            compare_contacts(ca, cb, f"step {s} list {LISTS[l]}", ctol)
             row[LISTS[l]] = len(ca["master"])
         counts.append(row)
         e = compare_state(gpu.get_particles(), ref.get_particles(), ftol)

@@ -80,31 +80,44 @@ def test_gradient_matches_oracle(L, S):
 
 
 def test_training_and_exact_culling(L, S):
+    # cull power is compared pair for pair, so these three evaluate every pair every step; a fourth
+    # runs the surrogate with separation certificates (its slacks come from the network)
     sc = small_scene(S)
-    ref = sc.load(L.Sim(sc.cfg))
-    sur = sc.load(L.Sim(sc.cfg))
-    none = sc.load(L.Sim({**sc.cfg, "no_sdf_cull": 1}))
-    rep = sur.surrogate_train(iterations=800, batch=2048, samples=300_000, validation=65_536, audit_every=10)
+    full = {**sc.cfg, "no_certificates": 1}
+    ref = sc.load(L.Sim(full))
+    sur = sc.load(L.Sim(full))
+    none = sc.load(L.Sim({**full, "no_sdf_cull": 1}))
+    surc = sc.load(L.Sim(sc.cfg))
+    train = dict(iterations=800, batch=2048, samples=300_000, validation=65_536, audit_every=10)
+    rep = sur.surrogate_train(**train)
     print("trained", rep)
     assert rep["trained"] and np.isfinite(rep["max_error"])
     assert rep["max_error"] < 0.25 * rep["length_scale"]
     assert rep["final_loss"] < 0.01
     sur.surrogate_enable(True)
+    surc.surrogate_train(**train)
+    surc.surrogate_enable(True)
     work = {}
-    for name, sim in (("exact", ref), ("surrogate", sur), ("spheres", none)):
+    for name, sim in (("exact", ref), ("surrogate", sur), ("spheres", none), ("certificates", surc)):
         sim.profile_begin(40)
     for s in range(40):
-        for sim in (ref, sur, none):
+        for sim in (ref, sur, none, surc):
             sim.step(1)
         for l in (0, 1, 2, 3, 4):
             ca = sur.contacts(l)
             compare_contacts(ca, ref.contacts(l), f"step {s} list {l} (surrogate)")
             compare_contacts(none.contacts(l), ref.contacts(l), f"step {s} list {l} (spheres only)")
+            compare_contacts(surc.contacts(l), ref.contacts(l), f"step {s} list {l} (surrogate, certificates)")
         compare_state(sur.get_particles(), ref.get_particles(), ftol=1e-12)
-    for name, sim in (("exact", ref), ("surrogate", sur), ("spheres", none)):
-        work[name] = sim.profile_end()["cluster_tests"]
+        compare_state(surc.get_particles(), ref.get_particles(), ftol=1e-12)
+    profs = {}
+    for name, sim in (("exact", ref), ("surrogate", sur), ("spheres", none), ("certificates", surc)):
+        profs[name] = sim.profile_end()
+        work[name] = profs[name]["cluster_tests"]
     r = sur.surrogate_report()
     print("cluster entries", work, "audits", r["audits"], r["audited_contacts"], r["missed_contacts"])
+    print("certificates: evaluated", profs["certificates"]["evaluated_pairs"], "of", profs["certificates"]["pairs"])
     assert r["audits"] >= 4 and r["audited_contacts"] > 0
     assert r["missed_contacts"] == 0
+    assert surc.surrogate_report()["missed_contacts"] == 0
     assert work["surrogate"] < work["spheres"]
