@@ -19,8 +19,7 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
 __global__ void k_scan_reduce(const int* in, long long n, int* blockSum, const int* gate);
 __global__ void k_scan_top(int* blockSum, int nb, int* totalOut, const int* gate);
 __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out, const int* gate);
-__global__ void k_kick_drift(Params P);
-__global__ void k_walls(Params P);
+__global__ void k_kick_drift(Params P, int forceRebuild);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
                                  int* bigCount, const int* gate);
@@ -33,11 +32,8 @@ __global__ void k_pairs_find(Params P);
 __global__ void k_pairs_emit(Params P);
 __global__ void k_rc_count(Params P);
 __global__ void k_rc_scatter(Params P);
-__global__ void k_hash_insert(Params P);
-__global__ void k_hash_clear(Params P);
 __global__ void k_gather_particles(Params P);
 __global__ void k_pair_check(Params P, int full);
-__global__ void k_pair_maps(Params P);
 template <bool SUR> __global__ void k_candidates(Params P);
 __global__ void k_sur_audit(Params P);
 __global__ void k_sur_masks(Params P);
@@ -46,8 +42,6 @@ __global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P, int kick);
 __global__ void k_kick(Params P);
-__global__ void k_step_begin(Params P, int forceRebuild);
-__global__ void k_step_end(Params P);
 }  // namespace lsd
 
 using namespace lsd;
@@ -239,7 +233,7 @@ struct lsdem_ctx {
   DBuf<ClusterF32> dClusters32, dSubs32;
   DBuf<d3> dXBuild, dXBuildW, dWBuildPos;
   DBuf<q4> dQBuild;
-  DBuf<int> dImgBuild, dImgFlip, dActive, dPairRec;
+  DBuf<int> dImgBuild, dImgFlip, dPairRec;
   DBuf<float4> dMotion, dWMotion, dCert;
   DBuf<float> dSurSlack;
   DBuf<q4> dWBuildQ;
@@ -290,8 +284,8 @@ struct lsdem_ctx {
   DBuf<uint32_t> dCInfo[2];
   DBuf<d3> dCSpring[2], dCPoint[2], dCNormal[2];
   DBuf<double> dCOverlap[2], dCEn[2], dCEs[2];
-  DBuf<unsigned long long> dHashKey;
-  DBuf<int> dHashVal, dHashSlot;
+  DBuf<unsigned long long> dHashKey[2];
+  DBuf<int> dHashVal[2], dHashSlot[2];
   DBuf<unsigned long long> dStatSlots;
   DBuf<int> dBI, dBJ, dBActive, dBcStart, dBcItem;
   DBuf<double> dBConst, dBOut;
@@ -329,7 +323,7 @@ void lsdem_ctx::releaseAll() {
     dGPos[s].release(); dGKey[s].release();
   }
   dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
-  dSPR.release(); dSPRw.release(); dPairSlab.release(); dHashSlot.release(); dStatSlots.release();
+  dSPR.release(); dSPRw.release(); dPairSlab.release(); dStatSlots.release();
   dSBoxR.release();
   dPairCount.release(); dPairStart.release(); dPairA.release(); dPairCoff.release(); dPairCcnt.release();
   dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release(); dRcTmp.release();
@@ -339,14 +333,16 @@ void lsdem_ctx::releaseAll() {
     dCNode[b].release(); dCInfo[b].release(); dCSpring[b].release(); dCPoint[b].release();
     dCNormal[b].release(); dCOverlap[b].release(); dCEn[b].release(); dCEs[b].release();
   }
-  dHashKey.release(); dHashVal.release();
+  for (int b = 0; b < 2; ++b) {
+    dHashKey[b].release(); dHashVal[b].release(); dHashSlot[b].release();
+  }
   dBI.release(); dBJ.release(); dBActive.release(); dBcStart.release(); dBcItem.release();
   dBConst.release(); dBOut.release(); dBLocal.release(); dBPoint.release(); dBF.release(); dBMi.release();
   dBMj.release();
   dDevInt.release(); dScanBlocks.release(); dDevStat.release();
   dNodeCluster.release();
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
-  dQBuild.release(); dImgBuild.release(); dImgFlip.release(); dActive.release(); dPairRec.release();
+  dQBuild.release(); dImgBuild.release(); dImgFlip.release(); dPairRec.release();
   dMotion.release(); dWMotion.release(); dCert.release(); dSurSlack.release();
   sur.release();
 }
@@ -424,6 +420,9 @@ void setCurrentBuffers(lsdem_ctx* c) {
   P.cPoint = c->dCPoint[b].p; P.cNormal = c->dCNormal[b].p; P.cOverlap = c->dCOverlap[b].p;
   P.cEn = c->dCEn[b].p; P.cEs = c->dCEs[b].p;
   P.prevSpring = c->dCSpring[o].p;
+  P.hashKey = c->dHashKey[b].p; P.hashVal = c->dHashVal[b].p; P.hashSlot = c->dHashSlot[b].p;
+  P.prevHashKey = c->dHashKey[o].p; P.prevHashVal = c->dHashVal[o].p;
+  P.curBuf = b;
 }
 
 constexpr int kPhaseEvents = LSDEM_NUM_PHASES + 1;
@@ -455,11 +454,10 @@ int runStep(lsdem_ctx* c) {
     ++mark;
   };
   phase();
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, P, c->forceRebuild ? 1 : 0);
+  const int force = c->forceRebuild ? 1 : 0;
   c->forceRebuild = false;
   const int* gate = P.devInt + 8;  // broad phase only on rebuild steps
-  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
-  if (P.nW > 0) hipLaunchKernelGGL(k_walls, dim3(blocksFor(P.nW, bs)), dim3(bs), 0, s, P);
+  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, force);
   phase();
   int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted, gate);
   if (rc) return rc;
@@ -486,8 +484,7 @@ int runStep(lsdem_ctx* c) {
   // narrow phase + contact law (an audit step evaluates every live pair)
   const bool useSur = c->sur.enabled && c->sur.trained;
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
-  hipLaunchKernelGGL(k_pair_check, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P, audit ? 1 : 0);
-  hipLaunchKernelGGL(k_pair_maps, dim3(unsigned(c->numCU * 8)), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_pair_check, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P, audit ? 1 : 0);
   if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
   if (useSur && !audit)
     hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
@@ -502,9 +499,8 @@ int runStep(lsdem_ctx* c) {
     c->sur.audits++;
   }
   phase();
-  // history of this step's contacts for the next step
-  hipLaunchKernelGGL(k_hash_clear, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
-  hipLaunchKernelGGL(k_hash_insert, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
+  // (history: this step's contacts went into this step's table inside k_exact_contacts; the phase
+  // stays in the profile layout)
   phase();
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
   phase();
@@ -517,7 +513,6 @@ int runStep(lsdem_ctx* c) {
   } else {
     hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 1);
   }
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, P);
   phase();
   if (prof) c->profSteps++;
   c->cur = 1 - c->cur;
@@ -918,7 +913,6 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dSurMask.alloc(pairCap));
   HIPCHK(c->dSurSlack.alloc(pairCap));
   HIPCHK(c->dCert.alloc(pairCap));
-  HIPCHK(c->dActive.alloc(pairCap));
   HIPCHK(c->dPairRec.alloc(pairCap));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
@@ -942,9 +936,11 @@ int lsdem_initialize(lsdem_ctx* c) {
   long long hs = 1024;
   while (hs < 2 * contactCap) hs <<= 1;
   c->hashSize = hs;
-  HIPCHK(c->dHashKey.alloc(hs));
-  HIPCHK(c->dHashVal.alloc(hs));
-  HIPCHK(c->dHashSlot.alloc(contactCap));
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(c->dHashKey[b].alloc(hs));
+    HIPCHK(c->dHashVal[b].alloc(hs));
+    HIPCHK(c->dHashSlot[b].alloc(contactCap));
+  }
   HIPCHK(c->dStatSlots.alloc(5 * kStatSlots));
   P.hashMask = (unsigned long long)(hs - 1);
 
@@ -985,7 +981,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
-  hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, c->dHashKey.p, hs, kHashEmpty);
+  for (int b = 0; b < 2; ++b)
+    hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, c->dHashKey[b].p, hs, kHashEmpty);
 
   // ---- pointer table ----
   P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
@@ -999,7 +996,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.nodeStart = c->dNodeStart.p; P.keyRef = c->dKeyRef.p; P.scell = c->dScell.p;
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.imgFlip = c->dImgFlip.p; P.motion = c->dMotion.p;
-  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.active = c->dActive.p; P.pairRec = c->dPairRec.p;
+  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
   P.surSlack = c->dSurSlack.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
@@ -1009,12 +1006,11 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
   P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
   P.sBoxR = c->dSBoxR.p;
-  P.hashSlot = c->dHashSlot.p; P.statSlots = c->dStatSlots.p;
+  P.statSlots = c->dStatSlots.p;
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
   P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p; P.surMask = c->dSurMask.p;
   P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
   P.rcTmp = c->dRcTmp.p;
-  P.hashKey = c->dHashKey.p; P.hashVal = c->dHashVal.p;
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
   P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
   P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
@@ -1211,7 +1207,8 @@ int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
   const size_t need = size_t(max_steps) * kPhaseEvents;
   while (c->events.size() < need) {
     hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
+    // timing only: no system-scope fence at each phase boundary (read after a stream sync)
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     c->events.push_back(e);
   }
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -143,7 +143,6 @@ struct Params {
   // separation certificates: a pair whose nodes were all culled with slack s (world units, less
   // the motion bound at that step) is skipped while the motion bound since the build stays below s
   float4* cert;     // pairCap: (s or -1, cB, cA, 0); bound = |dA - dB| + chordB cB + chordA cA
-  int* active;      // pairs the narrow phase evaluates this step (devInt[9] of them)
   int* pairRec;     // pair -> index of its map record this step
   int useCert;
   // reaction CSR (pairs whose partner is a particle that receives the reaction)
@@ -165,11 +164,15 @@ struct Params {
   double* cOverlap;
   double* cEn;
   double* cEs;
-  // previous step's springs, addressed through the hash table
+  // history: two hash tables keyed by (list, node, partner), used in alternate steps; this step
+  // inserts its contacts into one and looks up the previous step's springs in the other
   const d3* prevSpring;
-  unsigned long long* hashKey;
+  unsigned long long* hashKey;        // this step's table
   int* hashVal;
-  int* hashSlot;    // slot taken by each inserted contact (cleared at the next insert)
+  int* hashSlot;                      // slot taken by each contact slot (-1: none), per table
+  const unsigned long long* prevHashKey;  // the previous step's table
+  const int* prevHashVal;
+  int curBuf;                         // which of the two tables and contact buffers is this step's
 
   // ---- bonds ----
   const int* bI;
@@ -189,7 +192,8 @@ struct Params {
   // devInt: [1] contact slots allocated this step, [3] overflow flags (1 pairs, 2 contacts),
   //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert,
   //         [7] images at their source's own position, [8] rebuild flag of the step,
-  //         [9] pairs the narrow phase evaluates this step
+  //         [9] pairs the narrow phase evaluates this step, [10 + b] contact slots of the
+  //         last step that filled history table b
   int* devInt;
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries

@@ -2,7 +2,6 @@
 //
 // Per step (launch order, one stream):
 //   k_kick_drift      half-kick, drift, reference-grid key, search cell, periodic wrap + ghost images
-//   k_walls           wall drift + wall keys
 //   counting sort     particles (and ghost images) by search-grid cell, deterministic order
 //   k_pairs_find      FP32 bounding-sphere / grid-box candidate pairs per master particle: count,
 //                     first 16 kept in a per-master slab
@@ -228,9 +227,27 @@ __device__ __forceinline__ void halfKick(const Params& P, int i, d3 F, d3 T) {
   P.angvel[i] += (mul(rotInvInertia(P.quat[i], P.invI[i]), T)) * h;
 }
 
-__global__ void k_kick_drift(Params P) {
+__device__ void wallDrift(const Params& P, int w);
+
+// One thread per particle, then one per wall.  Also raises the step's rebuild flag when the host
+// asks for it, and empties the entries of the history table this step fills (left there two
+// steps ago; their slots were recorded at insertion).
+__global__ void k_kick_drift(Params P, int forceRebuild) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nP) return;
+  if (i == 0) {
+    if (forceRebuild) atomicOr(&P.devInt[8], 1);
+    P.devInt[1] = 0;  // contact slots allocated
+    P.devInt[9] = 0;  // pairs the narrow phase evaluates
+  }
+  const int nClear = P.devInt[10 + P.curBuf];
+  for (int c = i; c < nClear; c += gridDim.x * blockDim.x) {
+    const int slot = P.hashSlot[c];
+    if (slot >= 0) P.hashKey[slot] = kHashEmpty;
+  }
+  if (i >= P.nP) {
+    if (i - P.nP < P.nW) wallDrift(P, i - P.nP);
+    return;
+  }
   halfKick(P, i, P.force[i], P.torque[i]);
   d3 p = P.pos[i];
   p += P.dt * P.vel[i];
@@ -299,9 +316,7 @@ __global__ void k_kick_drift(Params P) {
   if (periodic) P.imgFlip[i] = imgShiftMask(P, p) ^ P.imgBuild[i];
 }
 
-__global__ void k_walls(Params P) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= P.nW) return;
+__device__ void wallDrift(const Params& P, int w) {
   d3 p = P.wpos[w];
   p += P.dt * P.wvel[w];
   P.wpos[w] = p;
@@ -681,42 +696,26 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
   x ^= x >> 31;
   return x;
 }
-// Remove the previous step's entries (their slots were recorded at insertion).
-__global__ void k_hash_clear(Params P) {
-  const int n = P.devInt[6];
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const int slot = P.hashSlot[c];
-    if (slot >= 0) P.hashKey[slot] = kHashEmpty;
-  }
-}
-__global__ void k_hash_insert(Params P) {
-  const int n = min(P.devInt[1], (int)P.contactCap);
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const uint32_t info = P.cInfo[c];
-    int slotOut = -1;
-    if (info != kInvalidInfo) {
-      const unsigned long long key = histKey(info >> 28, P.cNode[c], int(info & kPartnerMask));
-      unsigned long long slot = mix64(key) & P.hashMask;
-      for (;;) {
-        const unsigned long long old = atomicCAS(&P.hashKey[slot], kHashEmpty, key);
-        if (old == kHashEmpty || old == key) {
-          P.hashVal[slot] = c;
-          break;
-        }
-        slot = (slot + 1) & P.hashMask;
-      }
-      slotOut = int(slot);
+// Insert a contact of this step into this step's table (open addressing, linear probing); returns
+// the slot, recorded so that the entry can be removed when the table is reused two steps later.
+__device__ __forceinline__ int histInsert(const Params& P, unsigned long long key, int c) {
+  unsigned long long slot = mix64(key) & P.hashMask;
+  for (;;) {
+    const unsigned long long old = atomicCAS(&P.hashKey[slot], kHashEmpty, key);
+    if (old == kHashEmpty || old == key) {
+      P.hashVal[slot] = c;
+      return int(slot);
     }
-    P.hashSlot[c] = slotOut;  // -1 for unused contact slots
+    slot = (slot + 1) & P.hashMask;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[6] = n;
 }
 
+// The previous step's spring of a contact key (zero when it had none), from the previous table.
 __device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key) {
   unsigned long long slot = mix64(key) & P.hashMask;
   for (;;) {
-    const unsigned long long k = P.hashKey[slot];
-    if (k == key) return P.prevSpring[P.hashVal[slot]];
+    const unsigned long long k = P.prevHashKey[slot];
+    if (k == key) return P.prevSpring[P.prevHashVal[slot]];
     if (k == kHashEmpty) return mk3(0., 0., 0.);
     slot = (slot + 1) & P.hashMask;
   }
@@ -922,73 +921,80 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
 // image) or when its separation certificate still holds: the motion bound of its nodes since the
 // build stays below the slack recorded at its last evaluation (all its nodes were culled then
 // with that much room).  Skipped pairs have no contacts.  full != 0 evaluates every live pair.
-__global__ void k_pair_check(Params P, int full) {
+// A block compacts a tile of 1024 pairs in LDS, appends it with one atomic, then writes the FP32
+// map records of its evaluated pairs (FP64 poses in, one 160-byte record out, in evaluation
+// order).
+constexpr int kCheckTile = 1024;
+__global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
+  __shared__ int sList[kCheckTile];
+  PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
+  __shared__ int sCount, sBase;
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
   const bool rebuilt = P.devInt[8] != 0;  // a new pair list: no certificate exists yet
   const bool fresh = full || rebuilt || !P.useCert;
   const int lane = threadIdx.x & 63;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
-  const int stride = gridDim.x * blockDim.x;
-  for (int p0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); p0 < nPairs; p0 += stride) {
-    const int p = p0 + lane;
-    bool act = false;
-    if (p < nPairs) {
-      const uint32_t info = P.pairInfo[p];
-      const uint32_t type = info >> 28;
-      const int j = int(info & kPartnerMask);
-      const int A = P.pairA[p];
-      const bool live = type == PT_PP ? P.keyRef[j] >= 0
-                                      : (type == PT_PW ? P.wkey[j] >= 0 : P.gkey[type - PT_GX][j] >= 0);
-      if (live) {
-        masterNodes += (unsigned long long)P.shapes[P.shape[A]].nNodes;
-        act = true;
-        if (!fresh) {
-          const float4 c = P.cert[p];
-          if (c.x > 0.f) {
-            const float4 mA = P.motion[A];
-            float4 mB;
-            bool flip = false;
-            if (type == PT_PW) mB = P.wMotion[j];
-            else {
-              mB = P.motion[j];
-              if (type >= PT_GX) flip = (P.imgFlip[j] >> (type - PT_GX)) & 1;
-            }
-            if (!flip) {
-              const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
-              const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * c.z) * (1.f + 1e-5f);
-              act = !(bound < c.x);
+  for (int t0 = blockIdx.x * kCheckTile; t0 < nPairs; t0 += gridDim.x * kCheckTile) {
+    if (threadIdx.x == 0) sCount = 0;
+    __syncthreads();
+    for (int r = 0; r < kCheckTile / 256; ++r) {
+      const int p = t0 + r * 256 + int(threadIdx.x);
+      bool act = false;
+      if (p < nPairs) {
+        const uint32_t info = P.pairInfo[p];
+        const uint32_t type = info >> 28;
+        const int j = int(info & kPartnerMask);
+        const int A = P.pairA[p];
+        const bool live = type == PT_PP ? P.keyRef[j] >= 0
+                                        : (type == PT_PW ? P.wkey[j] >= 0 : P.gkey[type - PT_GX][j] >= 0);
+        if (live) {
+          masterNodes += (unsigned long long)(P.nodeStart[A + 1] - P.nodeStart[A]);
+          act = true;
+          if (!fresh) {
+            const float4 c = P.cert[p];
+            if (c.x > 0.f) {
+              const float4 mA = P.motion[A];
+              float4 mB;
+              bool flip = false;
+              if (type == PT_PW) mB = P.wMotion[j];
+              else {
+                mB = P.motion[j];
+                if (type >= PT_GX) flip = (P.imgFlip[j] >> (type - PT_GX)) & 1;
+              }
+              if (!flip) {
+                const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
+                const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * c.z) * (1.f + 1e-5f);
+                act = !(bound < c.x);
+              }
             }
           }
+        } else if (rebuilt) {
+          P.cert[p] = make_float4(-1.f, 0.f, 0.f, 0.f);
         }
-      } else if (rebuilt) {
-        P.cert[p] = make_float4(-1.f, 0.f, 0.f, 0.f);
+        if (!act) P.pairCcnt[p] = 0;
       }
-      if (!act) P.pairCcnt[p] = 0;
+      const unsigned long long m = __ballot(act);
+      int first = 0;
+      if (lane == 0 && m) first = atomicAdd(&sCount, __popcll(m));
+      first = __shfl(first, 0, 64);
+      if (act) sList[first + __popcll(m & ltMask)] = p;
     }
-    const unsigned long long m = __ballot(act);
-    int first = 0;
-    if (lane == 0 && m) first = atomicAdd(&P.devInt[9], __popcll(m));
-    first = __shfl(first, 0, 64);
-    if (act) P.active[first + __popcll(m & ltMask)] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) sBase = sCount ? atomicAdd(&P.devInt[9], sCount) : 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < sCount; k += 256) {
+      const int p = sList[k], a = sBase + k;
+      const PairView v = loadPair(P, p);
+      PairF32 f;
+      makeF32(P, v, f);
+      out[a] = f;
+      P.pairRec[p] = a;
+    }
+    __syncthreads();
   }
   const unsigned long long t = (unsigned long long)wsum(double(masterNodes));
   if (lane == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
-}
-
-// FP32 pair maps of the pairs evaluated this step, one thread per pair (FP64 poses in, one
-// 160-byte record out, in evaluation order).
-__global__ void k_pair_maps(Params P) {
-  const int nAct = P.devInt[9];
-  PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
-  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < nAct; a += gridDim.x * blockDim.x) {
-    const int p = P.active[a];
-    const PairView v = loadPair(P, p);
-    PairF32 f;
-    makeF32(P, v, f);
-    out[a] = f;
-    P.pairRec[p] = a;
-  }
 }
 
 __device__ __forceinline__ void mapF32(const PairF32& f, float nx, float ny, float nz, float& gx, float& gy, float& gz) {
@@ -1433,11 +1439,13 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
 // membership in FP64, then (for contacts) tangential history and the linear spring-dashpot.
 __global__ void k_exact_contacts(Params P) {
   const int n = min(P.devInt[1], (int)P.contactCap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[10 + P.curBuf] = n;  // slots to clear two steps on
   unsigned long long statQueries = 0, statContacts = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     const int p = P.cPair[c];
     if (p < 0) {
       P.cInfo[c] = kInvalidInfo;
+      P.hashSlot[c] = -1;
       continue;
     }
     const PairView v = loadPair(P, p);
@@ -1446,6 +1454,7 @@ __global__ void k_exact_contacts(Params P) {
     const bool hit = exactTest(P, v, P.cNode[c], true, h, tests, statQueries);
     if (!hit) {
       P.cInfo[c] = kInvalidInfo;
+      P.hashSlot[c] = -1;
       continue;
     }
     statContacts++;
@@ -1478,8 +1487,11 @@ __global__ void k_exact_contacts(Params P) {
       res = fmin(P.res[A], P.res[j]);
       mu = fmin(P.mu[A], P.mu[j]);
     }
-    // tangential history of the same (list, node, partner) from the previous step
-    d3 spring = histLookup(P, histKey(type, h.gnode, j));
+    // tangential history of the same (list, node, partner) from the previous step; this step's
+    // contact goes into this step's table for the next one
+    const unsigned long long key = histKey(type, h.gnode, j);
+    d3 spring = histLookup(P, key);
+    P.hashSlot[c] = histInsert(P, key, c);
     const d3 vc = vA + cross(wA, h.point - riA) - (vB + cross(wB, h.point - xf));
     d3 F = mk3(0., 0., 0.);
     linearContact(F, spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
@@ -1561,8 +1573,12 @@ __global__ void k_bonds(Params P) {
 // ------------------------------------------------------------------------------------------
 // Per-body segmented reduction + second half-kick
 // ------------------------------------------------------------------------------------------
+__device__ void stepEnd(const Params& P);
+
+// One thread per body.  Thread 0 also closes the step's bookkeeping.
 __global__ void k_reduce_kick(Params P, int kick) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) stepEnd(P);
   if (b >= P.nP) return;
   d3 F = mk3(0., 0., 0.), T = mk3(0., 0., 0.);
   const long long cap = P.pairCap;
@@ -1608,17 +1624,10 @@ __global__ void k_kick(Params P) {
 // ------------------------------------------------------------------------------------------
 // Step bookkeeping
 // ------------------------------------------------------------------------------------------
-__global__ void k_step_begin(Params P, int forceRebuild) {
-  if (threadIdx.x == 0) {
-    P.devInt[8] = forceRebuild;  // rebuild flag of this step (the drift kernels may raise it)
-    P.devInt[1] = 0;  // contact slots allocated
-    P.devInt[7] = 0;  // valid ghost images at their source's own position
-    P.devInt[9] = 0;  // pairs the narrow phase evaluates
-  }
-  for (int k = threadIdx.x; k < 5 * kStatSlots; k += blockDim.x) P.statSlots[k] = 0;
-}
-__global__ void k_step_end(Params P) {
-  if (threadIdx.x == 0) {
+// Step totals into the counters, then the per-step counters that the drift kernel accumulates into
+// cleared for the next step.
+__device__ void stepEnd(const Params& P) {
+  {
     const int np = P.pairStart[P.nP];
     if ((long long)np > P.pairCap) P.devInt[3] |= 1;
     P.devInt[4] = max(P.devInt[4], np);
@@ -1644,6 +1653,11 @@ __global__ void k_step_end(Params P) {
     P.devStat[6] += c;
     P.devStat[8] += e;
   }
+  P.devInt[8] = 0;  // rebuild flag (the drift kernel raises it)
+  P.devInt[7] = 0;  // valid ghost images at their source's own position
+  for (int k = 0; k < 5 * kStatSlots; ++k) P.statSlots[k] = 0;
+  // (the contact slot count [1] and the evaluated-pair count [9] stay readable after the step;
+  // the drift kernel clears them)
 }
 
 
