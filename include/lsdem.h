@@ -63,7 +63,14 @@ typedef struct lsdem_config {
                              radius; 0: rebuilt every step).  Contacts do not depend on it. */
   int32_t no_certificates; /* 1: evaluate every candidate pair every step (no separation
                               certificates); contacts do not depend on it */
-  int32_t reserved[3];
+  int32_t broad_phase_launches; /* 1: run the broad phase as separate launches gated on the
+                              device instead of one launch with grid barriers; results do not
+                              depend on it */
+  int32_t broad_phase_blocks; /* grid of the single-launch broad phase (0: automatic); capped at
+                              what can be resident together */
+  int32_t narrow_batch;   /* candidate pairs per wavefront batch of the narrow phase: 1, 2, 4 or 8
+                             (0: chosen each step from the number of pairs it evaluates) */
+  int32_t reserved[4];
 } lsdem_config;
 
 typedef struct lsdem_counters {
@@ -159,7 +166,9 @@ int lsdem_get_counters(lsdem_ctx* ctx, lsdem_counters* out);
 /* Per-phase timing of lsdem_step with HIP events recorded on the context's stream.  Phases:
  * 0 integrate (half-kick + drift + walls), 1 broad-phase sort, 2 candidate pairs,
  * 3 reaction lists, 4 narrow phase + contact law, 5 history table, 6 bonds,
- * 7 reduction + second half-kick.  Work counters are summed over the profiled steps.
+ * 7 reduction + second half-kick.  With the single-launch broad phase (the default) phases 1-3
+ * are one launch, reported under 1; the history table is filled inside phase 4.  Work counters
+ * are summed over the profiled steps.
  * No counterpart in the reference (it has no timers). */
 #define LSDEM_NUM_PHASES 8
 typedef struct lsdem_profile {

@@ -17,9 +17,9 @@ namespace lsd {
 __global__ void k_fill_int(int* a, long long n, int v, const int* gate);
 __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v);
 __global__ void k_scan_reduce(const int* in, long long n, int* blockSum, const int* gate);
-__global__ void k_scan_top(int* blockSum, int nb, int* totalOut, const int* gate);
 __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out, const int* gate);
 __global__ void k_kick_drift(Params P, int forceRebuild);
+__global__ void k_broad_phase(Params P);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
                                  int* bigCount, const int* gate);
@@ -208,6 +208,7 @@ struct lsdem_ctx {
   bool initialized = false;
   int numCU = 256;
   int narrowBlocks = 2048;
+  int coopBlocks = 0;  // grid of the single-launch broad phase (0: separate launches)
   // host staging
   std::vector<HostShape> shapes;
   std::vector<int> pShape;
@@ -299,6 +300,7 @@ struct lsdem_ctx {
   int cur = 0;  // contact buffer written by the next step
   // profiling: kPhaseEvents events per profiled step
   std::vector<hipEvent_t> events;
+  std::vector<int> eventOf;  // per profiled step and phase mark: the event slot recorded for it
   int profMax = 0, profSteps = 0;
   bool profiling = false;
 
@@ -380,14 +382,9 @@ inline unsigned blocksFor(long long n, int bs) { return unsigned(std::max<long l
 
 // Exclusive scan (out[n] = total).  `gate`: run only when *gate != 0 (nullptr: always).
 int scan(lsdem_ctx* c, const int* in, long long n, int* out, const int* gate) {
-  if (n <= 0) {
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, out, 1ll, 0, gate);
-    return LSDEM_OK;
-  }
-  const int nb = int((n + 2047) / 2048);
+  const int nb = int(std::max<long long>(1, (n + 2047) / 2048));
   if (nb > c->scanBlocksCap) return fail(LSDEM_ERR_STATE, "scan buffer too small");
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->dScanBlocks.p, gate);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, c->stream, c->dScanBlocks.p, nb, out + n, gate);
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, in, n, (const int*)c->dScanBlocks.p, out, gate);
   return LSDEM_OK;
 }
@@ -441,24 +438,11 @@ void deviceView(lsdem_ctx* c, lsdem_device_view* v) {
   v->stream = c->stream;
 }
 
-int runStep(lsdem_ctx* c) {
+// The broad phase as separate launches, each gated on the device rebuild flag.
+template <class Phase>
+int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
   Params& P = c->P;
-  setCurrentBuffers(c);
-  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
   hipStream_t s = c->stream;
-  const bool prof = c->profiling && c->profSteps < c->profMax;
-  hipEvent_t* ev = prof ? &c->events[size_t(c->profSteps) * kPhaseEvents] : nullptr;
-  int mark = 0;
-  auto phase = [&]() {
-    if (prof) (void)hipEventRecord(ev[mark], s);
-    ++mark;
-  };
-  phase();
-  const int force = c->forceRebuild ? 1 : 0;
-  c->forceRebuild = false;
-  const int* gate = P.devInt + 8;  // broad phase only on rebuild steps
-  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, force);
-  phase();
   int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
@@ -480,7 +464,47 @@ int runStep(lsdem_ctx* c) {
                      (const int*)P.rcTmp, P.rcPair, c->dSortBig.p, P.devInt + 2, gate);
   hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, s, (const int*)P.rcStart,
                      (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2), gate);
+  return LSDEM_OK;
+}
+
+int runStep(lsdem_ctx* c) {
+  Params& P = c->P;
+  setCurrentBuffers(c);
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  hipStream_t s = c->stream;
+  const bool prof = c->profiling && c->profSteps < c->profMax;
+  hipEvent_t* ev = prof ? &c->events[size_t(c->profSteps) * kPhaseEvents] : nullptr;
+  int* evOf = prof ? &c->eventOf[size_t(c->profSteps) * kPhaseEvents] : nullptr;
+  int mark = 0;
+  // phase(false): the phase had no work; its boundary reuses the previous event (an event
+  // record costs GPU time of its own)
+  auto phase = [&](bool work = true) {
+    if (prof && mark < kPhaseEvents) {
+      if (work || mark == 0) {
+        (void)hipEventRecord(ev[mark], s);
+        evOf[mark] = mark;
+      } else {
+        evOf[mark] = evOf[mark - 1];
+      }
+    }
+    ++mark;
+  };
   phase();
+  const int force = c->forceRebuild ? 1 : 0;
+  c->forceRebuild = false;
+  const int* gate = P.devInt + 8;  // broad phase only on rebuild steps
+  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, force);
+  phase();
+  if (c->coopBlocks > 0) {
+    // one launch with grid barriers; all of its blocks return at once on steps that keep the pairs
+    hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(256), 0, s, P);
+    phase();        // sort (the whole broad phase)
+    phase(false);   // candidate pairs
+  } else {
+    const int rc = broadPhaseLaunches(c, gate, phase);
+    if (rc) return rc;
+  }
+  phase(c->coopBlocks == 0);  // reaction lists
   // narrow phase + contact law (an audit step evaluates every live pair)
   const bool useSur = c->sur.enabled && c->sur.trained;
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
@@ -501,9 +525,9 @@ int runStep(lsdem_ctx* c) {
   phase();
   // (history: this step's contacts went into this step's table inside k_exact_contacts; the phase
   // stays in the profile layout)
-  phase();
+  phase(false);
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
-  phase();
+  phase(P.nBonds > 0);
   if (c->hookFn) {
     hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 0);
     lsdem_device_view v;
@@ -723,6 +747,11 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.quirks = c->cfg.quirks;
   P.sdfCull = c->cfg.no_sdf_cull ? 0 : 1;
   P.useCert = c->cfg.no_certificates ? 0 : 1;
+  {
+    const int nb = c->cfg.narrow_batch;
+    if (nb != 0 && nb != 1 && nb != 2 && nb != 4 && nb != 8) return fail(LSDEM_ERR_ARG, "narrow_batch must be 0, 1, 2, 4 or 8");
+    P.narrowBatch = nb;
+  }
   P.periodicX = c->cfg.periodic_x != 0;
   P.periodicY = c->cfg.periodic_y != 0;
   const d3 dmn = mk3(c->cfg.domain_min[0], c->cfg.domain_min[1], c->cfg.domain_min[2]);
@@ -897,6 +926,16 @@ int lsdem_initialize(lsdem_ctx* c) {
   long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);
   // narrow-phase grid: enough waves for ~16 candidate pairs per particle, at most 8 per SIMD
   c->narrowBlocks = int(std::min<long long>(c->numCU * 8ll, std::max<long long>(1, (16ll * nP + 8 * 4 - 1) / (8 * 4))));
+  // single-launch broad phase: a grid whose blocks can all be resident together (at most 2 per
+  // CU, within what the kernel's registers allow)
+  c->coopBlocks = 0;
+  if (!c->cfg.broad_phase_launches) {
+    int perCU = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k_broad_phase), 256, 0));
+    if (perCU < 1) return fail(LSDEM_ERR_HIP, "the single-launch broad phase does not fit a CU");
+    c->coopBlocks = std::min(perCU, 2) * c->numCU;
+    if (c->cfg.broad_phase_blocks > 0) c->coopBlocks = std::min(c->coopBlocks, int(c->cfg.broad_phase_blocks));
+  }
   // candidate nodes (slots of the exact test); reserved per batch at their exact count
   long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity : 64ll * nP + 16384;
   pairCap = std::min<long long>(pairCap, (1ll << 31) - 1);
@@ -909,11 +948,11 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairInfo.alloc(pairCap));
   HIPCHK(c->dPairCoff.alloc(pairCap));
   HIPCHK(c->dPairCcnt.alloc(pairCap));
-  HIPCHK(c->dPairMap.alloc(pairCap * 10));
   HIPCHK(c->dSurMask.alloc(pairCap));
   HIPCHK(c->dSurSlack.alloc(pairCap));
   HIPCHK(c->dCert.alloc(pairCap));
   HIPCHK(c->dPairRec.alloc(pairCap));
+  HIPCHK(c->dPairMap.alloc(pairCap * 10));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
   HIPCHK(c->dRcPair.alloc(pairCap));
@@ -1010,7 +1049,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
   P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p; P.surMask = c->dSurMask.p;
   P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
-  P.rcTmp = c->dRcTmp.p;
+  P.rcTmp = c->dRcTmp.p; P.sortTmp = c->dSortTmp.p; P.sortBig = c->dSortBig.p; P.scanTiles = c->dScanBlocks.p;
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
   P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
   P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
@@ -1040,6 +1079,8 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
   if (flags[3] & 1)
     return fail(LSDEM_ERR_CAPACITY, "candidate-pair buffer overflow (peak " + std::to_string(flags[4]) +
                                         " > capacity " + std::to_string(c->P.pairCap) + ")");
+  if (flags[3] & 4)
+    return fail(LSDEM_ERR_STATE, "broad phase: a grid barrier timed out (blocks were not resident together)");
   if (flags[3] & 2)
     return fail(LSDEM_ERR_CAPACITY, "contact buffer overflow (peak " + std::to_string(flags[5]) + " > capacity " +
                                         std::to_string(c->P.contactCap) + ")");
@@ -1211,6 +1252,7 @@ int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
     HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     c->events.push_back(e);
   }
+  c->eventOf.assign(need, 0);
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemset(c->dDevStat.p + 3, 0, 6 * sizeof(unsigned long long)));
   HIPCHK(hipMemset(c->dDevStat.p + 12, 0, 3 * sizeof(unsigned long long)));
@@ -1229,9 +1271,10 @@ int lsdem_profile_end(lsdem_ctx* c, lsdem_profile* out) {
   out->steps = c->profSteps;
   for (int st = 0; st < c->profSteps; ++st) {
     hipEvent_t* ev = &c->events[size_t(st) * kPhaseEvents];
+    const int* of = &c->eventOf[size_t(st) * kPhaseEvents];
     for (int ph = 0; ph < LSDEM_NUM_PHASES; ++ph) {
       float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, ev[ph], ev[ph + 1]));
+      if (of[ph] != of[ph + 1]) HIPCHK(hipEventElapsedTime(&ms, ev[of[ph]], ev[of[ph + 1]]));
       out->ms[ph] += ms;
     }
   }

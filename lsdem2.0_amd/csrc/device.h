@@ -55,6 +55,7 @@ struct Params {
   d3 g;
   double dt;
   int quirks, periodicX, periodicY, sdfCull;
+  int narrowBatch;    // pairs per wavefront batch of k_candidates (0: chosen on the device)
   GridGeom gridP, gridW;
   SearchGrid sg;
   double rMax;        // max particle bounding radius
@@ -69,7 +70,7 @@ struct Params {
   const NodeDev* nodes;
   const ClusterF32* clusters32;
   const ClusterF32* subs32;  // sub-clusters of <= 16 nodes
-  float4* pairMap;        // per evaluated pair: 160-byte FP32 map record (10 float4)
+  float4* pairMap;        // per evaluated pair (list position): 160-byte FP32 map record
   const float4* nodes32;  // same nodes in FP32 (x, y, z, 0) for the conservative prefilter
   const float* sdf;
   int maxClusters;        // largest cluster count of any shape
@@ -131,6 +132,9 @@ struct Params {
   float4* sPRw;     // FP32 (pos after the periodic wrap, bounding radius)
   float* sBoxR;     // scaled grid-box radius (ghost-image reach)
   int* rcTmp;       // reaction pairs before the per-partner sort
+  int* sortTmp;     // counting-sort scratch (items before the per-bucket sort)
+  int* sortBig;     // counting-sort scratch (list of buckets of more than 16 items)
+  int* scanTiles;   // scan scratch (tile sums)
   uint32_t* pairSlab;  // kPairSlab x nP: first candidate pairs of each master (column per master)
 
   // ---- candidate pairs (current step) ----
@@ -143,7 +147,7 @@ struct Params {
   // separation certificates: a pair whose nodes were all culled with slack s (world units, less
   // the motion bound at that step) is skipped while the motion bound since the build stays below s
   float4* cert;     // pairCap: (s or -1, cB, cA, 0); bound = |dA - dB| + chordB cB + chordA cA
-  int* pairRec;     // pair -> index of its map record this step
+  int* pairRec;     // pair -> position of its map record this step (devInt[9] records)
   int useCert;
   // reaction CSR (pairs whose partner is a particle that receives the reaction)
   int* rcCount;     // nP
@@ -193,7 +197,8 @@ struct Params {
   //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert,
   //         [7] images at their source's own position, [8] rebuild flag of the step,
   //         [9] pairs the narrow phase evaluates this step, [10 + b] contact slots of the
-  //         last step that filled history table b
+  //         last step that filled history table b, [12] grid-barrier arrivals of the broad phase;
+  //         overflow flag 4: a broad-phase grid barrier timed out
   int* devInt;
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
@@ -208,8 +213,9 @@ struct Params {
   const int* surSlot;      // family slot of each shape id, -1 outside the family; nullptr: untrained
   float surBand;           // cull band, unscaled SDF units (2 x measured max error)
   const int* nodeCluster;  // per shape node (caller order, at the shape's node offset): its cluster
-  unsigned long long* surMask;  // per map record: bit c = 0 when the network culls cluster c
-  float* surSlack;              // per map record: smallest slack of the clusters the network culled
+  unsigned long long* surMask;  // per evaluated pair (list position): bit c = 0 when the network
+                                // culls cluster c
+  float* surSlack;              // per evaluated pair: smallest slack of the clusters it culled
 };
 
 constexpr int kStatSlots = 32;

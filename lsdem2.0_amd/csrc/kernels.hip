@@ -120,7 +120,7 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
     a[i] = v;
 }
 
-// Exclusive scan of n ints (n + 1 outputs, out[n] = total), three kernels.
+// Exclusive scan of n ints (n + 1 outputs, out[n] = total) in two phases over tiles.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
@@ -152,52 +152,67 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int* lds, int& total) {
   return r;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* in, long long n, int* blockSum,
+// Grid-stride loop bounds of one phase: this thread's index and the number of threads.
+struct Span {
+  int tid, nth;
+};
+__device__ __forceinline__ Span gridSpan() {
+  return Span{int(blockIdx.x * blockDim.x + threadIdx.x), int(gridDim.x * blockDim.x)};
+}
+
+// Phase 1 of the scan: the sum of each tile of kScanTile inputs (blocks loop over tiles).
+__device__ void scanReduceTiles(const int* in, long long n, int* tileSum, int* lds) {
+  const long long nTiles = (n + kScanTile - 1) / kScanTile;
+  for (long long t = blockIdx.x; t < nTiles; t += gridDim.x) {
+    const long long base = t * kScanTile + (long long)threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+      if (base + k < n) s += in[base + k];
+    int total;
+    blockExclusiveScan(s, lds, total);
+    if (threadIdx.x == 0) tileSum[t] = total;
+  }
+}
+// Phase 2: each tile adds the sums of the tiles before it and writes its exclusive prefix; the
+// last tile writes the total to out[n].  n == 0 writes out[0] = 0.
+__device__ void scanApplyTiles(const int* in, long long n, const int* tileSum, int* out, int* lds) {
+  const long long nTiles = (n + kScanTile - 1) / kScanTile;
+  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+  for (long long t = blockIdx.x; t < nTiles; t += gridDim.x) {
+    int pre = 0;
+    for (long long u = threadIdx.x; u < t; u += blockDim.x) pre += tileSum[u];
+    int preTotal;
+    blockExclusiveScan(pre, lds, preTotal);
+    const long long base = t * kScanTile + (long long)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = (base + k < n) ? in[base + k] : 0;
+      s += v[k];
+    }
+    int total;
+    int ex = blockExclusiveScan(s, lds, total) + preTotal;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) out[base + k] = ex;
+      ex += v[k];
+    }
+    if (t == nTiles - 1 && threadIdx.x == 0) out[n] = preTotal + total;
+  }
+}
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* in, long long n, int* tileSum,
                                                                const int* gate) {
   LSD_GATE(gate);
   __shared__ int lds[kScanThreads / 64 + 1];
-  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k)
-    if (base + k < n) s += in[base + k];
-  int total;
-  blockExclusiveScan(s, lds, total);
-  if (threadIdx.x == 0) blockSum[blockIdx.x] = total;
+  scanReduceTiles(in, n, tileSum, lds);
 }
-__global__ __launch_bounds__(kScanThreads) void k_scan_top(int* blockSum, int nb, int* totalOut, const int* gate) {
-  LSD_GATE(gate);
-  __shared__ int lds[kScanThreads / 64 + 1];
-  int carry = 0;
-  for (int base = 0; base < nb; base += kScanThreads) {
-    const int i = base + threadIdx.x;
-    const int v = i < nb ? blockSum[i] : 0;
-    int total;
-    const int ex = blockExclusiveScan(v, lds, total);
-    if (i < nb) blockSum[i] = ex + carry;
-    carry += total;
-  }
-  if (threadIdx.x == 0) *totalOut = carry;
-}
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, long long n, const int* blockSum, int* out,
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, long long n, const int* tileSum, int* out,
                                                               const int* gate) {
   LSD_GATE(gate);
   __shared__ int lds[kScanThreads / 64 + 1];
-  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0;
-    s += v[k];
-  }
-  int total;
-  int ex = blockExclusiveScan(s, lds, total) + blockSum[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) out[base + k] = ex;
-    ex += v[k];
-  }
+  scanApplyTiles(in, n, tileSum, out, lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -236,8 +251,9 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     if (forceRebuild) atomicOr(&P.devInt[8], 1);
-    P.devInt[1] = 0;  // contact slots allocated
-    P.devInt[9] = 0;  // pairs the narrow phase evaluates
+    P.devInt[1] = 0;   // contact slots allocated
+    P.devInt[9] = 0;   // pairs the narrow phase evaluates
+    P.devInt[12] = 0;  // arrivals at the broad phase's grid barriers
   }
   const int nClear = P.devInt[10 + P.curBuf];
   for (int c = i; c < nClear; c += gridDim.x * blockDim.x) {
@@ -334,9 +350,11 @@ __device__ void wallDrift(const Params& P, int w) {
 }
 
 // Record the poses the candidate pairs were built from (rebuild steps only).
-__global__ void k_mark_build(Params P) {
-  LSD_GATE(P.devInt + 8);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void markBuildOne(const Params& P, int i);
+__device__ void markBuild(const Params& P, Span g) {
+  for (int i = g.tid; i < max(P.nP, P.nW); i += g.nth) markBuildOne(P, i);
+}
+__device__ void markBuildOne(const Params& P, int i) {
   if (i < P.nP) {
     P.xBuild[i] = P.posDet[i];
     P.xBuildW[i] = P.pos[i];
@@ -358,65 +376,58 @@ __global__ void k_mark_build(Params P) {
 // Deterministic counting sort by bucket: count (atomic rank) -> scan -> scatter -> per-bucket
 // insertion sort by item index.
 // ------------------------------------------------------------------------------------------
-__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate) {
-  LSD_GATE(gate);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = bucket[i];
-  rank[i] = b >= 0 ? atomicAdd(&count[b], 1) : -1;
+__device__ void bucketCount(const int* bucket, int n, int* count, int* rank, Span g) {
+  for (int i = g.tid; i < n; i += g.nth) {
+    const int b = bucket[i];
+    rank[i] = b >= 0 ? atomicAdd(&count[b], 1) : -1;
+  }
 }
-__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
-                                 int* bigCount, const int* gate) {
-  LSD_GATE(gate);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *bigCount = 0;  // list of large buckets of the following k_bucket_sort
-  if (i >= n) return;
-  const int b = bucket[i];
-  if (b >= 0) out[start[b] + rank[i]] = i;
+__device__ void bucketScatter(const int* bucket, int n, const int* start, const int* rank, int* out, Span g) {
+  for (int i = g.tid; i < n; i += g.nth) {
+    const int b = bucket[i];
+    if (b >= 0) out[start[b] + rank[i]] = i;
+  }
 }
 // Buckets of up to 16 items are sorted in registers (odd-even transposition network) from
-// `tmp` into `items`; larger ones are listed for k_bucket_sort_big.
-__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
-                              const int* gate) {
-  LSD_GATE(gate);
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const int s = start[b], e = start[b + 1];
-  const int n = e - s;
-  if (n == 0) return;
-  if (n == 1) {
-    items[s] = tmp[s];
-    return;
-  }
-  if (n > 16) {
-    bigList[atomicAdd(bigCount, 1)] = b;
-    return;
-  }
-  int v[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = k < n ? tmp[s + k] : 0x7fffffff;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-#pragma unroll
-    for (int k = (r & 1); k + 1 < 16; k += 2) {
-      const int a = v[k], c = v[k + 1];
-      v[k] = min(a, c);
-      v[k + 1] = max(a, c);
+// `tmp` into `items`; larger ones are listed for bucketSortBig (*bigCount starts at 0).
+__device__ void bucketSortSmall(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
+                                Span g) {
+  for (int b = g.tid; b < nb; b += g.nth) {
+    const int s = start[b], e = start[b + 1];
+    const int n = e - s;
+    if (n == 0) continue;
+    if (n == 1) {
+      items[s] = tmp[s];
+      continue;
     }
-  }
+    if (n > 16) {
+      bigList[atomicAdd(bigCount, 1)] = b;
+      continue;
+    }
+    int v[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k)
-    if (k < n) items[s + k] = v[k];
+    for (int k = 0; k < 16; ++k) v[k] = k < n ? tmp[s + k] : 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+#pragma unroll
+      for (int k = (r & 1); k + 1 < 16; k += 2) {
+        const int a = v[k], c = v[k + 1];
+        v[k] = min(a, c);
+        v[k + 1] = max(a, c);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < n) items[s + k] = v[k];
+  }
 }
 // One wavefront per listed bucket: each item's place is the number of smaller items (items are
-// distinct indices).
-__global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
-                                  const int* bigCount, const int* gate) {
-  LSD_GATE(gate);
+// distinct indices).  `wave` / `nWaves`: this wavefront's index and the number of wavefronts.
+__device__ void bucketSortBig(const int* start, const int* tmp, int* items, const int* bigList, const int* bigCount,
+                              int wave, int nWaves) {
   const int lane = threadIdx.x & 63;
   const int nBig = *bigCount;
-  const int nWaves = int(gridDim.x * (blockDim.x >> 6));
-  for (int q = int(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); q < nBig; q += nWaves) {
+  for (int q = wave; q < nBig; q += nWaves) {
     const int b = bigList[q];
     const int s = start[b], e = start[b + 1];
     for (int c0 = s; c0 < e; c0 += 64) {
@@ -430,13 +441,31 @@ __global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, 
     }
   }
 }
+__global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate) {
+  LSD_GATE(gate);
+  bucketCount(bucket, n, count, rank, gridSpan());
+}
+__global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
+                                 int* bigCount, const int* gate) {
+  LSD_GATE(gate);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *bigCount = 0;  // list of large buckets of k_bucket_sort
+  bucketScatter(bucket, n, start, rank, out, gridSpan());
+}
+__global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
+                              const int* gate) {
+  LSD_GATE(gate);
+  bucketSortSmall(start, nb, tmp, items, bigList, bigCount, gridSpan());
+}
+__global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
+                                  const int* bigCount, const int* gate) {
+  LSD_GATE(gate);
+  const Span g = gridSpan();
+  bucketSortBig(start, tmp, items, bigList, bigCount, g.tid >> 6, g.nth >> 6);
+}
 
 // Gather particle (or ghost-image) data into the sorted order of the search-grid table so
 // that the candidate loops stream contiguous memory.
-__global__ void k_gather_particles(Params P) {
-  LSD_GATE(P.devInt + 8);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.nP) return;
+__device__ void gatherParticle(const Params& P, int k) {
   const int i = P.sorted[k];
   const d3 x = P.posDet[i], w = P.pos[i];
   const float r = float(P.radius[i]);
@@ -446,6 +475,11 @@ __global__ void k_gather_particles(Params P) {
   P.sPR[k] = make_float4(float(x.x), float(x.y), float(x.z), eligible ? r : -1.f);
   P.sPRw[k] = make_float4(float(w.x), float(w.y), float(w.z), r);
   P.sBoxR[k] = float(P.scale[i] * P.shapes[P.shape[i]].boxR);
+}
+__global__ void k_gather_particles(Params P) {
+  LSD_GATE(P.devInt + 8);
+  const Span g = gridSpan();
+  for (int k = g.tid; k < P.nP; k += g.nth) gatherParticle(P, k);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -631,37 +665,46 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
 }
 
 // Broad phase, one thread per master in spatial order: count and keep the first candidates.
+__device__ void pairsFind(const Params& P, Span g) {
+  for (int t = g.tid; t < P.nP; t += g.nth) {
+    const int A = P.sorted[t];
+    P.pairCount[A] = pairsOf<false>(P, t, A, 0);
+  }
+}
 __global__ void k_pairs_find(Params P) {
   LSD_GATE(P.devInt + 8);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.nP) return;
-  const int A = P.sorted[t];
-  P.pairCount[A] = pairsOf<false>(P, t, A, 0);
+  pairsFind(P, gridSpan());
 }
 // Pair list in master order: copy the slab, or search again for masters with more candidates.
+__device__ void pairsEmit(const Params& P, Span g) {
+  for (int t = g.tid; t < P.nP; t += g.nth) {
+    const int A = P.sorted[t];
+    const int n = P.pairCount[A];
+    const long long out = P.pairStart[A];
+    if (n > kPairSlab) {
+      pairsOf<true>(P, t, A, out);
+      continue;
+    }
+    for (int k = 0; k < n; ++k) {
+      if (out + k >= P.pairCap) break;
+      P.pairA[out + k] = A;
+      P.pairInfo[out + k] = P.pairSlab[(long long)k * P.nP + t];
+    }
+  }
+}
 __global__ void k_pairs_emit(Params P) {
   LSD_GATE(P.devInt + 8);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.nP) return;
-  const int A = P.sorted[t];
-  const int n = P.pairCount[A];
-  const long long out = P.pairStart[A];
-  if (n > kPairSlab) {
-    pairsOf<true>(P, t, A, out);
-    return;
-  }
-  for (int k = 0; k < n; ++k) {
-    if (out + k >= P.pairCap) break;
-    P.pairA[out + k] = A;
-    P.pairInfo[out + k] = P.pairSlab[(long long)k * P.nP + t];
-  }
+  pairsEmit(P, gridSpan());
+}
+__global__ void k_mark_build(Params P) {
+  LSD_GATE(P.devInt + 8);
+  markBuild(P, gridSpan());
 }
 
 // Pairs whose partner particle receives the reaction (particle-particle, ghosts with j != A).
-__global__ void k_rc_count(Params P) {
-  LSD_GATE(P.devInt + 8);
+__device__ void rcCountPairs(const Params& P, Span g) {
   const int n = min(P.pairStart[P.nP], (int)P.pairCap);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+  for (int p = g.tid; p < n; p += g.nth) {
     const uint32_t info = P.pairInfo[p];
     const uint32_t type = info >> 28;
     const int partner = int(info & kPartnerMask);
@@ -670,16 +713,23 @@ __global__ void k_rc_count(Params P) {
     P.rcRank[p] = r;
   }
 }
-__global__ void k_rc_scatter(Params P) {
-  LSD_GATE(P.devInt + 8);
+__device__ void rcScatterPairs(const Params& P, Span g) {
   const int n = min(P.pairStart[P.nP], (int)P.pairCap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[2] = 0;  // large-bucket list of the following sort
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+  for (int p = g.tid; p < n; p += g.nth) {
     const int r = P.rcRank[p];
     if (r < 0) continue;
     const int partner = int(P.pairInfo[p] & kPartnerMask);
     P.rcTmp[P.rcStart[partner] + r] = p;
   }
+}
+__global__ void k_rc_count(Params P) {
+  LSD_GATE(P.devInt + 8);
+  rcCountPairs(P, gridSpan());
+}
+__global__ void k_rc_scatter(Params P) {
+  LSD_GATE(P.devInt + 8);
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[2] = 0;  // large-bucket list of the following sort
+  rcScatterPairs(P, gridSpan());
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1185,10 +1235,11 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const int iters = (maxC + 7) >> 3;
   const int j = lane >> 3, k = lane & 7;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int base = wave * kBatch; base < nAct; base += nWaves * kBatch) {
+  const int nb = P.narrowBatch > 0 ? P.narrowBatch : kBatch;  // pairs per batch
+  for (int base = wave * nb; base < nAct; base += nWaves * nb) {
     // ---- level 1: clusters ----
     const int a = base + j;  // map record
-    const bool own = a < nAct;
+    const bool own = j < nb && a < nAct;
     // the 8 lanes of a pair copy its 160-byte map record into LDS, 20 bytes each
     if (own) {
       const unsigned int* src = reinterpret_cast<const unsigned int*>(P.pairMap) + (long long)a * 40 + 5 * k;
@@ -1295,7 +1346,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         total = 0;
       }
     }
-    if (lane < kBatch && base + lane < nAct) {
+    if (lane < nb && base + lane < nAct) {
       const PairF32& fl = sF[lane];
       int off = first;
       for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
@@ -1660,6 +1711,93 @@ __device__ void stepEnd(const Params& P) {
   // the drift kernel clears them)
 }
 
+
+// ------------------------------------------------------------------------------------------
+// The whole broad phase in one launch (rebuild steps only; every block returns at once
+// otherwise).  Phases are separated by grid barriers; every block runs every phase, so all blocks
+// reach every barrier.  The host sizes the grid so that all its blocks are resident together.
+// ------------------------------------------------------------------------------------------
+// Grid barrier on the counter devInt[12] (cleared by the drift kernel every step): each block
+// adds one arrival per barrier and waits for nBlocks x (barriers passed).  The wait is bounded:
+// after about a second it sets overflow flag 4 and lets the block go on (the step then reports
+// an error instead of hanging).
+struct GridBarrier {
+  int* devInt;
+  int nBlocks, passed;
+  __device__ void sync() {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      ++passed;
+      const int target = passed * nBlocks;
+      int* count = devInt + 12;
+      __threadfence();  // this block's writes, before the arrival
+      atomicAdd(count, 1);
+      // relaxed polling (an acquire per poll would invalidate the caches every time), then one
+      // acquire fence
+      for (unsigned int it = 0;
+           __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
+        if (it > (1u << 24)) {
+          atomicOr(devInt + 3, 4);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();  // the other blocks' writes, after the wait
+    }
+    __syncthreads();
+  }
+};
+
+__global__ __launch_bounds__(256) void k_broad_phase(Params P) {
+  if (P.devInt[8] == 0) return;  // written by the drift kernel before this launch: grid-uniform
+  __shared__ int lds[kScanThreads / 64 + 1];
+  GridBarrier grid{P.devInt, int(gridDim.x), 0};
+  const Span g = gridSpan();
+  const int wave = g.tid >> 6, nWaves = g.nth >> 6;
+  const int ncell = P.sg.ncell;
+  // counting sort of the particles by search cell
+  for (int i = g.tid; i < ncell; i += g.nth) P.cellCount[i] = 0;
+  for (int i = g.tid; i < P.nP; i += g.nth) P.rcCount[i] = 0;
+  if (g.tid == 0) P.devInt[2] = 0;  // large-bucket list
+  grid.sync();
+  bucketCount(P.scell, P.nP, P.cellCount, P.rank, g);
+  grid.sync();
+  scanReduceTiles(P.cellCount, ncell, P.scanTiles, lds);
+  grid.sync();
+  scanApplyTiles(P.cellCount, ncell, P.scanTiles, P.cellStart, lds);
+  grid.sync();
+  bucketScatter(P.scell, P.nP, P.cellStart, P.rank, P.sortTmp, g);
+  grid.sync();
+  bucketSortSmall(P.cellStart, ncell, P.sortTmp, P.sorted, P.sortBig, P.devInt + 2, g);
+  grid.sync();
+  bucketSortBig(P.cellStart, P.sortTmp, P.sorted, P.sortBig, P.devInt + 2, wave, nWaves);
+  grid.sync();
+  // particle data in sorted order; candidate pairs per master, prefix, pair list
+  for (int k = g.tid; k < P.nP; k += g.nth) gatherParticle(P, k);
+  if (g.tid == 0) P.devInt[2] = 0;  // large-bucket list of the reaction sort
+  grid.sync();
+  pairsFind(P, g);
+  grid.sync();
+  scanReduceTiles(P.pairCount, P.nP, P.scanTiles, lds);
+  grid.sync();
+  scanApplyTiles(P.pairCount, P.nP, P.scanTiles, P.pairStart, lds);
+  grid.sync();
+  pairsEmit(P, g);
+  markBuild(P, g);
+  grid.sync();
+  // reaction CSR: pairs by partner, in pair order
+  rcCountPairs(P, g);
+  grid.sync();
+  scanReduceTiles(P.rcCount, P.nP, P.scanTiles, lds);
+  grid.sync();
+  scanApplyTiles(P.rcCount, P.nP, P.scanTiles, P.rcStart, lds);
+  grid.sync();
+  rcScatterPairs(P, g);
+  grid.sync();
+  bucketSortSmall(P.rcStart, P.nP, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, g);
+  grid.sync();
+  bucketSortBig(P.rcStart, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, wave, nWaves);
+}
 
 template __global__ void k_candidates<false>(Params P);
 template __global__ void k_candidates<true>(Params P);

@@ -41,7 +41,10 @@ class Config(C.Structure):
         ("no_sdf_cull", C.c_int32),
         ("skin", C.c_double),
         ("no_certificates", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("broad_phase_launches", C.c_int32),
+        ("broad_phase_blocks", C.c_int32),
+        ("narrow_batch", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -169,6 +172,9 @@ class Sim:
         c.device = int(cfg.get("device", 0))
         c.no_sdf_cull = int(cfg.get("no_sdf_cull", 0))
         c.no_certificates = int(cfg.get("no_certificates", 0))
+        c.broad_phase_launches = int(cfg.get("broad_phase_launches", 0))
+        c.broad_phase_blocks = int(cfg.get("broad_phase_blocks", 0))
+        c.narrow_batch = int(cfg.get("narrow_batch", 0))
         c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()

@@ -38,7 +38,7 @@ def test_sphere_bounces_back(tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     pdir = Path(str(prefix) + "_phase1")
     frames = sorted((pdir / "LSParticle").glob("Particle_*.vtu"))
-    assert len(frames) == 202  # frame 0, 200 frames, the final one
+    assert len(frames) == 202, r.stdout[-2000:] + r.stderr[-2000:]  # frame 0, 200 frames, the final one
     for sub, stem in (("LSParticle", "LSObjectMesh"), ("Wall", "LSObjectMesh"),
                       ("LSParticleInteraction", "LSInteraction"), ("LSParticle-WallInteraction", "LSInteraction")):
         assert (pdir / sub / f"{stem}_0000.vtu").exists(), (sub, stem)
