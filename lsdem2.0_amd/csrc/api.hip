@@ -497,7 +497,7 @@ int runStep(lsdem_ctx* c) {
   phase();
   if (c->coopBlocks > 0) {
     // one launch with grid barriers; all of its blocks return at once on steps that keep the pairs
-    hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(kBroadThreadsHost), 0, s, P);
     phase();        // sort (the whole broad phase)
     phase(false);   // candidate pairs
   } else {
@@ -508,7 +508,8 @@ int runStep(lsdem_ctx* c) {
   // narrow phase + contact law (an audit step evaluates every live pair)
   const bool useSur = c->sur.enabled && c->sur.trained;
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
-  hipLaunchKernelGGL(k_pair_check, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P, audit ? 1 : 0);
+  // (a multiple of 8 workgroups: the kernel gives each XCD an eighth of the pairs)
+  hipLaunchKernelGGL(k_pair_check, dim3(unsigned((c->numCU * 4 + 7) / 8 * 8)), dim3(256), 0, s, P, audit ? 1 : 0);
   if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
   if (useSur && !audit)
     hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
@@ -884,7 +885,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.rMax = rMax;
   P.boxRMax = boxRMax;
   // skin of the reused candidate pairs (rebuilt when a body has moved half of it)
-  const double skin = c->cfg.skin >= 0.0 ? c->cfg.skin : 0.02 * rMax;
+  const double skin = c->cfg.skin >= 0.0 ? c->cfg.skin : 0.04 * rMax;
   P.halfSkin = 0.5 * skin;
   c->forceRebuild = true;
   {
@@ -926,14 +927,14 @@ int lsdem_initialize(lsdem_ctx* c) {
   long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);
   // narrow-phase grid: enough waves for ~16 candidate pairs per particle, at most 8 per SIMD
   c->narrowBlocks = int(std::min<long long>(c->numCU * 8ll, std::max<long long>(1, (16ll * nP + 8 * 4 - 1) / (8 * 4))));
-  // single-launch broad phase: a grid whose blocks can all be resident together (at most 2 per
-  // CU, within what the kernel's registers allow)
+  // single-launch broad phase: a grid whose blocks can all be resident together
   c->coopBlocks = 0;
   if (!c->cfg.broad_phase_launches) {
     int perCU = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k_broad_phase), 256, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k_broad_phase),
+                                                        kBroadThreadsHost, 0));
     if (perCU < 1) return fail(LSDEM_ERR_HIP, "the single-launch broad phase does not fit a CU");
-    c->coopBlocks = std::min(perCU, 2) * c->numCU;
+    c->coopBlocks = c->numCU;  // one block of 1024 threads per CU
     if (c->cfg.broad_phase_blocks > 0) c->coopBlocks = std::min(c->coopBlocks, int(c->cfg.broad_phase_blocks));
   }
   // candidate nodes (slots of the exact test); reserved per batch at their exact count

@@ -221,7 +221,8 @@ struct Params {
 constexpr int kStatSlots = 32;
 constexpr int kDevInts = 16;   // devInt entries
 constexpr int kDevStats = 16;  // devStat entries
-constexpr int kPairSlabHost = 16;  // slab depth of the broad phase (kPairSlab in the kernels)
+constexpr int kPairSlabHost = 16;
+constexpr int kBroadThreadsHost = 1024;  // block of the single-launch broad phase (kBroadThreads)  // slab depth of the broad phase (kPairSlab in the kernels)
 constexpr uint32_t kInvalidInfo = 0xFFFFFFFFu;  // unused contact slot
 
 }  // namespace lsd

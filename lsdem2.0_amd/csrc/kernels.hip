@@ -121,12 +121,12 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
 }
 
 // Exclusive scan of n ints (n + 1 outputs, out[n] = total) in two phases over tiles.
-constexpr int kScanThreads = 256;
+constexpr int kScanThreads = 256;   // block of the separate scan launches
+constexpr int kScanLds = 1024 / 64 + 1;  // LDS ints of blockExclusiveScan (blocks of up to 1024)
 constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
 
 __device__ __forceinline__ int blockExclusiveScan(int v, int* lds, int& total) {
-  // lds needs kScanThreads/64 + 1 ints
+  // any block size that is a multiple of 64 up to 1024; lds needs kScanLds ints
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int incl = v;
 #pragma unroll
@@ -138,15 +138,15 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int* lds, int& total) {
   __syncthreads();
   if (threadIdx.x == 0) {
     int acc = 0;
-    for (int w = 0; w < kScanThreads / 64; ++w) {
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) {
       int t = lds[w];
       lds[w] = acc;
       acc += t;
     }
-    lds[kScanThreads / 64] = acc;
+    lds[blockDim.x >> 6] = acc;
   }
   __syncthreads();
-  total = lds[kScanThreads / 64];
+  total = lds[blockDim.x >> 6];
   int r = lds[wid] + incl - v;
   __syncthreads();
   return r;
@@ -160,11 +160,13 @@ __device__ __forceinline__ Span gridSpan() {
   return Span{int(blockIdx.x * blockDim.x + threadIdx.x), int(gridDim.x * blockDim.x)};
 }
 
-// Phase 1 of the scan: the sum of each tile of kScanTile inputs (blocks loop over tiles).
+// Phase 1 of the scan: the sum of each tile of blockDim.x x kScanItems inputs (blocks loop over
+// tiles).
 __device__ void scanReduceTiles(const int* in, long long n, int* tileSum, int* lds) {
-  const long long nTiles = (n + kScanTile - 1) / kScanTile;
+  const long long tile = (long long)blockDim.x * kScanItems;
+  const long long nTiles = (n + tile - 1) / tile;
   for (long long t = blockIdx.x; t < nTiles; t += gridDim.x) {
-    const long long base = t * kScanTile + (long long)threadIdx.x * kScanItems;
+    const long long base = t * tile + (long long)threadIdx.x * kScanItems;
     int s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
@@ -177,14 +179,15 @@ __device__ void scanReduceTiles(const int* in, long long n, int* tileSum, int* l
 // Phase 2: each tile adds the sums of the tiles before it and writes its exclusive prefix; the
 // last tile writes the total to out[n].  n == 0 writes out[0] = 0.
 __device__ void scanApplyTiles(const int* in, long long n, const int* tileSum, int* out, int* lds) {
-  const long long nTiles = (n + kScanTile - 1) / kScanTile;
+  const long long tile = (long long)blockDim.x * kScanItems;
+  const long long nTiles = (n + tile - 1) / tile;
   if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
   for (long long t = blockIdx.x; t < nTiles; t += gridDim.x) {
     int pre = 0;
     for (long long u = threadIdx.x; u < t; u += blockDim.x) pre += tileSum[u];
     int preTotal;
     blockExclusiveScan(pre, lds, preTotal);
-    const long long base = t * kScanTile + (long long)threadIdx.x * kScanItems;
+    const long long base = t * tile + (long long)threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
 #pragma unroll
@@ -205,13 +208,13 @@ __device__ void scanApplyTiles(const int* in, long long n, const int* tileSum, i
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* in, long long n, int* tileSum,
                                                                const int* gate) {
   LSD_GATE(gate);
-  __shared__ int lds[kScanThreads / 64 + 1];
+  __shared__ int lds[kScanLds];
   scanReduceTiles(in, n, tileSum, lds);
 }
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, long long n, const int* tileSum, int* out,
                                                               const int* gate) {
   LSD_GATE(gate);
-  __shared__ int lds[kScanThreads / 64 + 1];
+  __shared__ int lds[kScanLds];
   scanApplyTiles(in, n, tileSum, out, lds);
 }
 
@@ -985,7 +988,13 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   const int lane = threadIdx.x & 63;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
-  for (int t0 = blockIdx.x * kCheckTile; t0 < nPairs; t0 += gridDim.x * kCheckTile) {
+  // XCD-aware tiles: workgroups go to the 8 XCDs in turn (blockIdx mod 8), so each XCD takes one
+  // contiguous eighth of the pair list and its partners' data stays in that XCD's L2
+  const int nTiles = (nPairs + kCheckTile - 1) / kCheckTile;
+  const int xcd = int(blockIdx.x) & 7, perX = max(1, int(gridDim.x) >> 3);
+  const int tLo = int((long long)nTiles * xcd / 8), tHi = int((long long)nTiles * (xcd + 1) / 8);
+  for (int t = tLo + (int(blockIdx.x) >> 3); t < tHi; t += perX) {
+    const int t0 = t * kCheckTile;
     if (threadIdx.x == 0) sCount = 0;
     __syncthreads();
     for (int r = 0; r < kCheckTile / 256; ++r) {
@@ -1047,7 +1056,13 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   if (lane == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
 }
 
+// The FP32 prefilters may contract a * b + c into one rounding: their margins bound the rounding
+// of separate operations, and a fused operation rounds less.  (The FP64 membership tests keep the
+// file-wide -ffp-contract=off.)
+#define FP32_CONTRACT _Pragma("clang fp contract(fast)")
+
 __device__ __forceinline__ void mapF32(const PairF32& f, float nx, float ny, float nz, float& gx, float& gy, float& gz) {
+  FP32_CONTRACT
   gx = f.m[0] * nx + f.m[1] * ny + f.m[2] * nz + f.t[0];
   gy = f.m[3] * nx + f.m[4] * ny + f.m[5] * nz + f.t[1];
   gz = f.m[6] * nx + f.m[7] * ny + f.m[8] * nz + f.t[2];
@@ -1058,6 +1073,7 @@ __device__ __forceinline__ void mapF32(const PairF32& f, float nx, float ny, flo
 constexpr float kRelPhi = 4e-6f;
 __device__ __forceinline__ float phiF32(const float* __restrict__ sdf, const PairF32& f, float gx, float gy, float gz,
                                         float& cmax) {
+  FP32_CONTRACT
   const int i0 = min(max(int(floorf(gx)), 0), f.nx - 2), j0 = min(max(int(floorf(gy)), 0), f.ny - 2),
             k0 = min(max(int(floorf(gz)), 0), f.nz - 2);
   const float x = fminf(fmaxf(gx - float(i0), 0.f), 1.f), y = fminf(fmaxf(gy - float(j0), 0.f), 1.f),
@@ -1087,6 +1103,7 @@ __device__ __forceinline__ float phiF32(const float* __restrict__ sdf, const Pai
 // grid box (then an SDF bound at the centre can cull the cluster).
 __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF32& cl, float& gx, float& gy, float& gz,
                                                float& rho, bool& inside, float& slack) {
+  FP32_CONTRACT
   mapF32(f, cl.cx, cl.cy, cl.cz, gx, gy, gz);
   rho = cl.rho * f.rhoScale;
   inside = false;
@@ -1113,11 +1130,13 @@ __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF3
 }
 // Radius growth that a Lipschitz bound phi > lipH * 1.001 * radius + lim still allows.
 __device__ __forceinline__ float lipSlack(const PairF32& f, float phi, float lim, float rho) {
+  FP32_CONTRACT
   return f.lipH > 0.f ? (phi - lim) / (f.lipH * 1.001f) * 0.9999f - rho : 0.f;
 }
 // SDF part with the exact grid: phi(centre) beyond Lipschitz reach of zero culls the cluster.
 __device__ __forceinline__ bool clusterSdfF32(const float* __restrict__ sdf, const PairF32& f, float gx, float gy,
                                               float gz, float rho, float& slack) {
+  FP32_CONTRACT
   float cmax;
   const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
   const float lim = f.phiMargin + kRelPhi * cmax;
@@ -1153,6 +1172,7 @@ __device__ __forceinline__ void surQueryPoint(const Params& P, const PairF32& f,
 // Node acceptance (conservative superset of the exact FP64 test); slack as for clusters, with
 // the node as a cluster of radius 0.
 __device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const PairF32& f, float4 n, float& slack) {
+  FP32_CONTRACT
   float gx, gy, gz;
   mapF32(f, n.x, n.y, n.z, gx, gy, gz);
   slack = 0.f;
@@ -1172,6 +1192,56 @@ __device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const Pai
   }
   float cmax;
   const float phi = phiF32(sdf, f, gx, gy, gz, cmax);
+  const float lim = f.phiMargin + kRelPhi * cmax;
+  if (phi <= lim) return true;
+  slack = lipSlack(f, phi, lim, 0.f);
+  return false;
+}
+
+// nodeF32 in three stages, so that the grid reads of several nodes can be in flight together:
+// the sphere and box tests (slack when they cull), the eight corner reads, the SDF test.
+__device__ __forceinline__ bool nodeGeomF32(const PairF32& f, float4 n, float& gx, float& gy, float& gz, float& slack) {
+  FP32_CONTRACT
+  mapF32(f, n.x, n.y, n.z, gx, gy, gz);
+  slack = 0.f;
+  if (f.r2 >= 0.f) {
+    const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > f.r2) {
+      slack = sqrtf(d2) * 0.9999f - sqrtf(f.r2);
+      return false;
+    }
+  }
+  const float out = fmaxf(fmaxf(fmaxf(f.lo - gx, gx - f.hi[0]), fmaxf(f.lo - gy, gy - f.hi[1])),
+                          fmaxf(f.lo - gz, gz - f.hi[2]));
+  if (out > 0.f) {
+    slack = out * 0.9999f;
+    return false;
+  }
+  return true;
+}
+__device__ __forceinline__ void sdfCorners(const float* __restrict__ sdf, const PairF32& f, float gx, float gy, float gz,
+                                           float (&c)[8], float& x, float& y, float& z) {
+  FP32_CONTRACT
+  const int i0 = min(max(int(floorf(gx)), 0), f.nx - 2), j0 = min(max(int(floorf(gy)), 0), f.ny - 2),
+            k0 = min(max(int(floorf(gz)), 0), f.nz - 2);
+  x = fminf(fmaxf(gx - float(i0), 0.f), 1.f);
+  y = fminf(fmaxf(gy - float(j0), 0.f), 1.f);
+  z = fminf(fmaxf(gz - float(k0), 0.f), 1.f);
+  const int sx = 1, sy = f.nx, sz = f.nx * f.ny;
+  const float* g = sdf + f.sdfOff + i0 + j0 * sy + k0 * sz;
+  c[0] = g[0]; c[1] = g[sx]; c[2] = g[sy]; c[3] = g[sx + sy];
+  c[4] = g[sz]; c[5] = g[sx + sz]; c[6] = g[sy + sz]; c[7] = g[sx + sy + sz];
+}
+// Same arithmetic as phiF32 and the SDF part of nodeF32.
+__device__ __forceinline__ bool nodeSdfF32(const PairF32& f, const float (&c)[8], float x, float y, float z,
+                                           float& slack) {
+  FP32_CONTRACT
+  const float cmax = fmaxf(fmaxf(fmaxf(fabsf(c[0]), fabsf(c[1])), fmaxf(fabsf(c[2]), fabsf(c[3]))),
+                           fmaxf(fmaxf(fabsf(c[4]), fabsf(c[5])), fmaxf(fabsf(c[6]), fabsf(c[7]))));
+  const float a = (c[0] * (1.f - x) + c[1] * x) * (1.f - y) + (c[2] * (1.f - x) + c[3] * x) * y;
+  const float b = (c[4] * (1.f - x) + c[5] * x) * (1.f - y) + (c[6] * (1.f - x) + c[7] * x) * y;
+  const float phi = a * (1.f - z) + b * z;
   const float lim = f.phiMargin + kRelPhi * cmax;
   if (phi <= lim) return true;
   slack = lipSlack(f, phi, lim, 0.f);
@@ -1235,10 +1305,13 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const int iters = (maxC + 7) >> 3;
   const int j = lane >> 3, k = lane & 7;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // batch bi takes the evaluated pairs bi, bi + NB, bi + 2 NB, ...: pairs near contact come in
+  // spatial runs, and spreading them over the batches keeps the slowest wavefront short
   const int nb = P.narrowBatch > 0 ? P.narrowBatch : kBatch;  // pairs per batch
-  for (int base = wave * nb; base < nAct; base += nWaves * nb) {
+  const int NB = (nAct + nb - 1) / nb;
+  for (int bi = wave; bi < NB; bi += nWaves) {
     // ---- level 1: clusters ----
-    const int a = base + j;  // map record
+    const int a = bi + j * NB;  // map record
     const bool own = j < nb && a < nAct;
     // the 8 lanes of a pair copy its 160-byte map record into LDS, 20 bytes each
     if (own) {
@@ -1309,31 +1382,54 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       NS += __popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- level 3: nodes, 4 sub-entries x 16 per pass ----
+    // ---- level 3: nodes, 4 sub-entries x 16 per pass, two passes per iteration: the node loads,
+    // then the grid reads of the nodes that pass the sphere and box tests, are issued for both
+    // passes before either is used ----
     int total = 0;
-    for (int e0 = 0; e0 < NS; e0 += 4) {
-      const int e = e0 + q;
-      bool cand = false;
-      float smin = kInf;
-      int jj = 0;
-      if (e < NS) {
-        const int2 se = sSub[e];
-        jj = se.y >> 16;
-        if (b16 < (se.y & 0xFFFF)) {
-          float slack;
-          cand = nodeF32(P.sdf, sF[jj], P.nodes32[se.x + b16], slack);
-          smin = cand ? 0.f : fmaxf(slack, 0.f);
+    for (int e0 = 0; e0 < NS; e0 += 8) {
+      int jjv[2];
+      bool live[2], geo[2];
+      float4 nd[2];
+      float gx[2], gy[2], gz[2], sl[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = e0 + 4 * h + q;
+        jjv[h] = 0;
+        live[h] = false;
+        nd[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (e < NS) {
+          const int2 se = sSub[e];
+          jjv[h] = se.y >> 16;
+          live[h] = b16 < (se.y & 0xFFFF);
+          if (live[h]) nd[h] = P.nodes32[se.x + b16];
         }
       }
-      smin = groupMin<16>(smin);  // the 16 lanes of a sub-entry share its pair
-      const unsigned long long m = __ballot(cand);
-      const unsigned int mq = unsigned((m >> (16 * q)) & 0xFFFFull);
-      if (b16 == 0 && e < NS) {
-        sMask[e] = mq;
-        atomicAdd(&sCnt[jj], __popc(mq));  // LDS: 4 groups may share a pair
-        if (smin < kInf) atomicMin(&sSlack[jj], __float_as_uint(smin));
+#pragma unroll
+      for (int h = 0; h < 2; ++h) geo[h] = live[h] && nodeGeomF32(sF[jjv[h]], nd[h], gx[h], gy[h], gz[h], sl[h]);
+      float cc[2][8], fx[2], fy[2], fz[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (geo[h]) sdfCorners(P.sdf, sF[jjv[h]], gx[h], gy[h], gz[h], cc[h], fx[h], fy[h], fz[h]);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = e0 + 4 * h + q;
+        bool cand = false;
+        float smin = kInf;
+        if (live[h]) {
+          float slack = sl[h];
+          if (geo[h]) cand = nodeSdfF32(sF[jjv[h]], cc[h], fx[h], fy[h], fz[h], slack);
+          smin = cand ? 0.f : fmaxf(slack, 0.f);
+        }
+        smin = groupMin<16>(smin);  // the 16 lanes of a sub-entry share its pair
+        const unsigned long long m = __ballot(cand);
+        const unsigned int mq = unsigned((m >> (16 * q)) & 0xFFFFull);
+        if (b16 == 0 && e < NS) {
+          sMask[e] = mq;
+          atomicAdd(&sCnt[jjv[h]], __popc(mq));  // LDS: 4 groups may share a pair
+          if (smin < kInf) atomicMin(&sSlack[jjv[h]], __float_as_uint(smin));
+        }
+        total += __popcll(m);
       }
-      total += __popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
     // ---- one reservation per batch, exact size ----
@@ -1346,7 +1442,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         total = 0;
       }
     }
-    if (lane < nb && base + lane < nAct) {
+    if (lane < nb && bi + lane * NB < nAct) {
       const PairF32& fl = sF[lane];
       int off = first;
       for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
@@ -1355,7 +1451,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       sOff[lane] = off;  // running write offset of each pair
       // separation certificate in world units, net of the motion since the build
       float sg = __uint_as_float(sSlack[lane]);
-      if (SUR) sg = fminf(sg, P.surSlack[base + lane]);
+      if (SUR) sg = fminf(sg, P.surSlack[bi + lane * NB]);
       float sw = -1.f;
       const float sgr = sg * (1.f - 1e-4f) + 2.f * fl.lo;  // lo = -(map rounding bound)
       if (sgr > 0.f && P.useCert) sw = (sgr * fl.gToWorld - fl.certD0) * (1.f - 1e-5f);
@@ -1717,6 +1813,10 @@ __device__ void stepEnd(const Params& P) {
 // otherwise).  Phases are separated by grid barriers; every block runs every phase, so all blocks
 // reach every barrier.  The host sizes the grid so that all its blocks are resident together.
 // ------------------------------------------------------------------------------------------
+// Blocks of the single-launch broad phase: large, so that few blocks meet at each barrier (the
+// cost of a barrier grows with the number of blocks) while many threads work between them.
+constexpr int kBroadThreads = 1024;
+
 // Grid barrier on the counter devInt[12] (cleared by the drift kernel every step): each block
 // adds one arrival per barrier and waits for nBlocks x (barriers passed).  The wait is bounded:
 // after about a second it sets overflow flag 4 and lets the block go on (the step then reports
@@ -1736,11 +1836,11 @@ struct GridBarrier {
       // acquire fence
       for (unsigned int it = 0;
            __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
-        if (it > (1u << 24)) {
+        if (it > (1u << 21)) {
           atomicOr(devInt + 3, 4);
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(20);  // ~1300 cycles: few polls of the shared counter line
       }
       __threadfence();  // the other blocks' writes, after the wait
     }
@@ -1748,9 +1848,9 @@ struct GridBarrier {
   }
 };
 
-__global__ __launch_bounds__(256) void k_broad_phase(Params P) {
+__global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
   if (P.devInt[8] == 0) return;  // written by the drift kernel before this launch: grid-uniform
-  __shared__ int lds[kScanThreads / 64 + 1];
+  __shared__ int lds[kScanLds];
   GridBarrier grid{P.devInt, int(gridDim.x), 0};
   const Span g = gridSpan();
   const int wave = g.tid >> 6, nWaves = g.nth >> 6;
