@@ -14,7 +14,7 @@ Surrogate path: a second context from the same initial state trains the culling 
 GPU (not timed in the step rate; reported), then runs W + K steps with the network culling node
 clusters and an exact audit every 100 steps (audits are inside the timed steps).
 
-Roofline (the narrow gather phase: k_pair_maps + k_candidates + k_exact_contacts, one launch
+Roofline (the narrow gather phase: k_pair_check + k_candidates + k_exact_contacts, one launch
 each per step).  Algorithmic bytes per step, 4 B per FP32/int32 scalar:
     P x 18 x 4   candidate pairs (pair record, both poses, map)
   + Q x 3 x 4    boundary nodes of the masters (a node-by-node scan reads every one)
@@ -46,7 +46,7 @@ B_CONTACT = 16 * 4
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per step of the narrow phase, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a
 # 256 MiB copy (profiles/c2_200k_narrow_pmc_traffic.md); None when the kernels changed since.
-NARROW_TRAFFIC_BYTES = 698.1e6
+NARROW_TRAFFIC_BYTES = 139.0e6
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 
 
@@ -210,7 +210,7 @@ def main():
         "config": {"workload": "200k superquadrics (10 types, SDF 50^3 FP32, 512 nodes), periodic x/y 0.3 m "
                                "+ floor, dt 5e-6; value = plain path", "particles": args.n, "seed": args.seed,
                    "lattice_gap": args.gap},
-        "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_maps + k_candidates + k_exact_contacts)",
+        "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_check + k_candidates + k_exact_contacts)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": NARROW_TRAFFIC_BYTES, "bytes_per_launch": per_step_bytes, "ms_per_launch": narrow_ms,
                      "traffic_source": "profiles/c2_200k_narrow_pmc_traffic.md"},
