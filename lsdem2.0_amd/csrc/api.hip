@@ -234,8 +234,9 @@ struct lsdem_ctx {
   DBuf<ClusterF32> dClusters32, dSubs32;
   DBuf<d3> dXBuild, dXBuildW, dWBuildPos;
   DBuf<q4> dQBuild;
-  DBuf<int> dImgBuild, dImgFlip, dPairRec;
-  DBuf<float4> dMotion, dWMotion, dCert;
+  DBuf<int> dImgBuild, dPairRec;
+  DBuf<BodyStep> dStep;
+  DBuf<float4> dWMotion, dCert;
   DBuf<float> dSurSlack;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
@@ -344,8 +345,8 @@ void lsdem_ctx::releaseAll() {
   dDevInt.release(); dScanBlocks.release(); dDevStat.release();
   dNodeCluster.release();
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
-  dQBuild.release(); dImgBuild.release(); dImgFlip.release(); dPairRec.release();
-  dMotion.release(); dWMotion.release(); dCert.release(); dSurSlack.release();
+  dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
+  dWMotion.release(); dCert.release(); dSurSlack.release();
   sur.release();
 }
 
@@ -860,8 +861,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dXBuildW.alloc(std::max(nP, 1)));
   HIPCHK(c->dQBuild.alloc(std::max(nP, 1)));
   HIPCHK(c->dImgBuild.alloc(std::max(nP, 1)));
-  HIPCHK(c->dImgFlip.alloc(std::max(nP, 1)));
-  HIPCHK(c->dMotion.alloc(std::max(nP, 1)));
+  HIPCHK(c->dStep.alloc(std::max(nP, 1)));
   HIPCHK(c->dScell.alloc(std::max(nP, 1)));
 
   // ---- walls ----
@@ -1035,7 +1035,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
   P.nodeStart = c->dNodeStart.p; P.keyRef = c->dKeyRef.p; P.scell = c->dScell.p;
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
-  P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.imgFlip = c->dImgFlip.p; P.motion = c->dMotion.p;
+  P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
   P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
   P.surSlack = c->dSurSlack.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;

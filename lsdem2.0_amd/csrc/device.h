@@ -49,6 +49,18 @@ struct SearchGrid {  // this library's own broad-phase grid
   int ncell;
 };
 
+// A particle's per-step record for the pair check (one 32-byte read per partner):
+//   motion: displacement of posDet since the build (x, y, z) and the chord 2 sin(angle / 2) of the
+//           rotation since the build (w);
+//   key:    reference-grid key (x; -1 outside the domain) and the keys of ghost images 0..2 (y, z, w;
+//           -1: no image), each with bit kFlipBit set when the image's shift state differs from the
+//           build's.
+struct alignas(16) BodyStep {
+  float4 motion;
+  int4 key;
+};
+constexpr int kFlipBit = 30;
+
 struct Params {
   // ---- sizes and scalars ----
   int nP, nW, nShapes, nBonds;
@@ -101,9 +113,7 @@ struct Params {
   d3* xBuildW;           // pos when the candidate pairs were built
   q4* qBuild;            // quat when the candidate pairs were built
   int* imgBuild;         // image shift states at the build (bit g: image g lies a period away)
-  float4* motion;        // displacement of posDet since the build (x, y, z) and the chord
-                         // 2 sin(angle / 2) of the rotation since the build (w)
-  int* imgFlip;          // bit g: image g's shift state differs from the build's
+  BodyStep* step;        // per particle, written every step: motion since the build and keys
 
   // ---- walls ----
   d3* wpos;

@@ -331,8 +331,18 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
   if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[8], 1);
   // motion since the build for the separation certificates (a step that wraps rebuilds, so posDet
   // and pos move alike on every step that reads this)
-  P.motion[i] = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(P.qBuild[i], P.quat[i])));
-  if (periodic) P.imgFlip[i] = imgShiftMask(P, p) ^ P.imgBuild[i];
+  BodyStep st;
+  st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(P.qBuild[i], P.quat[i])));
+  st.key = make_int4(P.keyRef[i], -1, -1, -1);
+  if (periodic) {
+    const int flip = imgShiftMask(P, p) ^ P.imgBuild[i];
+    int* gk = &st.key.y;
+    for (int g = 0; g < 3; ++g) {
+      const int k = (g == 0 ? P.periodicX : (g == 1 ? P.periodicY : (P.periodicX && P.periodicY))) ? P.gkey[g][i] : -1;
+      gk[g] = k >= 0 ? (k | (((flip >> g) & 1) << kFlipBit)) : -1;
+    }
+  }
+  P.step[i] = st;
 }
 
 __device__ void wallDrift(const Params& P, int w) {
@@ -362,11 +372,13 @@ __device__ void markBuildOne(const Params& P, int i) {
     P.xBuild[i] = P.posDet[i];
     P.xBuildW[i] = P.pos[i];
     P.qBuild[i] = P.quat[i];
-    P.motion[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (P.periodicX || P.periodicY) {
-      P.imgBuild[i] = imgShiftMask(P, P.pos[i]);
-      P.imgFlip[i] = 0;
-    }
+    BodyStep st = P.step[i];
+    st.motion = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.key.y = st.key.y >= 0 ? (st.key.y & ~(1 << kFlipBit)) : -1;
+    st.key.z = st.key.z >= 0 ? (st.key.z & ~(1 << kFlipBit)) : -1;
+    st.key.w = st.key.w >= 0 ? (st.key.w & ~(1 << kFlipBit)) : -1;
+    P.step[i] = st;
+    if (P.periodicX || P.periodicY) P.imgBuild[i] = imgShiftMask(P, P.pos[i]);
   }
   if (i < P.nW) {
     P.wBuildPos[i] = P.wpos[i];
@@ -951,13 +963,14 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
   // g = R_B^T (d + R_A u) / spacing, d = x_A - x_B.  Between the build and now it moved at most
   //   |dA - dB| + chord_B |d_build| + (chord_A + chord_B) rA,   |d_build| <= |d| + |dA - dB|,
   // with dA, dB the displacements and chord the rotation chords since the build.
-  const float4 mA = P.motion[v.A];
+  const float4 mA = P.step[v.A].motion;
   float4 mB;
   bool flip = false;  // an image that changed its shift state has no motion bound
   if (v.type == PT_PW) mB = P.wMotion[v.partner];
   else {
-    mB = P.motion[v.partner];
-    if (v.type >= PT_GX) flip = (P.imgFlip[v.partner] >> (v.type - PT_GX)) & 1;
+    const BodyStep sb = P.step[v.partner];
+    mB = sb.motion;
+    if (v.type >= PT_GX) flip = (((&sb.key.y)[v.type - PT_GX] >> kFlipBit) & 1) != 0;
   }
   const double ex = double(mA.x) - double(mB.x), ey = double(mA.y) - double(mB.y), ez = double(mA.z) - double(mB.z);
   const double del = sqrt(ex * ex + ey * ey + ez * ez);
@@ -1005,22 +1018,26 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         const uint32_t type = info >> 28;
         const int j = int(info & kPartnerMask);
         const int A = P.pairA[p];
-        const bool live = type == PT_PP ? P.keyRef[j] >= 0
-                                        : (type == PT_PW ? P.wkey[j] >= 0 : P.gkey[type - PT_GX][j] >= 0);
+        // the partner's step record: liveness key (and image shift flip) and motion, one 32-byte read
+        int key;
+        float4 mB;
+        if (type == PT_PW) {
+          key = P.wkey[j];
+          mB = P.wMotion[j];
+        } else {
+          const BodyStep sb = P.step[j];
+          key = type == PT_PP ? sb.key.x : (&sb.key.y)[type - PT_GX];
+          mB = sb.motion;
+        }
+        const bool live = key >= 0;
         if (live) {
           masterNodes += (unsigned long long)(P.nodeStart[A + 1] - P.nodeStart[A]);
           act = true;
           if (!fresh) {
             const float4 c = P.cert[p];
             if (c.x > 0.f) {
-              const float4 mA = P.motion[A];
-              float4 mB;
-              bool flip = false;
-              if (type == PT_PW) mB = P.wMotion[j];
-              else {
-                mB = P.motion[j];
-                if (type >= PT_GX) flip = (P.imgFlip[j] >> (type - PT_GX)) & 1;
-              }
+              const float4 mA = P.step[A].motion;
+              const bool flip = type >= PT_GX && ((key >> kFlipBit) & 1);
               if (!flip) {
                 const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
                 const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * c.z) * (1.f + 1e-5f);
@@ -1099,6 +1116,12 @@ __device__ __forceinline__ float phiF32(const float* __restrict__ sdf, const Pai
 // further than that in the partner's frame, the test still rules out every node of the cluster.
 // Slacks are rounded down; a cull that holds by less than the rounding gives a slack <= 0.
 
+// A slack is rounded down by kSlackRound, which covers the FP32 rounding of computing it (a few
+// ulps of the distances involved).  The rounding of the map itself needs no allowance: every
+// threshold already contains it (the margins abs, phiMargin), so a test that fails with a radius
+// grown by the slack fails for every exact point of the grown ball.
+constexpr float kSlackRound = 1.f - 2e-6f;
+
 // Geometric part: bounding spheres and grid box.  `inside` reports whether the centre lies in the
 // grid box (then an SDF bound at the centre can cull the cluster).
 __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF32& cl, float& gx, float& gy, float& gz,
@@ -1113,7 +1136,7 @@ __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF3
     const float d2 = dx * dx + dy * dy + dz * dz;
     const float r = sqrtf(f.r2) + rho;
     if (d2 > r * r * 1.0001f) {
-      slack = sqrtf(d2) * 0.9999f - r;
+      slack = sqrtf(d2) * kSlackRound - r * 1.00005f;
       return false;
     }
   }
@@ -1122,7 +1145,7 @@ __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF3
   const float dz = fmaxf(0.f, fmaxf(f.lo - gz, gz - f.hi[2]));
   const float d2 = dx * dx + dy * dy + dz * dz;
   if (d2 > rho * rho * 1.0001f) {
-    slack = sqrtf(d2) * 0.9999f - rho;
+    slack = sqrtf(d2) * kSlackRound - rho * 1.00005f;
     return false;
   }
   inside = dx == 0.f && dy == 0.f && dz == 0.f;
@@ -1131,7 +1154,8 @@ __device__ __forceinline__ bool clusterGeomF32(const PairF32& f, const ClusterF3
 // Radius growth that a Lipschitz bound phi > lipH * 1.001 * radius + lim still allows.
 __device__ __forceinline__ float lipSlack(const PairF32& f, float phi, float lim, float rho) {
   FP32_CONTRACT
-  return f.lipH > 0.f ? (phi - lim) / (f.lipH * 1.001f) * 0.9999f - rho : 0.f;
+  // hardware reciprocal (1 ulp), covered by kSlackRound
+  return f.lipH > 0.f ? (phi - lim) * __builtin_amdgcn_rcpf(f.lipH * 1.001f) * kSlackRound - rho : 0.f;
 }
 // SDF part with the exact grid: phi(centre) beyond Lipschitz reach of zero culls the cluster.
 __device__ __forceinline__ bool clusterSdfF32(const float* __restrict__ sdf, const PairF32& f, float gx, float gy,
@@ -1180,14 +1204,14 @@ __device__ __forceinline__ bool nodeF32(const float* __restrict__ sdf, const Pai
     const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
     const float d2 = dx * dx + dy * dy + dz * dz;
     if (d2 > f.r2) {
-      slack = sqrtf(d2) * 0.9999f - sqrtf(f.r2);
+      slack = sqrtf(d2) * kSlackRound - sqrtf(f.r2);
       return false;
     }
   }
   const float out = fmaxf(fmaxf(fmaxf(f.lo - gx, gx - f.hi[0]), fmaxf(f.lo - gy, gy - f.hi[1])),
                           fmaxf(f.lo - gz, gz - f.hi[2]));
   if (out > 0.f) {  // outside the grid box on some axis, by `out`
-    slack = out * 0.9999f;
+    slack = out * kSlackRound;
     return false;
   }
   float cmax;
@@ -1208,14 +1232,14 @@ __device__ __forceinline__ bool nodeGeomF32(const PairF32& f, float4 n, float& g
     const float dx = gx - f.c[0], dy = gy - f.c[1], dz = gz - f.c[2];
     const float d2 = dx * dx + dy * dy + dz * dz;
     if (d2 > f.r2) {
-      slack = sqrtf(d2) * 0.9999f - sqrtf(f.r2);
+      slack = sqrtf(d2) * kSlackRound - sqrtf(f.r2);
       return false;
     }
   }
   const float out = fmaxf(fmaxf(fmaxf(f.lo - gx, gx - f.hi[0]), fmaxf(f.lo - gy, gy - f.hi[1])),
                           fmaxf(f.lo - gz, gz - f.hi[2]));
   if (out > 0.f) {
-    slack = out * 0.9999f;
+    slack = out * kSlackRound;
     return false;
   }
   return true;
@@ -1260,11 +1284,23 @@ __host__ __device__ inline int candWaveBytes(int maxClusters) {
 }
 size_t candidatesLdsBytes(int maxClusters) { return size_t(candWaveBytes(maxClusters)) * (kNarrowBlock / 64); }
 
-// Minimum over aligned groups of W lanes (W a power of two <= 64).
+// Value of another lane of the same 16-lane row (every lane active).
+template <int CTRL>
+__device__ __forceinline__ float dppMove(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// Minimum over aligned groups of W lanes (4, 8 or 16).
 template <int W>
 __device__ __forceinline__ float groupMin(float v) {
-#pragma unroll
-  for (int o = 1; o < W; o <<= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  // lane exchanges within 16-lane rows (DPP): quad permutations, half-row mirror, row rotations
+  static_assert(W == 4 || W == 8 || W == 16, "group of 4, 8 or 16 lanes");
+  v = fminf(v, dppMove<0xB1>(v));  // quad_perm [1,0,3,2]
+  v = fminf(v, dppMove<0x4E>(v));  // quad_perm [2,3,0,1]: min of the quad
+  if (W == 8) v = fminf(v, dppMove<0x141>(v));  // row_half_mirror: the other quad of the 8
+  if (W == 16) {
+    v = fminf(v, dppMove<0x124>(v));  // row_ror 4
+    v = fminf(v, dppMove<0x128>(v));  // row_ror 8
+  }
   return v;
 }
 
@@ -1384,8 +1420,28 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     __builtin_amdgcn_wave_barrier();
     // ---- level 3: nodes, 4 sub-entries x 16 per pass, two passes per iteration: the node loads,
     // then the grid reads of the nodes that pass the sphere and box tests, are issued for both
-    // passes before either is used ----
+    // passes before either is used; the next iteration's node loads are issued before this
+    // iteration's tests ----
     int total = 0;
+    int jjn[2];
+    bool liven[2];
+    float4 ndn[2];
+    auto loadNodes = [&](int e0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = e0 + 4 * h + q;
+        jjn[h] = 0;
+        liven[h] = false;
+        ndn[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (e < NS) {
+          const int2 se = sSub[e];
+          jjn[h] = se.y >> 16;
+          liven[h] = b16 < (se.y & 0xFFFF);
+          if (liven[h]) ndn[h] = P.nodes32[se.x + b16];
+        }
+      }
+    };
+    if (NS > 0) loadNodes(0);
     for (int e0 = 0; e0 < NS; e0 += 8) {
       int jjv[2];
       bool live[2], geo[2];
@@ -1393,17 +1449,11 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       float gx[2], gy[2], gz[2], sl[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int e = e0 + 4 * h + q;
-        jjv[h] = 0;
-        live[h] = false;
-        nd[h] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (e < NS) {
-          const int2 se = sSub[e];
-          jjv[h] = se.y >> 16;
-          live[h] = b16 < (se.y & 0xFFFF);
-          if (live[h]) nd[h] = P.nodes32[se.x + b16];
-        }
+        jjv[h] = jjn[h];
+        live[h] = liven[h];
+        nd[h] = ndn[h];
       }
+      if (e0 + 8 < NS) loadNodes(e0 + 8);
 #pragma unroll
       for (int h = 0; h < 2; ++h) geo[h] = live[h] && nodeGeomF32(sF[jjv[h]], nd[h], gx[h], gy[h], gz[h], sl[h]);
       float cc[2][8], fx[2], fy[2], fz[2];
@@ -1453,7 +1503,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       float sg = __uint_as_float(sSlack[lane]);
       if (SUR) sg = fminf(sg, P.surSlack[bi + lane * NB]);
       float sw = -1.f;
-      const float sgr = sg * (1.f - 1e-4f) + 2.f * fl.lo;  // lo = -(map rounding bound)
+      const float sgr = sg * (1.f - 1e-5f) - 1e-6f;  // g units; the map's rounding is in the thresholds
       if (sgr > 0.f && P.useCert) sw = (sgr * fl.gToWorld - fl.certD0) * (1.f - 1e-5f);
       P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, fl.certCA, 0.f);
     }
