@@ -236,7 +236,7 @@ struct lsdem_ctx {
   DBuf<q4> dQBuild;
   DBuf<int> dImgBuild, dPairRec;
   DBuf<BodyStep> dStep;
-  DBuf<float4> dWMotion, dCert;
+  DBuf<float4> dWMotion, dWQrel, dCert, dCertE, dCertQA, dCertQB;
   DBuf<float> dSurSlack;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
@@ -346,6 +346,7 @@ void lsdem_ctx::releaseAll() {
   dNodeCluster.release();
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
+  dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
   dWMotion.release(); dCert.release(); dSurSlack.release();
   sur.release();
 }
@@ -876,6 +877,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dWBuildPos.alloc(std::max(nW, 1)));
   HIPCHK(c->dWBuildQ.alloc(std::max(nW, 1)));
   HIPCHK(c->dWMotion.alloc(std::max(nW, 1)));
+  HIPCHK(c->dWQrel.alloc(std::max(nW, 1)));
 
   // ---- grids ----
   double cellW = 0;
@@ -952,6 +954,9 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dSurMask.alloc(pairCap));
   HIPCHK(c->dSurSlack.alloc(pairCap));
   HIPCHK(c->dCert.alloc(pairCap));
+  HIPCHK(c->dCertE.alloc(pairCap));
+  HIPCHK(c->dCertQA.alloc(pairCap));
+  HIPCHK(c->dCertQB.alloc(pairCap));
   HIPCHK(c->dPairRec.alloc(pairCap));
   HIPCHK(c->dPairMap.alloc(pairCap * 10));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
@@ -1037,6 +1042,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
   P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
+  P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
   P.surSlack = c->dSurSlack.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;

@@ -49,14 +49,16 @@ struct SearchGrid {  // this library's own broad-phase grid
   int ncell;
 };
 
-// A particle's per-step record for the pair check (one 32-byte read per partner):
+// A particle's per-step record for the pair check:
 //   motion: displacement of posDet since the build (x, y, z) and the chord 2 sin(angle / 2) of the
 //           rotation since the build (w);
+//   qrel:   the rotation since the build as a quaternion (conj(q_build) q, FP32);
 //   key:    reference-grid key (x; -1 outside the domain) and the keys of ghost images 0..2 (y, z, w;
 //           -1: no image), each with bit kFlipBit set when the image's shift state differs from the
 //           build's.
 struct alignas(16) BodyStep {
   float4 motion;
+  float4 qrel;
   int4 key;
 };
 constexpr int kFlipBit = 30;
@@ -127,6 +129,7 @@ struct Params {
   d3* wBuildPos;         // wall pose when the candidate pairs were built
   q4* wBuildQ;
   float4* wMotion;       // as motion, for walls
+  float4* wQrel;         // as qrel, for walls
 
   // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
   d3* gpos[3];
@@ -156,7 +159,11 @@ struct Params {
   int* pairCcnt;    // contacts of the pair
   // separation certificates: a pair whose nodes were all culled with slack s (world units, less
   // the motion bound at that step) is skipped while the motion bound since the build stays below s
-  float4* cert;     // pairCap: (s or -1, cB, cA, 0); bound = |dA - dB| + chordB cB + chordA cA
+  float4* cert;     // pairCap: (s_build or -1, cB, cA, s or -1): s_build against the motion since the
+                    // build, s against the motion since the pair's last evaluation (below)
+  float4* certE;    // pairCap: at the last evaluation, dA - dB (x, y, z) and the image's flip state (w)
+  float4* certQA;   // pairCap: at the last evaluation, the master's qrel
+  float4* certQB;   // pairCap: at the last evaluation, the partner's qrel
   int* pairRec;     // pair -> position of its map record this step (devInt[9] records)
   int useCert;
   // reaction CSR (pairs whose partner is a particle that receives the reaction)

@@ -230,6 +230,23 @@ __device__ __forceinline__ double relChord(q4 b, q4 q) {
   return 2.0 * sqrt(vx * vx + vy * vy + vz * vz);
 }
 
+// conj(b) q in FP32: the rotation from b to q as a quaternion.
+__device__ __forceinline__ float4 relQuat(q4 b, q4 q) {
+  const double w = b.q0 * q.q0 + b.q1 * q.q1 + b.q2 * q.q2 + b.q3 * q.q3;
+  const double vx = b.q0 * q.q1 - q.q0 * b.q1 - (b.q2 * q.q3 - b.q3 * q.q2);
+  const double vy = b.q0 * q.q2 - q.q0 * b.q2 - (b.q3 * q.q1 - b.q1 * q.q3);
+  const double vz = b.q0 * q.q3 - q.q0 * b.q3 - (b.q1 * q.q2 - b.q2 * q.q1);
+  return make_float4(float(w), float(vx), float(vy), float(vz));
+}
+// Chord between the rotations of two FP32 quaternions (as relChord), plus an allowance for their
+// FP32 rounding.
+__device__ __forceinline__ float relChordF(float4 b, float4 q) {
+  const float vx = b.x * q.y - q.x * b.y - (b.z * q.w - b.w * q.z);
+  const float vy = b.x * q.z - q.x * b.z - (b.w * q.y - b.y * q.w);
+  const float vz = b.x * q.w - q.x * b.w - (b.y * q.z - b.z * q.y);
+  return 2.f * sqrtf(vx * vx + vy * vy + vz * vz) * (1.f + 1e-5f) + 1e-6f;
+}
+
 // Which ghost images of a particle at (wrapped) position p lie a period away from it: bit 0 the
 // +x image (first reference-grid column), bit 1 the +y image, bit 2 the corner image.
 __device__ __forceinline__ int imgShiftMask(const Params& P, d3 p) {
@@ -332,7 +349,9 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
   // motion since the build for the separation certificates (a step that wraps rebuilds, so posDet
   // and pos move alike on every step that reads this)
   BodyStep st;
-  st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(P.qBuild[i], P.quat[i])));
+  const q4 qb = P.qBuild[i], qn = P.quat[i];
+  st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qn)));
+  st.qrel = relQuat(qb, qn);
   st.key = make_int4(P.keyRef[i], -1, -1, -1);
   if (periodic) {
     const int flip = imgShiftMask(P, p) ^ P.imgBuild[i];
@@ -360,6 +379,7 @@ __device__ void wallDrift(const Params& P, int w) {
   const double disp = len(dp) + R * chord;
   if (disp > P.halfSkin) atomicOr(&P.devInt[8], 1);
   P.wMotion[w] = make_float4(float(dp.x), float(dp.y), float(dp.z), float(chord));
+  P.wQrel[w] = relQuat(P.wBuildQ[w], q);
 }
 
 // Record the poses the candidate pairs were built from (rebuild steps only).
@@ -374,6 +394,7 @@ __device__ void markBuildOne(const Params& P, int i) {
     P.qBuild[i] = P.quat[i];
     BodyStep st = P.step[i];
     st.motion = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.qrel = make_float4(1.f, 0.f, 0.f, 0.f);
     st.key.y = st.key.y >= 0 ? (st.key.y & ~(1 << kFlipBit)) : -1;
     st.key.z = st.key.z >= 0 ? (st.key.z & ~(1 << kFlipBit)) : -1;
     st.key.w = st.key.w >= 0 ? (st.key.w & ~(1 << kFlipBit)) : -1;
@@ -384,6 +405,7 @@ __device__ void markBuildOne(const Params& P, int i) {
     P.wBuildPos[i] = P.wpos[i];
     P.wBuildQ[i] = P.wquat[i];
     P.wMotion[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.wQrel[i] = make_float4(1.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -1018,16 +1040,18 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         const uint32_t type = info >> 28;
         const int j = int(info & kPartnerMask);
         const int A = P.pairA[p];
-        // the partner's step record: liveness key (and image shift flip) and motion, one 32-byte read
+        // the partner's step record: liveness key (and image shift flip), motion and rotation
         int key;
-        float4 mB;
+        float4 mB, qB;
         if (type == PT_PW) {
           key = P.wkey[j];
           mB = P.wMotion[j];
+          qB = P.wQrel[j];
         } else {
           const BodyStep sb = P.step[j];
           key = type == PT_PP ? sb.key.x : (&sb.key.y)[type - PT_GX];
           mB = sb.motion;
+          qB = sb.qrel;
         }
         const bool live = key >= 0;
         if (live) {
@@ -1035,13 +1059,26 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
           act = true;
           if (!fresh) {
             const float4 c = P.cert[p];
-            if (c.x > 0.f) {
-              const float4 mA = P.step[A].motion;
-              const bool flip = type >= PT_GX && ((key >> kFlipBit) & 1);
-              if (!flip) {
-                const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
+            const bool flip = type >= PT_GX && ((key >> kFlipBit) & 1);
+            if (c.x > 0.f || c.w > 0.f) {
+              const BodyStep sa = P.step[A];
+              const float4 mA = sa.motion;
+              const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
+              // against the motion since the build (the slack net of the motion at the evaluation)
+              if (!flip && c.x > 0.f) {
                 const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * c.z) * (1.f + 1e-5f);
                 act = !(bound < c.x);
+              }
+              // against the motion since the evaluation: the same bound with the displacement and the
+              // rotations taken from that step
+              if (act && c.w > 0.f) {
+                const float4 e0 = P.certE[p];
+                if ((e0.w != 0.f) == flip) {
+                  const float fx = ex - e0.x, fy = ey - e0.y, fz = ez - e0.z;
+                  const float chA = relChordF(P.certQA[p], sa.qrel), chB = relChordF(P.certQB[p], qB);
+                  const float bound = (sqrtf(fx * fx + fy * fy + fz * fz) + chB * c.y + chA * c.z) * (1.f + 1e-5f);
+                  act = !(bound < c.w);
+                }
               }
             }
           }
@@ -1066,6 +1103,22 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       makeF32(P, v, f);
       out[a] = f;
       P.pairRec[p] = a;
+      // the motion state of this evaluation, for the pair's certificate against later motion
+      const BodyStep sa = P.step[v.A];
+      float4 mB, qB;
+      float flip = 0.f;
+      if (v.type == PT_PW) {
+        mB = P.wMotion[v.partner];
+        qB = P.wQrel[v.partner];
+      } else {
+        const BodyStep sb = P.step[v.partner];
+        mB = sb.motion;
+        qB = sb.qrel;
+        if (v.type >= PT_GX) flip = float(((&sb.key.y)[v.type - PT_GX] >> kFlipBit) & 1);
+      }
+      P.certE[p] = make_float4(sa.motion.x - mB.x, sa.motion.y - mB.y, sa.motion.z - mB.z, flip);
+      P.certQA[p] = sa.qrel;
+      P.certQB[p] = qB;
     }
     __syncthreads();
   }
@@ -1343,7 +1396,9 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // batch bi takes the evaluated pairs bi, bi + NB, bi + 2 NB, ...: pairs near contact come in
   // spatial runs, and spreading them over the batches keeps the slowest wavefront short
-  const int nb = P.narrowBatch > 0 ? P.narrowBatch : kBatch;  // pairs per batch
+  // pairs per batch: when few pairs are evaluated, small batches spread the slowest pairs (those
+  // near contact, with many surviving clusters) over more wavefronts
+  const int nb = P.narrowBatch > 0 ? P.narrowBatch : (nAct <= nWaves / 2 ? 2 : (nAct <= nWaves ? 4 : kBatch));
   const int NB = (nAct + nb - 1) / nb;
   for (int bi = wave; bi < NB; bi += nWaves) {
     // ---- level 1: clusters ----
@@ -1502,10 +1557,13 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       // separation certificate in world units, net of the motion since the build
       float sg = __uint_as_float(sSlack[lane]);
       if (SUR) sg = fminf(sg, P.surSlack[bi + lane * NB]);
-      float sw = -1.f;
+      float sw = -1.f, s0 = -1.f;
       const float sgr = sg * (1.f - 1e-5f) - 1e-6f;  // g units; the map's rounding is in the thresholds
-      if (sgr > 0.f && P.useCert) sw = (sgr * fl.gToWorld - fl.certD0) * (1.f - 1e-5f);
-      P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, fl.certCA, 0.f);
+      if (sgr > 0.f && P.useCert) {
+        s0 = sgr * fl.gToWorld * (1.f - 1e-5f);
+        sw = (sgr * fl.gToWorld - fl.certD0) * (1.f - 1e-5f);
+      }
+      P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, fl.certCA, s0 > 0.f ? s0 : -1.f);
     }
     __builtin_amdgcn_wave_barrier();
     // ---- write candidate slots; a pair's candidates are contiguous ----
