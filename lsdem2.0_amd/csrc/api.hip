@@ -38,7 +38,6 @@ template <bool SUR> __global__ void k_candidates(Params P);
 __global__ void k_sur_audit(Params P);
 __global__ void k_sur_masks(Params P);
 size_t candidatesLdsBytes(int maxClusters);
-__global__ void k_exact_contacts(Params P);
 __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P, int kick);
 __global__ void k_kick(Params P);
@@ -236,7 +235,7 @@ struct lsdem_ctx {
   DBuf<q4> dQBuild;
   DBuf<int> dImgBuild, dPairRec;
   DBuf<BodyStep> dStep;
-  DBuf<float4> dWMotion, dWQrel, dCert, dCertE, dCertQA, dCertQB;
+  DBuf<float4> dQrel, dWMotion, dWQrel, dCert, dCertE, dCertQA, dCertQB;
   DBuf<float> dSurSlack;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
@@ -346,7 +345,7 @@ void lsdem_ctx::releaseAll() {
   dNodeCluster.release();
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
-  dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
+  dQrel.release(); dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
   dWMotion.release(); dCert.release(); dSurSlack.release();
   sur.release();
 }
@@ -519,14 +518,13 @@ int runStep(lsdem_ctx* c) {
   else
     hipLaunchKernelGGL(k_candidates<false>, dim3(unsigned(c->narrowBlocks)), dim3(256),
                        unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
-  hipLaunchKernelGGL(k_exact_contacts, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
   if (audit) {
     // the step ran the exact path; check every contact against the surrogate's decision
     hipLaunchKernelGGL(k_sur_audit, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P);
     c->sur.audits++;
   }
   phase();
-  // (history: this step's contacts went into this step's table inside k_exact_contacts; the phase
+  // (history: this step's contacts went into this step's table inside k_candidates; the phase
   // stays in the profile layout)
   phase(false);
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
@@ -863,6 +861,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dQBuild.alloc(std::max(nP, 1)));
   HIPCHK(c->dImgBuild.alloc(std::max(nP, 1)));
   HIPCHK(c->dStep.alloc(std::max(nP, 1)));
+  HIPCHK(c->dQrel.alloc(std::max(nP, 1)));
   HIPCHK(c->dScell.alloc(std::max(nP, 1)));
 
   // ---- walls ----
@@ -1042,7 +1041,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
   P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
-  P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
+  P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
   P.surSlack = c->dSurSlack.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;

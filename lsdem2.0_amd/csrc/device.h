@@ -52,13 +52,13 @@ struct SearchGrid {  // this library's own broad-phase grid
 // A particle's per-step record for the pair check:
 //   motion: displacement of posDet since the build (x, y, z) and the chord 2 sin(angle / 2) of the
 //           rotation since the build (w);
-//   qrel:   the rotation since the build as a quaternion (conj(q_build) q, FP32);
 //   key:    reference-grid key (x; -1 outside the domain) and the keys of ghost images 0..2 (y, z, w;
 //           -1: no image), each with bit kFlipBit set when the image's shift state differs from the
 //           build's.
+// (32 bytes: one read per partner.)  The rotation since the build as a quaternion is kept apart
+// (qrel), read only when a certificate is checked against the motion since an evaluation.
 struct alignas(16) BodyStep {
   float4 motion;
-  float4 qrel;
   int4 key;
 };
 constexpr int kFlipBit = 30;
@@ -116,6 +116,7 @@ struct Params {
   q4* qBuild;            // quat when the candidate pairs were built
   int* imgBuild;         // image shift states at the build (bit g: image g lies a period away)
   BodyStep* step;        // per particle, written every step: motion since the build and keys
+  float4* qrel;          // per particle: conj(q_build) q in FP32 (rotation since the build)
 
   // ---- walls ----
   d3* wpos;

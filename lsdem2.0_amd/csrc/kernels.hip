@@ -1,17 +1,17 @@
 // HIP kernels of the LS-DEM timestep for gfx950 (CDNA4, wave64).
 //
 // Per step (launch order, one stream):
-//   k_kick_drift      half-kick, drift, reference-grid key, search cell, periodic wrap + ghost images
-//   counting sort     particles (and ghost images) by search-grid cell, deterministic order
-//   k_pairs_find      FP32 bounding-sphere / grid-box candidate pairs per master particle: count,
-//                     first 16 kept in a per-master slab
-//   k_pairs_emit      pair list in master order (slab copy; masters with more candidates search again)
-//   reaction CSR      pairs grouped by the partner that receives the reaction force
-//   k_candidates      one wavefront per batch of 8 candidate pairs: FP32 cluster cull and node
-//                     prefilter (conservative), survivors written to a contiguous range per pair
-//   k_exact_contacts  one thread per candidate node: exact FP64 node-vs-SDF test; for contacts,
-//                     tangential history + linear spring-dashpot
-//   k_hash_*          history table of this step's contacts for the next step
+//   k_kick_drift      half-kick, drift, reference-grid key, search cell, periodic wrap + ghost images,
+//                     per-body motion record for the rebuild test and the separation certificates
+//   k_broad_phase     on rebuild steps only (else it returns at once): counting sort of particles and
+//                     ghost images by search-grid cell, FP32 bounding-sphere / grid-box candidate pairs,
+//                     pair list in master order and the reaction CSR, phases joined by grid barriers
+//                     (the same phases also run as separate gated launches)
+//   k_pair_check      per candidate pair: partner this step, separation certificate against the motion
+//                     since it was issued; the pairs to evaluate are compacted
+//   k_candidates      one wavefront per batch of evaluated pairs: FP32 cluster cull and node prefilter
+//                     (conservative), survivors written to a contiguous range per pair, then the exact
+//                     FP64 node-vs-SDF test, tangential history and linear spring-dashpot per survivor
 //   k_bonds           parallel bonds
 //   k_reduce_kick     per-body segmented sum of contact and bond contributions + second half-kick
 #include "device.h"
@@ -351,7 +351,7 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
   BodyStep st;
   const q4 qb = P.qBuild[i], qn = P.quat[i];
   st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qn)));
-  st.qrel = relQuat(qb, qn);
+  P.qrel[i] = relQuat(qb, qn);
   st.key = make_int4(P.keyRef[i], -1, -1, -1);
   if (periodic) {
     const int flip = imgShiftMask(P, p) ^ P.imgBuild[i];
@@ -394,7 +394,7 @@ __device__ void markBuildOne(const Params& P, int i) {
     P.qBuild[i] = P.quat[i];
     BodyStep st = P.step[i];
     st.motion = make_float4(0.f, 0.f, 0.f, 0.f);
-    st.qrel = make_float4(1.f, 0.f, 0.f, 0.f);
+    P.qrel[i] = make_float4(1.f, 0.f, 0.f, 0.f);
     st.key.y = st.key.y >= 0 ? (st.key.y & ~(1 << kFlipBit)) : -1;
     st.key.z = st.key.z >= 0 ? (st.key.z & ~(1 << kFlipBit)) : -1;
     st.key.w = st.key.w >= 0 ? (st.key.w & ~(1 << kFlipBit)) : -1;
@@ -1032,66 +1032,79 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
     const int t0 = t * kCheckTile;
     if (threadIdx.x == 0) sCount = 0;
     __syncthreads();
-    for (int r = 0; r < kCheckTile / 256; ++r) {
-      const int p = t0 + r * 256 + int(threadIdx.x);
-      bool act = false;
-      if (p < nPairs) {
-        const uint32_t info = P.pairInfo[p];
-        const uint32_t type = info >> 28;
-        const int j = int(info & kPartnerMask);
-        const int A = P.pairA[p];
-        // the partner's step record: liveness key (and image shift flip), motion and rotation
-        int key;
-        float4 mB, qB;
-        if (type == PT_PW) {
-          key = P.wkey[j];
-          mB = P.wMotion[j];
-          qB = P.wQrel[j];
-        } else {
-          const BodyStep sb = P.step[j];
-          key = type == PT_PP ? sb.key.x : (&sb.key.y)[type - PT_GX];
-          mB = sb.motion;
-          qB = sb.qrel;
+    // the tile's pairs of this thread, with the loads of one kind issued together
+    constexpr int R = kCheckTile / 256;
+    int pv[R], Av[R], jv[R], key[R];
+    uint32_t tv[R];
+    float4 cv[R], mB[R], mA[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      pv[r] = t0 + r * 256 + int(threadIdx.x);
+      const bool in = pv[r] < nPairs;
+      const uint32_t info = in ? P.pairInfo[pv[r]] : (uint32_t(PT_PW) << 28);
+      tv[r] = info >> 28;
+      jv[r] = int(info & kPartnerMask);
+      Av[r] = in ? P.pairA[pv[r]] : 0;
+      cv[r] = (in && !fresh) ? P.cert[pv[r]] : make_float4(-1.f, 0.f, 0.f, -1.f);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // the partner's step record (liveness key with the image's flip, motion): one 32-byte read
+      if (pv[r] >= nPairs) {
+        key[r] = -1;
+        mB[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+      } else if (tv[r] == PT_PW) {
+        key[r] = P.wkey[jv[r]];
+        mB[r] = P.wMotion[jv[r]];
+      } else {
+        const BodyStep sb = P.step[jv[r]];
+        key[r] = tv[r] == PT_PP ? sb.key.x : (&sb.key.y)[tv[r] - PT_GX];
+        mB[r] = sb.motion;
+      }
+      mA[r] = (cv[r].x > 0.f || cv[r].w > 0.f) ? P.step[Av[r]].motion : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    bool actv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int p = pv[r];
+      const bool live = key[r] >= 0;
+      bool act = live;
+      if (live && !fresh && (cv[r].x > 0.f || cv[r].w > 0.f)) {
+        const float4 c = cv[r];
+        const bool flip = tv[r] >= PT_GX && ((key[r] >> kFlipBit) & 1);
+        const float ex = mA[r].x - mB[r].x, ey = mA[r].y - mB[r].y, ez = mA[r].z - mB[r].z;
+        // against the motion since the build (the slack net of the motion at the evaluation)
+        if (!flip && c.x > 0.f) {
+          const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB[r].w * c.y + mA[r].w * c.z) * (1.f + 1e-5f);
+          act = !(bound < c.x);
         }
-        const bool live = key >= 0;
-        if (live) {
-          masterNodes += (unsigned long long)(P.nodeStart[A + 1] - P.nodeStart[A]);
-          act = true;
-          if (!fresh) {
-            const float4 c = P.cert[p];
-            const bool flip = type >= PT_GX && ((key >> kFlipBit) & 1);
-            if (c.x > 0.f || c.w > 0.f) {
-              const BodyStep sa = P.step[A];
-              const float4 mA = sa.motion;
-              const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
-              // against the motion since the build (the slack net of the motion at the evaluation)
-              if (!flip && c.x > 0.f) {
-                const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * c.z) * (1.f + 1e-5f);
-                act = !(bound < c.x);
-              }
-              // against the motion since the evaluation: the same bound with the displacement and the
-              // rotations taken from that step
-              if (act && c.w > 0.f) {
-                const float4 e0 = P.certE[p];
-                if ((e0.w != 0.f) == flip) {
-                  const float fx = ex - e0.x, fy = ey - e0.y, fz = ez - e0.z;
-                  const float chA = relChordF(P.certQA[p], sa.qrel), chB = relChordF(P.certQB[p], qB);
-                  const float bound = (sqrtf(fx * fx + fy * fy + fz * fz) + chB * c.y + chA * c.z) * (1.f + 1e-5f);
-                  act = !(bound < c.w);
-                }
-              }
-            }
+        // against the motion since the evaluation: the same bound with the displacement and the
+        // rotations taken from that step
+        if (act && c.w > 0.f) {
+          const float4 e0 = P.certE[p];
+          if ((e0.w != 0.f) == flip) {
+            const float4 qB = tv[r] == PT_PW ? P.wQrel[jv[r]] : P.qrel[jv[r]];
+            const float fx = ex - e0.x, fy = ey - e0.y, fz = ez - e0.z;
+            const float chA = relChordF(P.certQA[p], P.qrel[Av[r]]), chB = relChordF(P.certQB[p], qB);
+            const float bound = (sqrtf(fx * fx + fy * fy + fz * fz) + chB * c.y + chA * c.z) * (1.f + 1e-5f);
+            act = !(bound < c.w);
           }
-        } else if (rebuilt) {
-          P.cert[p] = make_float4(-1.f, 0.f, 0.f, 0.f);
         }
+      }
+      if (p < nPairs) {
+        if (live) masterNodes += (unsigned long long)(P.nodeStart[Av[r] + 1] - P.nodeStart[Av[r]]);
+        else if (rebuilt) P.cert[p] = make_float4(-1.f, 0.f, 0.f, -1.f);
         if (!act) P.pairCcnt[p] = 0;
       }
-      const unsigned long long m = __ballot(act);
+      actv[r] = act && p < nPairs;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const unsigned long long m = __ballot(actv[r]);
       int first = 0;
       if (lane == 0 && m) first = atomicAdd(&sCount, __popcll(m));
       first = __shfl(first, 0, 64);
-      if (act) sList[first + __popcll(m & ltMask)] = p;
+      if (actv[r]) sList[first + __popcll(m & ltMask)] = pv[r];
     }
     __syncthreads();
     if (threadIdx.x == 0) sBase = sCount ? atomicAdd(&P.devInt[9], sCount) : 0;
@@ -1113,11 +1126,11 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       } else {
         const BodyStep sb = P.step[v.partner];
         mB = sb.motion;
-        qB = sb.qrel;
+        qB = P.qrel[v.partner];
         if (v.type >= PT_GX) flip = float(((&sb.key.y)[v.type - PT_GX] >> kFlipBit) & 1);
       }
       P.certE[p] = make_float4(sa.motion.x - mB.x, sa.motion.y - mB.y, sa.motion.z - mB.z, flip);
-      P.certQA[p] = sa.qrel;
+      P.certQA[p] = P.qrel[v.A];
       P.certQB[p] = qB;
     }
     __syncthreads();
@@ -1325,6 +1338,70 @@ __device__ __forceinline__ bool nodeSdfF32(const PairF32& f, const float (&c)[8]
   return false;
 }
 
+// Exact test and contact law of one candidate slot: the reference's node-vs-SDF membership in
+// FP64, then (for contacts) tangential history and the linear spring-dashpot.
+__device__ void exactSlot(const Params& P, int c, unsigned long long& statQueries, unsigned long long& statContacts) {
+  const int p = P.cPair[c];
+  if (p < 0) {
+    P.cInfo[c] = kInvalidInfo;
+    P.hashSlot[c] = -1;
+    return;
+  }
+  const PairView v = loadPair(P, p);
+  NodeHit h;
+  unsigned long long tests = 0;
+  const bool hit = exactTest(P, v, P.cNode[c], true, h, tests, statQueries);
+  if (!hit) {
+    P.cInfo[c] = kInvalidInfo;
+    P.hashSlot[c] = -1;
+    return;
+  }
+  statContacts++;
+  P.cInfo[c] = v.info;
+  P.cNode[c] = h.gnode;
+  P.cPoint[c] = h.point;
+  P.cNormal[c] = h.normal;
+  P.cOverlap[c] = h.ov;
+  const uint32_t type = v.type;
+  const int j = v.partner;
+  const int A = v.A;
+  const d3 riA = P.pos[A];
+  const d3 vA = P.vel[A], wA = P.angvel[A];
+  const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A];
+  d3 xf, vB, wB;
+  double kn = 0., ks = 0., effM = 0., mu, res;
+  if (type == PT_PW) {
+    xf = P.wpos[j]; vB = P.wvel[j]; wB = P.wangvel[j];
+    kn = knA; ks = ksA;
+    if (invMA > 0.) effM = 1. / invMA;
+    mu = fmin(P.mu[A], P.wmu[j]);
+    res = fmin(P.res[A], P.wres[j]);
+  } else {
+    xf = (type == PT_PP) ? P.pos[j] : P.gpos[type - PT_GX][j];
+    vB = P.vel[j]; wB = P.angvel[j];
+    const double knB = P.kn[j], ksB = P.ks[j], invMB = P.invM[j];
+    if (knA > 0. && knB > 0.) kn = knA * knB / (knA + knB);
+    if (ksA > 0. && ksB > 0.) ks = ksA * ksB / (ksA + ksB);
+    if (invMA > 0. || invMB > 0.) effM = 1. / (invMA + invMB);
+    res = fmin(P.res[A], P.res[j]);
+    mu = fmin(P.mu[A], P.mu[j]);
+  }
+  // tangential history of the same (list, node, partner) from the previous step; this step's
+  // contact goes into this step's table for the next one
+  const unsigned long long key = histKey(type, h.gnode, j);
+  d3 spring = histLookup(P, key);
+  P.hashSlot[c] = histInsert(P, key, c);
+  const d3 vc = vA + cross(wA, h.point - riA) - (vB + cross(wB, h.point - xf));
+  d3 F = mk3(0., 0., 0.);
+  linearContact(F, spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
+  P.cSpring[c] = spring;
+  P.cEn[c] = 0.5 * h.ov * h.ov * kn;
+  P.cEs[c] = 0.5 * len2(spring) * ks;
+  P.cF[c] = F;
+  P.cTA[c] = cross(h.point - riA, F);
+  P.cTB[c] = cross(h.point - xf, -F);
+}
+
 constexpr int kBatch = 8;  // candidate pairs per wave batch
 constexpr int kMaxSub = 4;  // sub-clusters per cluster (64 / 16)
 
@@ -1387,7 +1464,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const int q = lane >> 4, b16 = lane & 15;
   const unsigned int lt16 = (1u << b16) - 1u;
   const float kInf = __builtin_huge_valf();
-  unsigned long long statCand = 0, statEnt = 0;
+  unsigned long long statCand = 0, statEnt = 0, statQueries = 0, statContacts = 0;
   const int nAct = P.devInt[9];
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
@@ -1543,7 +1620,13 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       if (lane == 0) first = atomicAdd(&P.devInt[1], total);
       first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
       if ((long long)first + total > P.contactCap) {
+        // the step fails with a capacity error; slots reserved below the capacity stay empty
         if (lane == 0) atomicOr(&P.devInt[3], 2);
+        for (int c = first + lane; c < (int)P.contactCap; c += 64) {
+          if (c >= first + total) break;
+          P.cInfo[c] = kInvalidInfo;
+          P.hashSlot[c] = -1;
+        }
         total = 0;
       }
     }
@@ -1589,14 +1672,21 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         }
         __builtin_amdgcn_wave_barrier();
       }
+      // the exact test and contact law of this batch's candidates, by the whole wavefront
+      __threadfence_block();
+      for (int c = first + lane; c < first + total; c += 64) exactSlot(P, c, statQueries, statContacts);
     }
     statCand += (lane == 0) ? (unsigned long long)total : 0ull;
   }
   const unsigned long long t = (unsigned long long)wsum(double(statCand));
   const unsigned long long te = (unsigned long long)wsum(double(statEnt));
+  const unsigned long long tq = (unsigned long long)wsum(double(statQueries));
+  const unsigned long long tc = (unsigned long long)wsum(double(statContacts));
   if (lane == 0 && (t | te)) {
     const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
     atomicAdd(&P.statSlots[slot], t);
+    atomicAdd(&P.statSlots[kStatSlots + slot], tq);
+    atomicAdd(&P.statSlots[2 * kStatSlots + slot], tc);
     atomicAdd(&P.statSlots[3 * kStatSlots + slot], te);
   }
 }
@@ -1687,83 +1777,6 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
   if (lane == 0 && (tc | tm)) {
     atomicAdd(&P.devStat[10], tc);
     atomicAdd(&P.devStat[11], tm);
-  }
-}
-
-// Exact test and contact law, one thread per candidate slot: the reference's node-vs-SDF
-// membership in FP64, then (for contacts) tangential history and the linear spring-dashpot.
-__global__ void k_exact_contacts(Params P) {
-  const int n = min(P.devInt[1], (int)P.contactCap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[10 + P.curBuf] = n;  // slots to clear two steps on
-  unsigned long long statQueries = 0, statContacts = 0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const int p = P.cPair[c];
-    if (p < 0) {
-      P.cInfo[c] = kInvalidInfo;
-      P.hashSlot[c] = -1;
-      continue;
-    }
-    const PairView v = loadPair(P, p);
-    NodeHit h;
-    unsigned long long tests = 0;
-    const bool hit = exactTest(P, v, P.cNode[c], true, h, tests, statQueries);
-    if (!hit) {
-      P.cInfo[c] = kInvalidInfo;
-      P.hashSlot[c] = -1;
-      continue;
-    }
-    statContacts++;
-    P.cInfo[c] = v.info;
-    P.cNode[c] = h.gnode;
-    P.cPoint[c] = h.point;
-    P.cNormal[c] = h.normal;
-    P.cOverlap[c] = h.ov;
-    const uint32_t type = v.type;
-    const int j = v.partner;
-    const int A = v.A;
-    const d3 riA = P.pos[A];
-    const d3 vA = P.vel[A], wA = P.angvel[A];
-    const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A];
-    d3 xf, vB, wB;
-    double kn = 0., ks = 0., effM = 0., mu, res;
-    if (type == PT_PW) {
-      xf = P.wpos[j]; vB = P.wvel[j]; wB = P.wangvel[j];
-      kn = knA; ks = ksA;
-      if (invMA > 0.) effM = 1. / invMA;
-      mu = fmin(P.mu[A], P.wmu[j]);
-      res = fmin(P.res[A], P.wres[j]);
-    } else {
-      xf = (type == PT_PP) ? P.pos[j] : P.gpos[type - PT_GX][j];
-      vB = P.vel[j]; wB = P.angvel[j];
-      const double knB = P.kn[j], ksB = P.ks[j], invMB = P.invM[j];
-      if (knA > 0. && knB > 0.) kn = knA * knB / (knA + knB);
-      if (ksA > 0. && ksB > 0.) ks = ksA * ksB / (ksA + ksB);
-      if (invMA > 0. || invMB > 0.) effM = 1. / (invMA + invMB);
-      res = fmin(P.res[A], P.res[j]);
-      mu = fmin(P.mu[A], P.mu[j]);
-    }
-    // tangential history of the same (list, node, partner) from the previous step; this step's
-    // contact goes into this step's table for the next one
-    const unsigned long long key = histKey(type, h.gnode, j);
-    d3 spring = histLookup(P, key);
-    P.hashSlot[c] = histInsert(P, key, c);
-    const d3 vc = vA + cross(wA, h.point - riA) - (vB + cross(wB, h.point - xf));
-    d3 F = mk3(0., 0., 0.);
-    linearContact(F, spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
-    P.cSpring[c] = spring;
-    P.cEn[c] = 0.5 * h.ov * h.ov * kn;
-    P.cEs[c] = 0.5 * len2(spring) * ks;
-    P.cF[c] = F;
-    P.cTA[c] = cross(h.point - riA, F);
-    P.cTB[c] = cross(h.point - xf, -F);
-  }
-  const int lane = threadIdx.x & 63;
-  const unsigned long long q = (unsigned long long)wsum(double(statQueries));
-  const unsigned long long cc = (unsigned long long)wsum(double(statContacts));
-  if (lane == 0 && (q | cc)) {
-    const int slot = (blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1);
-    atomicAdd(&P.statSlots[kStatSlots + slot], q);
-    atomicAdd(&P.statSlots[2 * kStatSlots + slot], cc);
   }
 }
 
@@ -1887,6 +1900,7 @@ __device__ void stepEnd(const Params& P) {
     if ((long long)np > P.pairCap) P.devInt[3] |= 1;
     P.devInt[4] = max(P.devInt[4], np);
     P.devInt[5] = max(P.devInt[5], P.devInt[1]);
+    P.devInt[10 + P.curBuf] = min(P.devInt[1], (int)P.contactCap);  // slots to clear two steps on
     unsigned long long t = 0, q = 0, c = 0, e = 0, mn = 0;
     for (int k = 0; k < kStatSlots; ++k) {
       t += P.statSlots[k];
