@@ -70,7 +70,9 @@ typedef struct lsdem_config {
                               what can be resident together */
   int32_t narrow_batch;   /* candidate pairs per wavefront batch of the narrow phase: 1, 2, 4 or 8
                              (0: chosen each step from the number of pairs it evaluates) */
-  int32_t reserved[4];
+  int32_t no_fused_steps; /* 1: launch each step's kick/drift and its closing sums separately
+                             instead of one launch per step boundary; results do not depend on it */
+  int32_t reserved[3];
 } lsdem_config;
 
 typedef struct lsdem_counters {

@@ -19,6 +19,7 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
 __global__ void k_scan_reduce(const int* in, long long n, int* blockSum, const int* gate);
 __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out, const int* gate);
 __global__ void k_kick_drift(Params P, int forceRebuild);
+__global__ void k_reduce_drift(Params P, StepHead H);
 __global__ void k_broad_phase(Params P);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
@@ -468,7 +469,9 @@ int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
   return LSDEM_OK;
 }
 
-int runStep(lsdem_ctx* c) {
+// One step.  headDone: the previous launch already ran this step's kick and drift.  fuseNext: the
+// step ends with the next step's kick and drift (another step follows with no host work between).
+int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
   Params& P = c->P;
   setCurrentBuffers(c);
   const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
@@ -491,11 +494,13 @@ int runStep(lsdem_ctx* c) {
     ++mark;
   };
   phase();
-  const int force = c->forceRebuild ? 1 : 0;
-  c->forceRebuild = false;
-  const int* gate = P.devInt + 8;  // broad phase only on rebuild steps
-  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, force);
-  phase();
+  const int* gate = P.devInt + rebuildFlagIdx(P.curBuf);  // broad phase only on rebuild steps
+  if (!headDone) {
+    const int force = c->forceRebuild ? 1 : 0;
+    c->forceRebuild = false;
+    hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, force);
+  }
+  phase(!headDone);
   if (c->coopBlocks > 0) {
     // one launch with grid barriers; all of its blocks return at once on steps that keep the pairs
     hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(kBroadThreadsHost), 0, s, P);
@@ -535,6 +540,11 @@ int runStep(lsdem_ctx* c) {
     deviceView(c, &v);
     c->hookFn(c->hookUser, c->time, &v);
     hipLaunchKernelGGL(k_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  } else if (fuseNext) {
+    const int nb = 1 - c->cur;  // the next step's buffers
+    const StepHead H{c->dHashKey[nb].p, c->dHashSlot[nb].p, nb, c->forceRebuild ? 1 : 0};
+    c->forceRebuild = false;
+    hipLaunchKernelGGL(k_reduce_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, H);
   } else {
     hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 1);
   }
@@ -1074,8 +1084,11 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
   if (n < 0) return fail(LSDEM_ERR_ARG, "negative step count");
   HIPCHK(hipSetDevice(c->cfg.device));
   if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
+  // consecutive steps share one launch at each boundary (reduce + next kick/drift) unless an
+  // external-force hook runs between the second half-kick's sums and the kick
+  const bool fuse = !c->hookFn && !c->cfg.no_fused_steps;
   for (int k = 0; k < n; ++k) {
-    int rc = runStep(c);
+    int rc = runStep(c, fuse && k > 0, fuse && k + 1 < n);
     if (rc) return rc;
   }
   HIPCHK(hipGetLastError());

@@ -236,8 +236,22 @@ struct Params {
   float* surSlack;              // per evaluated pair: smallest slack of the clusters it culled
 };
 
+// What opens a step besides the bodies: the table of tangential history this step fills, its slot
+// list from two steps ago, the step's buffer (flag set, clear count) and the host's rebuild request.
+struct StepHead {
+  unsigned long long* hashKey;
+  const int* hashSlot;
+  int buf;
+  int forceRebuild;
+};
+
 constexpr int kStatSlots = 32;
-constexpr int kDevInts = 16;   // devInt entries
+constexpr int kDevInts = 32;   // devInt entries
+// The step's rebuild flag and its count of ghost images left at their source's position, two sets
+// used in alternate steps (buffer = Params::curBuf): the kernel that closes one step may already
+// raise the next step's.
+__host__ __device__ inline int rebuildFlagIdx(int buf) { return 16 + buf; }
+__host__ __device__ inline int ghostCountIdx(int buf) { return 18 + buf; }
 constexpr int kDevStats = 16;  // devStat entries
 constexpr int kPairSlabHost = 16;
 constexpr int kBroadThreadsHost = 1024;  // block of the single-launch broad phase (kBroadThreads)  // slab depth of the broad phase (kPairSlab in the kernels)

@@ -262,32 +262,50 @@ __device__ __forceinline__ void halfKick(const Params& P, int i, d3 F, d3 T) {
   P.angvel[i] += (mul(rotInvInertia(P.quat[i], P.invI[i]), T)) * h;
 }
 
-__device__ void wallDrift(const Params& P, int w);
+__device__ void wallDrift(const Params& P, int w, int buf);
 
-// One thread per particle, then one per wall.  Also raises the step's rebuild flag when the host
-// asks for it, and empties the entries of the history table this step fills (left there two
-// steps ago; their slots were recorded at insertion).
-__global__ void k_kick_drift(Params P, int forceRebuild) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Thread 0 clears the step's counters and raises the host's rebuild request; all threads empty the
+// entries of the history table this step fills (left there two steps ago; their slots were
+// recorded at insertion).
+__device__ __forceinline__ void stepHead(const Params& P, const StepHead& H, int i, int nth) {
   if (i == 0) {
-    if (forceRebuild) atomicOr(&P.devInt[8], 1);
+    if (H.forceRebuild) atomicOr(&P.devInt[rebuildFlagIdx(H.buf)], 1);
     P.devInt[1] = 0;   // contact slots allocated
     P.devInt[9] = 0;   // pairs the narrow phase evaluates
     P.devInt[12] = 0;  // arrivals at the broad phase's grid barriers
   }
-  const int nClear = P.devInt[10 + P.curBuf];
-  for (int c = i; c < nClear; c += gridDim.x * blockDim.x) {
-    const int slot = P.hashSlot[c];
-    if (slot >= 0) P.hashKey[slot] = kHashEmpty;
+  const int nClear = P.devInt[10 + H.buf];
+  for (int c = i; c < nClear; c += nth) {
+    const int slot = H.hashSlot[c];
+    if (slot >= 0) H.hashKey[slot] = kHashEmpty;
   }
-  if (i >= P.nP) {
-    if (i - P.nP < P.nW) wallDrift(P, i - P.nP);
-    return;
+}
+
+// Half-kick(s) with (F, T), drift, reference-grid key, search cell, periodic wrap and ghost images,
+// the rebuild test and the motion record of particle i.  kicks = 2 applies the previous step's
+// second half-kick and this step's first, the same increment twice (the force of a step is used by
+// both), exactly as two separate half-kicks round.
+__device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, int kicks, int buf) {
+  d3 vel = P.vel[i], w = P.angvel[i];
+  const q4 q0 = P.quat[i];
+  const double invM = P.invM[i];
+  if (!isZero(invM)) {
+    const double h = 0.5 * P.dt;
+    const d3 dv = (F * invM + P.g) * h;
+    const d3 dw = (mul(rotInvInertia(q0, P.invI[i]), T)) * h;
+    vel += dv;
+    w += dw;
+    if (kicks == 2) {
+      vel += dv;
+      w += dw;
+    }
+    P.vel[i] = vel;
+    P.angvel[i] = w;
   }
-  halfKick(P, i, P.force[i], P.torque[i]);
   d3 p = P.pos[i];
-  p += P.dt * P.vel[i];
-  P.quat[i] = qintegrate(P.quat[i], P.angvel[i], P.dt);
+  p += P.dt * vel;
+  const q4 qn = qintegrate(q0, w, P.dt);
+  P.quat[i] = qn;
   const bool periodic = P.periodicX || P.periodicY;
   const bool wrapAfter = (P.quirks & 4) != 0;
   const GridGeom& G = P.gridP;
@@ -339,17 +357,17 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
     if (potOutOfDomain(P, p, P.halfSkin))
       for (int s = 0; s < 3; ++s) unshifted += potOutColumn(G, p, s, P.halfSkin);
   }
-  if (unshifted) atomicAdd(&P.devInt[7], unshifted);
+  if (unshifted) atomicAdd(&P.devInt[ghostCountIdx(buf)], unshifted);
   // rebuild the candidate pairs when a body has moved half the skin since the last build (a
   // periodic wrap always does)
   const double hs2 = P.halfSkin * P.halfSkin;
   const d3 dd = P.posDet[i] - P.xBuild[i], dw = p - P.xBuildW[i];
   const bool moved = len2(dd) > hs2 || len2(dw) > hs2;
-  if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[8], 1);
+  if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[rebuildFlagIdx(buf)], 1);
   // motion since the build for the separation certificates (a step that wraps rebuilds, so posDet
   // and pos move alike on every step that reads this)
   BodyStep st;
-  const q4 qb = P.qBuild[i], qn = P.quat[i];
+  const q4 qb = P.qBuild[i];
   st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qn)));
   P.qrel[i] = relQuat(qb, qn);
   st.key = make_int4(P.keyRef[i], -1, -1, -1);
@@ -364,7 +382,18 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
   P.step[i] = st;
 }
 
-__device__ void wallDrift(const Params& P, int w) {
+// One thread per particle, then one per wall.
+__global__ void k_kick_drift(Params P, int forceRebuild) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  stepHead(P, StepHead{P.hashKey, P.hashSlot, P.curBuf, forceRebuild}, i, int(gridDim.x * blockDim.x));
+  if (i >= P.nP) {
+    if (i - P.nP < P.nW) wallDrift(P, i - P.nP, P.curBuf);
+    return;
+  }
+  kickDrift(P, i, P.force[i], P.torque[i], 1, P.curBuf);
+}
+
+__device__ void wallDrift(const Params& P, int w, int buf) {
   d3 p = P.wpos[w];
   p += P.dt * P.wvel[w];
   P.wpos[w] = p;
@@ -377,7 +406,7 @@ __device__ void wallDrift(const Params& P, int w) {
   const double R = P.shapes[P.wshape[w]].boxR;
   const d3 dp = p - P.wBuildPos[w];
   const double disp = len(dp) + R * chord;
-  if (disp > P.halfSkin) atomicOr(&P.devInt[8], 1);
+  if (disp > P.halfSkin) atomicOr(&P.devInt[rebuildFlagIdx(buf)], 1);
   P.wMotion[w] = make_float4(float(dp.x), float(dp.y), float(dp.z), float(chord));
   P.wQrel[w] = relQuat(P.wBuildQ[w], q);
 }
@@ -514,7 +543,7 @@ __device__ void gatherParticle(const Params& P, int k) {
   P.sBoxR[k] = float(P.scale[i] * P.shapes[P.shape[i]].boxR);
 }
 __global__ void k_gather_particles(Params P) {
-  LSD_GATE(P.devInt + 8);
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   const Span g = gridSpan();
   for (int k = g.tid; k < P.nP; k += g.nth) gatherParticle(P, k);
 }
@@ -667,7 +696,7 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
     const GridGeom& G = P.gridP;
     const double hs = P.halfSkin;
     const float lx = float(G.mx.x - G.mn.x), ly = float(G.mx.y - G.mn.y);
-    const bool anyUnshifted = P.devInt[7] > 0;
+    const bool anyUnshifted = P.devInt[ghostCountIdx(P.curBuf)] > 0;
     for (int s = 0; s < 3; ++s) {
       const bool on = (s == 0 && P.periodicX) || (s == 1 && P.periodicY) || (s == 2 && P.periodicX && P.periodicY);
       if (!on) continue;
@@ -709,7 +738,7 @@ __device__ void pairsFind(const Params& P, Span g) {
   }
 }
 __global__ void k_pairs_find(Params P) {
-  LSD_GATE(P.devInt + 8);
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   pairsFind(P, gridSpan());
 }
 // Pair list in master order: copy the slab, or search again for masters with more candidates.
@@ -730,11 +759,11 @@ __device__ void pairsEmit(const Params& P, Span g) {
   }
 }
 __global__ void k_pairs_emit(Params P) {
-  LSD_GATE(P.devInt + 8);
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   pairsEmit(P, gridSpan());
 }
 __global__ void k_mark_build(Params P) {
-  LSD_GATE(P.devInt + 8);
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   markBuild(P, gridSpan());
 }
 
@@ -760,11 +789,11 @@ __device__ void rcScatterPairs(const Params& P, Span g) {
   }
 }
 __global__ void k_rc_count(Params P) {
-  LSD_GATE(P.devInt + 8);
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   rcCountPairs(P, gridSpan());
 }
 __global__ void k_rc_scatter(Params P) {
-  LSD_GATE(P.devInt + 8);
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   if (blockIdx.x == 0 && threadIdx.x == 0) P.devInt[2] = 0;  // large-bucket list of the following sort
   rcScatterPairs(P, gridSpan());
 }
@@ -1018,7 +1047,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
   __shared__ int sCount, sBase;
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
-  const bool rebuilt = P.devInt[8] != 0;  // a new pair list: no certificate exists yet
+  const bool rebuilt = P.devInt[rebuildFlagIdx(P.curBuf)] != 0;  // a new pair list: no certificate exists yet
   const bool fresh = full || rebuilt || !P.useCert;
   const int lane = threadIdx.x & 63;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1843,14 +1872,12 @@ __global__ void k_bonds(Params P) {
 // ------------------------------------------------------------------------------------------
 __device__ void stepEnd(const Params& P);
 
-// One thread per body.  Thread 0 also closes the step's bookkeeping.
-__global__ void k_reduce_kick(Params P, int kick) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b == 0) stepEnd(P);
-  if (b >= P.nP) return;
-  d3 F = mk3(0., 0., 0.), T = mk3(0., 0., 0.);
+// Contact and bond sums on body b in a fixed order: its master pairs contact by contact, then the
+// reactions of the pairs where it is the partner, then its bonds.
+__device__ __forceinline__ void reduceBody(const Params& P, int b, d3& F, d3& T) {
+  F = mk3(0., 0., 0.);
+  T = mk3(0., 0., 0.);
   const long long cap = P.pairCap;
-  // contacts where b is the master, pair by pair, contact by contact
   for (int p = P.pairStart[b]; p < P.pairStart[b + 1] && p < cap; ++p) {
     const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
     for (int c = c0; c < c0 + n; ++c) {
@@ -1859,7 +1886,6 @@ __global__ void k_reduce_kick(Params P, int kick) {
       T += P.cTA[c];
     }
   }
-  // reactions on b as the partner
   for (int q = P.rcStart[b]; q < P.rcStart[b + 1]; ++q) {
     const int p = P.rcPair[q];
     const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
@@ -1877,9 +1903,37 @@ __global__ void k_reduce_kick(Params P, int kick) {
       else { F += P.bF[k]; T += P.bMi[k]; }
     }
   }
+}
+
+// One thread per body.  Thread 0 also closes the step's bookkeeping.
+__global__ void k_reduce_kick(Params P, int kick) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) stepEnd(P);
+  if (b >= P.nP) return;
+  d3 F, T;
+  reduceBody(P, b, F, T);
   P.force[b] = F;
   P.torque[b] = T;
   if (kick) halfKick(P, b, F, T);
+}
+
+// The end of one step and the start of the next in one pass over the bodies: sums, second
+// half-kick, then the next step's first half-kick and drift (one thread per particle, then one per
+// wall).  H describes the next step; P is this step's.  Nothing here reads another body's state,
+// and this step's flags and counters are closed by thread 0 before it clears the next step's.
+__global__ void k_reduce_drift(Params P, StepHead H) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) stepEnd(P);
+  stepHead(P, H, b, int(gridDim.x * blockDim.x));
+  if (b >= P.nP) {
+    if (b - P.nP < P.nW) wallDrift(P, b - P.nP, H.buf);
+    return;
+  }
+  d3 F, T;
+  reduceBody(P, b, F, T);
+  P.force[b] = F;
+  P.torque[b] = T;
+  kickDrift(P, b, F, T, 2, H.buf);
 }
 
 // Second half-kick from the stored sums (after an external-force hook has added to them).
@@ -1911,7 +1965,7 @@ __device__ void stepEnd(const Params& P) {
     }
     P.devStat[12] += mn;
     P.devStat[13] += (unsigned long long)P.devInt[9];
-    P.devStat[14] += P.devInt[8] != 0 ? 1ull : 0ull;
+    P.devStat[14] += P.devInt[rebuildFlagIdx(P.curBuf)] != 0 ? 1ull : 0ull;
     P.devStat[0] = t;
     P.devStat[1] = q;
     P.devStat[2] = c;
@@ -1922,8 +1976,8 @@ __device__ void stepEnd(const Params& P) {
     P.devStat[6] += c;
     P.devStat[8] += e;
   }
-  P.devInt[8] = 0;  // rebuild flag (the drift kernel raises it)
-  P.devInt[7] = 0;  // valid ghost images at their source's own position
+  P.devInt[rebuildFlagIdx(P.curBuf)] = 0;  // rebuild flag (the drift kernel raises it)
+  P.devInt[ghostCountIdx(P.curBuf)] = 0;  // valid ghost images at their source's own position
   for (int k = 0; k < 5 * kStatSlots; ++k) P.statSlots[k] = 0;
   // (the contact slot count [1] and the evaluated-pair count [9] stay readable after the step;
   // the drift kernel clears them)
@@ -1971,7 +2025,7 @@ struct GridBarrier {
 };
 
 __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
-  if (P.devInt[8] == 0) return;  // written by the drift kernel before this launch: grid-uniform
+  if (P.devInt[rebuildFlagIdx(P.curBuf)] == 0) return;  // written by the drift kernel before this launch: grid-uniform
   __shared__ int lds[kScanLds];
   GridBarrier grid{P.devInt, int(gridDim.x), 0};
   const Span g = gridSpan();

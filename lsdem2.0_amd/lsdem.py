@@ -44,7 +44,8 @@ class Config(C.Structure):
         ("broad_phase_launches", C.c_int32),
         ("broad_phase_blocks", C.c_int32),
         ("narrow_batch", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("no_fused_steps", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -175,6 +176,7 @@ class Sim:
         c.broad_phase_launches = int(cfg.get("broad_phase_launches", 0))
         c.broad_phase_blocks = int(cfg.get("broad_phase_blocks", 0))
         c.narrow_batch = int(cfg.get("narrow_batch", 0))
+        c.no_fused_steps = int(cfg.get("no_fused_steps", 0))
         c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()
