@@ -238,6 +238,7 @@ struct lsdem_ctx {
   DBuf<BodyStep> dStep;
   DBuf<float4> dQrel, dWMotion, dWQrel, dCert, dCertE, dCertQA, dCertQB;
   DBuf<float> dSurSlack;
+  DBuf<unsigned char> dBodyHit;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
   DBuf<int> dNodeCluster;
@@ -347,7 +348,7 @@ void lsdem_ctx::releaseAll() {
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
   dQrel.release(); dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
-  dWMotion.release(); dCert.release(); dSurSlack.release();
+  dWMotion.release(); dCert.release(); dSurSlack.release(); dBodyHit.release();
   sur.release();
 }
 
@@ -962,6 +963,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dPairCcnt.alloc(pairCap));
   HIPCHK(c->dSurMask.alloc(pairCap));
   HIPCHK(c->dSurSlack.alloc(pairCap));
+  HIPCHK(c->dBodyHit.alloc(std::max(nP, 1)));
   HIPCHK(c->dCert.alloc(pairCap));
   HIPCHK(c->dCertE.alloc(pairCap));
   HIPCHK(c->dCertQA.alloc(pairCap));
@@ -1033,6 +1035,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, kDevInts * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, kDevStats * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(c->dBodyHit.p, 0, std::max(nP, 1), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
   for (int b = 0; b < 2; ++b)
@@ -1053,6 +1056,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
   P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
   P.surSlack = c->dSurSlack.p;
+  P.bodyHit = c->dBodyHit.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
   for (int g = 0; g < 3; ++g) {

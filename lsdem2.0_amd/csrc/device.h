@@ -234,6 +234,8 @@ struct Params {
   unsigned long long* surMask;  // per evaluated pair (list position): bit c = 0 when the network
                                 // culls cluster c
   float* surSlack;              // per evaluated pair: smallest slack of the clusters it culled
+  unsigned char* bodyHit;       // per particle: 1 when a pair of it holds candidate nodes this step
+                                // (set by the narrow phase, cleared by the sums that read it)
 };
 
 // What opens a step besides the bodies: the table of tangential history this step fills, its slot

@@ -1665,6 +1665,12 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
       P.pairCoff[fl.pair] = off;
       P.pairCcnt[fl.pair] = total ? sCnt[lane] : 0;
+      if (total && sCnt[lane] > 0) {
+        // both bodies have contact terms to sum this step
+        P.bodyHit[P.pairA[fl.pair]] = 1;
+        const uint32_t info = P.pairInfo[fl.pair];
+        if ((info >> 28) != PT_PW) P.bodyHit[info & kPartnerMask] = 1;
+      }
       sOff[lane] = off;  // running write offset of each pair
       // separation certificate in world units, net of the motion since the build
       float sg = __uint_as_float(sSlack[lane]);
@@ -1874,11 +1880,15 @@ __device__ void stepEnd(const Params& P);
 
 // Contact and bond sums on body b in a fixed order: its master pairs contact by contact, then the
 // reactions of the pairs where it is the partner, then its bonds.
+// A body none of whose pairs holds candidate nodes this step has no contact terms (its pairs'
+// contact counts are all 0) and skips the walk over its pairs.
 __device__ __forceinline__ void reduceBody(const Params& P, int b, d3& F, d3& T) {
   F = mk3(0., 0., 0.);
   T = mk3(0., 0., 0.);
+  const bool hit = P.bodyHit[b] != 0;
+  if (hit) P.bodyHit[b] = 0;
   const long long cap = P.pairCap;
-  for (int p = P.pairStart[b]; p < P.pairStart[b + 1] && p < cap; ++p) {
+  for (int p = P.pairStart[b]; hit && p < P.pairStart[b + 1] && p < cap; ++p) {
     const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
     for (int c = c0; c < c0 + n; ++c) {
       if (P.cInfo[c] == kInvalidInfo) continue;
@@ -1886,7 +1896,7 @@ __device__ __forceinline__ void reduceBody(const Params& P, int b, d3& F, d3& T)
       T += P.cTA[c];
     }
   }
-  for (int q = P.rcStart[b]; q < P.rcStart[b + 1]; ++q) {
+  for (int q = P.rcStart[b]; hit && q < P.rcStart[b + 1]; ++q) {
     const int p = P.rcPair[q];
     const int c0 = P.pairCoff[p], n = P.pairCcnt[p];
     for (int c = c0; c < c0 + n; ++c) {
