@@ -14,8 +14,7 @@ Surrogate path: a second context from the same initial state trains the culling 
 GPU (not timed in the step rate; reported), then runs W + K steps with the network culling node
 clusters and an exact audit every 100 steps (audits are inside the timed steps).
 
-Roofline (the narrow gather phase: k_pair_check + k_candidates + k_exact_contacts, one launch
-each per step).  Algorithmic bytes per step, 4 B per FP32/int32 scalar:
+Roofline (the narrow gather phase: k_pair_check + k_candidates, one launch each per step).  Algorithmic bytes per step, 4 B per FP32/int32 scalar:
     P x 18 x 4   candidate pairs (pair record, both poses, map)
   + Q x 3 x 4    boundary nodes of the masters (a node-by-node scan reads every one)
   + S x 8 x 4    SDF queries (8 corners)
@@ -179,6 +178,8 @@ def main():
                "speedup_vs_plain": (args.n * args.steps / swall) / value,
                "narrow_ms_per_step": sprof["ms"]["narrow"] / sK,
                "cluster_tests_per_step": sprof["cluster_tests"] / sK,
+               "evaluated_pairs_per_step": sprof["evaluated_pairs"] / sK,
+               "plain_evaluated_pairs_per_step": prof["evaluated_pairs"] / K,
                "plain_cluster_tests_per_step": prof["cluster_tests"] / K,
                "train_ms": rep["train_ms"], "train_iterations": rep["iterations"], "final_loss": rep["final_loss"],
                "max_error": rep["max_error"], "band": rep["band"], "length_scale": rep["length_scale"],
@@ -210,7 +211,7 @@ def main():
         "config": {"workload": "200k superquadrics (10 types, SDF 50^3 FP32, 512 nodes), periodic x/y 0.3 m "
                                "+ floor, dt 5e-6; value = plain path", "particles": args.n, "seed": args.seed,
                    "lattice_gap": args.gap},
-        "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_check + k_candidates + k_exact_contacts)",
+        "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_check + k_candidates)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": NARROW_TRAFFIC_BYTES, "bytes_per_launch": per_step_bytes, "ms_per_launch": narrow_ms,
                      "traffic_source": "profiles/c2_200k_narrow_pmc_traffic.md"},

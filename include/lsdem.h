@@ -226,7 +226,8 @@ int lsdem_set_time(lsdem_ctx* ctx, double time);
  * surrogate distance exceeds their Lipschitz reach plus the band (band_factor x the maximum
  * validation error) are culled before any node is transformed; everything else takes the exact
  * grid path.  Every audit_every steps the step runs the exact path and checks that each contact
- * lies in a cluster the surrogate keeps (missed_contacts must stay 0). */
+ * lies in a cluster the surrogate keeps (missed_contacts must stay 0); pairs skipped under a
+ * separation certificate of the exact culls alone hold no contact and are not re-evaluated. */
 typedef struct lsdem_surrogate_params {
   int32_t iterations;      /* optimizer steps (3000) */
   int32_t batch;           /* samples per step, a multiple of 64 (4096) */

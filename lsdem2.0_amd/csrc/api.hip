@@ -512,7 +512,8 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
     if (rc) return rc;
   }
   phase(c->coopBlocks == 0);  // reaction lists
-  // narrow phase + contact law (an audit step evaluates every live pair)
+  // narrow phase + contact law (an audit step runs the exact path on every live pair that holds no
+  // certificate of the exact culls)
   const bool useSur = c->sur.enabled && c->sur.trained;
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
   // (a multiple of 8 workgroups: the kernel gives each XCD an eighth of the pairs)

@@ -1037,7 +1037,8 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
 // skipped when its partner does not exist now (pp: outside the domain; wall: its key; ghost: the
 // image) or when its separation certificate still holds: the motion bound of its nodes since the
 // build stays below the slack recorded at its last evaluation (all its nodes were culled then
-// with that much room).  Skipped pairs have no contacts.  full != 0 evaluates every live pair.
+// with that much room).  Skipped pairs have no contacts.  full != 0 (an audit step of the
+// surrogate) also evaluates every live pair whose certificate rests on the surrogate's slack.
 // A block compacts a tile of 1024 pairs in LDS, appends it with one atomic, then writes the FP32
 // map records of its evaluated pairs (FP64 poses in, one 160-byte record out, in evaluation
 // order).
@@ -1046,16 +1047,17 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   __shared__ int sList[kCheckTile];
   PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
   __shared__ int sCount, sBase;
+  if (P.devInt[3] & 4) return;  // a grid barrier of the broad phase timed out: no pair list to read
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
   const bool rebuilt = P.devInt[rebuildFlagIdx(P.curBuf)] != 0;  // a new pair list: no certificate exists yet
-  const bool fresh = full || rebuilt || !P.useCert;
+  const bool fresh = rebuilt || !P.useCert;
   const int lane = threadIdx.x & 63;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
   // XCD-aware tiles: workgroups go to the 8 XCDs in turn (blockIdx mod 8), so each XCD takes one
   // contiguous eighth of the pair list and its partners' data stays in that XCD's L2
   const int nTiles = (nPairs + kCheckTile - 1) / kCheckTile;
-  const int xcd = int(blockIdx.x) & 7, perX = max(1, int(gridDim.x) >> 3);
+  const int xcd = int(blockIdx.x) & 7, perX = (int(gridDim.x) - xcd + 7) >> 3;  // blocks on this XCD
   const int tLo = int((long long)nTiles * xcd / 8), tHi = int((long long)nTiles * (xcd + 1) / 8);
   for (int t = tLo + (int(blockIdx.x) >> 3); t < tHi; t += perX) {
     const int t0 = t * kCheckTile;
@@ -1098,8 +1100,10 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       const int p = pv[r];
       const bool live = key[r] >= 0;
       bool act = live;
-      if (live && !fresh && (cv[r].x > 0.f || cv[r].w > 0.f)) {
-        const float4 c = cv[r];
+      // an audit step of the surrogate (full) trusts only certificates of the exact culls
+      if (live && !fresh && (cv[r].x > 0.f || cv[r].w > 0.f) && !(full && signbit(cv[r].z))) {
+        float4 c = cv[r];
+        c.z = fabsf(c.z);
         const bool flip = tv[r] >= PT_GX && ((key[r] >> kFlipBit) & 1);
         const float ex = mA[r].x - mB[r].x, ey = mA[r].y - mB[r].y, ez = mA[r].z - mB[r].z;
         // against the motion since the build (the slack net of the motion at the evaluation)
@@ -1123,7 +1127,9 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       if (p < nPairs) {
         if (live) masterNodes += (unsigned long long)(P.nodeStart[Av[r] + 1] - P.nodeStart[Av[r]]);
         else if (rebuilt) P.cert[p] = make_float4(-1.f, 0.f, 0.f, -1.f);
-        if (!act) P.pairCcnt[p] = 0;
+        // a pair skipped under a certificate has had no contacts since the evaluation that issued
+        // it (a pair with candidate nodes gets no certificate); others drop last step's count
+        if (!act && !(cv[r].x > 0.f || cv[r].w > 0.f)) P.pairCcnt[p] = 0;
       }
       actv[r] = act && p < nPairs;
     }
@@ -1674,14 +1680,23 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       sOff[lane] = off;  // running write offset of each pair
       // separation certificate in world units, net of the motion since the build
       float sg = __uint_as_float(sSlack[lane]);
-      if (SUR) sg = fminf(sg, P.surSlack[bi + lane * NB]);
+      bool surDep = false;  // the surrogate's slack is the smaller one
+      if (SUR) {
+        const float ss = P.surSlack[bi + lane * NB];
+        if (ss < sg) {
+          sg = ss;
+          surDep = true;
+        }
+      }
       float sw = -1.f, s0 = -1.f;
       const float sgr = sg * (1.f - 1e-5f) - 1e-6f;  // g units; the map's rounding is in the thresholds
       if (sgr > 0.f && P.useCert) {
         s0 = sgr * fl.gToWorld * (1.f - 1e-5f);
         sw = (sgr * fl.gToWorld - fl.certD0) * (1.f - 1e-5f);
       }
-      P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, fl.certCA, s0 > 0.f ? s0 : -1.f);
+      // (the sign of .z marks a certificate that rests on the surrogate's slack)
+      P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, surDep ? -fl.certCA : fl.certCA,
+                                    s0 > 0.f ? s0 : -1.f);
     }
     __builtin_amdgcn_wave_barrier();
     // ---- write candidate slots; a pair's candidates are contiguous ----
