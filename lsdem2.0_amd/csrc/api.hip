@@ -19,6 +19,7 @@ __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long lon
 __global__ void k_scan_reduce(const int* in, long long n, int* blockSum, const int* gate);
 __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out, const int* gate);
 __global__ void k_kick_drift(Params P, int forceRebuild);
+__global__ void k_search_cells(Params P);
 __global__ void k_reduce_drift(Params P, StepHead H);
 __global__ void k_broad_phase(Params P);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
@@ -238,7 +239,7 @@ struct lsdem_ctx {
   DBuf<BodyStep> dStep;
   DBuf<float4> dQrel, dWMotion, dWQrel, dCert, dCertE, dCertQA, dCertQB;
   DBuf<float> dSurSlack;
-  DBuf<unsigned char> dBodyHit;
+  DBuf<unsigned char> dBodyHit, dForceSet;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
   DBuf<int> dNodeCluster;
@@ -268,13 +269,12 @@ struct lsdem_ctx {
   DBuf<q4> dQuat;
   DBuf<s6> dInvI;
   DBuf<double> dInvM, dRadius, dKn, dKs, dMu, dRes, dScale, dInvSp;
-  DBuf<int> dShape, dNodeStart, dKeyRef, dScell;
+  DBuf<int> dShape, dNodeStart, dScell;
   DBuf<d3> dWPos, dWVel, dWAng;
   DBuf<q4> dWQuat;
   DBuf<double> dWMu, dWRes;
   DBuf<int> dWShape, dWKey;
   DBuf<d3> dGPos[3];
-  DBuf<int> dGKey[3];
   DBuf<int> dCellCount, dCellStart, dSorted, dRank;
   DBuf<float4> dSPR, dSPRw;
   DBuf<float> dSBoxR;
@@ -320,11 +320,11 @@ void lsdem_ctx::releaseAll() {
   dPos.release(); dPosDet.release(); dVel.release(); dAng.release(); dForce.release(); dTorque.release();
   dGorig.release(); dQuat.release(); dInvI.release();
   dInvM.release(); dRadius.release(); dKn.release(); dKs.release(); dMu.release(); dRes.release();
-  dScale.release(); dInvSp.release(); dShape.release(); dNodeStart.release(); dKeyRef.release(); dScell.release();
+  dScale.release(); dInvSp.release(); dShape.release(); dNodeStart.release(); dScell.release();
   dWPos.release(); dWVel.release(); dWAng.release(); dWQuat.release(); dWMu.release(); dWRes.release();
   dWShape.release(); dWKey.release();
   for (int s = 0; s < 3; ++s) {
-    dGPos[s].release(); dGKey[s].release();
+    dGPos[s].release();
   }
   dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
   dSPR.release(); dSPRw.release(); dPairSlab.release(); dStatSlots.release();
@@ -348,7 +348,7 @@ void lsdem_ctx::releaseAll() {
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
   dQrel.release(); dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
-  dWMotion.release(); dCert.release(); dSurSlack.release(); dBodyHit.release();
+  dWMotion.release(); dCert.release(); dSurSlack.release(); dBodyHit.release(); dForceSet.release();
   sur.release();
 }
 
@@ -446,6 +446,7 @@ template <class Phase>
 int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
   Params& P = c->P;
   hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_search_cells, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
   int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
@@ -867,7 +868,6 @@ int lsdem_initialize(lsdem_ctx* c) {
   if ((rc = upload(c->dGorig, gorig, s))) return rc;
   if ((rc = upload(c->dShape, c->pShape, s))) return rc;
   if ((rc = upload(c->dNodeStart, nodeStart, s))) return rc;
-  HIPCHK(c->dKeyRef.alloc(std::max(nP, 1)));
   HIPCHK(c->dXBuild.alloc(std::max(nP, 1)));
   HIPCHK(c->dXBuildW.alloc(std::max(nP, 1)));
   HIPCHK(c->dQBuild.alloc(std::max(nP, 1)));
@@ -933,7 +933,6 @@ int lsdem_initialize(lsdem_ctx* c) {
     const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
     const size_t n = on ? size_t(std::max(nP, 1)) : 1;
     HIPCHK(c->dGPos[g].alloc(n));
-    HIPCHK(c->dGKey[g].alloc(n));
   }
 
   // ---- pairs and contacts ----
@@ -965,6 +964,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dSurMask.alloc(pairCap));
   HIPCHK(c->dSurSlack.alloc(pairCap));
   HIPCHK(c->dBodyHit.alloc(std::max(nP, 1)));
+  HIPCHK(c->dForceSet.alloc(std::max(nP, 1)));
   HIPCHK(c->dCert.alloc(pairCap));
   HIPCHK(c->dCertE.alloc(pairCap));
   HIPCHK(c->dCertQA.alloc(pairCap));
@@ -1037,6 +1037,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, kDevStats * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dBodyHit.p, 0, std::max(nP, 1), s));
+  HIPCHK(hipMemsetAsync(c->dForceSet.p, 1, std::max(nP, 1), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
   HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
   for (int b = 0; b < 2; ++b)
@@ -1051,17 +1052,18 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
   P.radius = c->dRadius.p; P.kn = c->dKn.p; P.ks = c->dKs.p; P.mu = c->dMu.p; P.res = c->dRes.p;
   P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
-  P.nodeStart = c->dNodeStart.p; P.keyRef = c->dKeyRef.p; P.scell = c->dScell.p;
+  P.nodeStart = c->dNodeStart.p; P.scell = c->dScell.p;
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
   P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
   P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
   P.surSlack = c->dSurSlack.p;
   P.bodyHit = c->dBodyHit.p;
+  P.forceSet = c->dForceSet.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
   for (int g = 0; g < 3; ++g) {
-    P.gpos[g] = c->dGPos[g].p; P.gkey[g] = c->dGKey[g].p;
+    P.gpos[g] = c->dGPos[g].p;
   }
   P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
   P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
@@ -1144,6 +1146,7 @@ int lsdem_set_particles(lsdem_ctx* c, const double* pos, const double* quat, con
   if (angvel) HIPCHK(hipMemcpy(c->dAng.p, angvel, n * sizeof(d3), hipMemcpyHostToDevice));
   if (force) HIPCHK(hipMemcpy(c->dForce.p, force, n * sizeof(d3), hipMemcpyHostToDevice));
   if (torque) HIPCHK(hipMemcpy(c->dTorque.p, torque, n * sizeof(d3), hipMemcpyHostToDevice));
+  if (force || torque) HIPCHK(hipMemset(c->dForceSet.p, 1, n));  // stored sums no longer known to be zero
   return LSDEM_OK;
 }
 

@@ -62,6 +62,7 @@ struct alignas(16) BodyStep {
   int4 key;
 };
 constexpr int kFlipBit = 30;
+static_assert(sizeof(BodyStep) == 32 && alignof(BodyStep) == 16, "step record: motion then key, 16 B each");
 
 struct Params {
   // ---- sizes and scalars ----
@@ -109,8 +110,7 @@ struct Params {
   const d3* gorig;       // scale * grid origin
   const int* shape;
   const int* nodeStart;  // N+1 prefix of boundary-node counts
-  int* keyRef;           // reference-grid key of posDet (-1 = outside the domain)
-  int* scell;            // search-grid cell of pos (post-wrap)
+  int* scell;            // search-grid cell of pos (post-wrap), rebuild steps only
   d3* xBuild;            // posDet when the candidate pairs were built
   d3* xBuildW;           // pos when the candidate pairs were built
   q4* qBuild;            // quat when the candidate pairs were built
@@ -133,8 +133,8 @@ struct Params {
   float4* wQrel;         // as qrel, for walls
 
   // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
-  d3* gpos[3];
-  int* gkey[3];      // reference-grid key of the image, -1 when the image does not exist
+  d3* gpos[3];       // image positions, written only for images that exist (key >= 0 in the step
+                     // record); the keys live in the step record
 
   // ---- search grid of particles ----
   int* cellCount;   // ncell (scratch)
@@ -236,6 +236,7 @@ struct Params {
   float* surSlack;              // per evaluated pair: smallest slack of the clusters it culled
   unsigned char* bodyHit;       // per particle: 1 when a pair of it holds candidate nodes this step
                                 // (set by the narrow phase, cleared by the sums that read it)
+  unsigned char* forceSet;      // per particle: 0 when its stored force and torque are exactly zero
 };
 
 // What opens a step besides the bodies: the table of tangential history this step fills, its slot

@@ -315,7 +315,8 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
     if (P.periodicY) { if (p.y < G.mn.y) p += oy; else if (p.y >= G.mx.y) p -= oy; }
   }
   P.posDet[i] = p;
-  P.keyRef[i] = refKey(G, p);
+  const int keyRef = refKey(G, p);
+  int gk[3] = {-1, -1, -1};  // image keys; an image's position is stored only when it exists
   int unshifted = 0;  // valid images at the particle's own position (particle outside the domain)
   if (periodic) {
     d3 img;
@@ -324,8 +325,8 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
       img = p;
       if (refCell(G, p).x == 0) img += ox;
       const int k = ghostKey(P, img);
-      P.gpos[0][i] = img;
-      P.gkey[0][i] = k;
+      if (k >= 0) P.gpos[0][i] = img;
+      gk[0] = k;
       unshifted += (k >= 0 && refCell(G, p).x != 0);
     }
     if (P.periodicY) {
@@ -333,8 +334,8 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
       img = p;
       if (refCell(G, p).y == 0) img += oy;
       const int k = ghostKey(P, img);
-      P.gpos[1][i] = img;
-      P.gkey[1][i] = k;
+      if (k >= 0) P.gpos[1][i] = img;
+      gk[1] = k;
       unshifted += (k >= 0 && refCell(G, p).y != 0);
     }
     if (P.periodicX && P.periodicY) {
@@ -342,15 +343,12 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
       const i3 c = refCell(G, p);
       if (c.x == 0 && c.y == 0) img += ox + oy;
       const int k = ghostKey(P, img);
-      P.gpos[2][i] = img;
-      P.gkey[2][i] = k;
+      if (k >= 0) P.gpos[2][i] = img;
+      gk[2] = k;
       unshifted += (k >= 0 && !(c.x == 0 && c.y == 0));
     }
   }
   P.pos[i] = p;
-  // search cell of the post-wrap centre: ghost images are searched through their sources, and a
-  // particle that wrapped this step is outside the domain before the wrap (never a pp partner)
-  P.scell[i] = lin3(searchCell(P.sg, p), P.sg.size);
   // sources that may hold an unshifted image before the next rebuild
   if (periodic && P.halfSkin > 0.0) {
     unshifted = 0;
@@ -370,14 +368,11 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
   const q4 qb = P.qBuild[i];
   st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qn)));
   P.qrel[i] = relQuat(qb, qn);
-  st.key = make_int4(P.keyRef[i], -1, -1, -1);
+  st.key = make_int4(keyRef, -1, -1, -1);
   if (periodic) {
     const int flip = imgShiftMask(P, p) ^ P.imgBuild[i];
-    int* gk = &st.key.y;
-    for (int g = 0; g < 3; ++g) {
-      const int k = (g == 0 ? P.periodicX : (g == 1 ? P.periodicY : (P.periodicX && P.periodicY))) ? P.gkey[g][i] : -1;
-      gk[g] = k >= 0 ? (k | (((flip >> g) & 1) << kFlipBit)) : -1;
-    }
+    int* sk = &st.key.y;
+    for (int g = 0; g < 3; ++g) sk[g] = gk[g] >= 0 ? (gk[g] | (((flip >> g) & 1) << kFlipBit)) : -1;
   }
   P.step[i] = st;
 }
@@ -529,6 +524,13 @@ __global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, 
   bucketSortBig(start, tmp, items, bigList, bigCount, g.tid >> 6, g.nth >> 6);
 }
 
+// Search-grid cell of a particle's post-wrap centre: ghost images are searched through their
+// sources, and a particle that wrapped this step is outside the domain before the wrap (never a pp
+// partner).
+__device__ __forceinline__ int searchCellOf(const Params& P, int i) {
+  return lin3(searchCell(P.sg, P.pos[i]), P.sg.size);
+}
+
 // Gather particle (or ghost-image) data into the sorted order of the search-grid table so
 // that the candidate loops stream contiguous memory.
 __device__ void gatherParticle(const Params& P, int k) {
@@ -537,10 +539,15 @@ __device__ void gatherParticle(const Params& P, int k) {
   const float r = float(P.radius[i]);
   // a pp partner must be inside the domain when the pair is used; one that can get there before
   // the next rebuild (within half the skin) stays a candidate
-  const bool eligible = P.keyRef[i] >= 0 || domainDist2(P.gridP, x) <= P.halfSkin * P.halfSkin;
+  const bool eligible = P.step[i].key.x >= 0 || domainDist2(P.gridP, x) <= P.halfSkin * P.halfSkin;
   P.sPR[k] = make_float4(float(x.x), float(x.y), float(x.z), eligible ? r : -1.f);
   P.sPRw[k] = make_float4(float(w.x), float(w.y), float(w.z), r);
   P.sBoxR[k] = float(P.scale[i] * P.shapes[P.shape[i]].boxR);
+}
+__global__ void k_search_cells(Params P) {
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
+  const Span g = gridSpan();
+  for (int i = g.tid; i < P.nP; i += g.nth) P.scell[i] = searchCellOf(P, i);
 }
 __global__ void k_gather_particles(Params P) {
   LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
@@ -888,8 +895,14 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p) {
     const ShapeDev SB = P.shapes[v.shapeB];
     v.invSpB = 1. / SB.h; v.goB = mk3(SB.ox, SB.oy, SB.oz); v.keyB = P.wkey[j];
   } else {
-    if (v.type == PT_PP) { v.xB = P.posDet[j]; v.keyB = P.keyRef[j]; }
-    else { const int g = int(v.type - PT_GX); v.xB = P.gpos[g][j]; v.keyB = P.gkey[g][j]; }
+    // keys from the partner's step record (an image key carries its shift-flip bit there)
+    const int* keys = reinterpret_cast<const int*>(P.step + j) + 4;  // BodyStep::key as 4 ints
+    const int k = keys[v.type == PT_PP ? 0 : 1 + int(v.type - PT_GX)];
+    if (v.type == PT_PP) { v.xB = P.posDet[j]; v.keyB = k; }
+    else {
+      v.xB = P.gpos[int(v.type - PT_GX)][j];
+      v.keyB = k >= 0 ? (k & ~(1 << kFlipBit)) : -1;
+    }
     v.qB = P.quat[j]; v.rB = P.radius[j]; v.shapeB = P.shape[j]; v.scB = P.scale[j];
     v.invSpB = P.invSp[j]; v.goB = P.gorig[j];
   }
@@ -1896,8 +1909,9 @@ __device__ void stepEnd(const Params& P);
 // Contact and bond sums on body b in a fixed order: its master pairs contact by contact, then the
 // reactions of the pairs where it is the partner, then its bonds.
 // A body none of whose pairs holds candidate nodes this step has no contact terms (its pairs'
-// contact counts are all 0) and skips the walk over its pairs.
-__device__ __forceinline__ void reduceBody(const Params& P, int b, d3& F, d3& T) {
+// contact counts are all 0) and skips the walk over its pairs.  Returns whether any term was
+// summed (false: F and T are exactly zero).
+__device__ __forceinline__ bool reduceBody(const Params& P, int b, d3& F, d3& T) {
   F = mk3(0., 0., 0.);
   T = mk3(0., 0., 0.);
   const bool hit = P.bodyHit[b] != 0;
@@ -1920,14 +1934,17 @@ __device__ __forceinline__ void reduceBody(const Params& P, int b, d3& F, d3& T)
       T += P.cTB[c];
     }
   }
+  bool any = hit;
   if (P.nBonds > 0) {
     for (int q = P.bcStart[b]; q < P.bcStart[b + 1]; ++q) {
       const int it = P.bcItem[q];
       const int k = it >> 1;
       if (it & 1) { F += -P.bF[k]; T += P.bMj[k]; }
       else { F += P.bF[k]; T += P.bMi[k]; }
+      any = true;
     }
   }
+  return any;
 }
 
 // One thread per body.  Thread 0 also closes the step's bookkeeping.
@@ -1939,6 +1956,7 @@ __global__ void k_reduce_kick(Params P, int kick) {
   reduceBody(P, b, F, T);
   P.force[b] = F;
   P.torque[b] = T;
+  P.forceSet[b] = 1;  // an external-force hook may add to the stored sums
   if (kick) halfKick(P, b, F, T);
 }
 
@@ -1955,9 +1973,13 @@ __global__ void k_reduce_drift(Params P, StepHead H) {
     return;
   }
   d3 F, T;
-  reduceBody(P, b, F, T);
-  P.force[b] = F;
-  P.torque[b] = T;
+  const bool any = reduceBody(P, b, F, T);
+  // the stored sums of a body without terms are already zero unless written otherwise since
+  if (any || P.forceSet[b]) {
+    P.force[b] = F;
+    P.torque[b] = T;
+    P.forceSet[b] = any ? 1 : 0;
+  }
   kickDrift(P, b, F, T, 2, H.buf);
 }
 
@@ -2058,7 +2080,10 @@ __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
   const int ncell = P.sg.ncell;
   // counting sort of the particles by search cell
   for (int i = g.tid; i < ncell; i += g.nth) P.cellCount[i] = 0;
-  for (int i = g.tid; i < P.nP; i += g.nth) P.rcCount[i] = 0;
+  for (int i = g.tid; i < P.nP; i += g.nth) {
+    P.rcCount[i] = 0;
+    P.scell[i] = searchCellOf(P, i);
+  }
   if (g.tid == 0) P.devInt[2] = 0;  // large-bucket list
   grid.sync();
   bucketCount(P.scell, P.nP, P.cellCount, P.rank, g);

@@ -66,3 +66,33 @@ def test_fused_moving_wall_and_poses_set_between_calls(L, S):
         _same(fu, se, f"call {i} ({n} steps)")
         pw += len(fu.contacts(1)["master"])
     assert pw > 0
+
+
+def test_fused_after_hook_and_forces_set_between_calls(L, S):
+    # a damping hook on some calls (every step then launches separately and the hook adds to the
+    # stored sums), off on others; forces and torques set from the host between calls
+    import ctypes as C
+    sc = S.scene_c1(n=27, lattice=9.8e-3, jitter=0.2e-3)
+    fu = sc.load(L.Sim(sc.cfg))
+    se = sc.load(L.Sim({**sc.cfg, "no_fused_steps": 1}))
+    f = fu.lib.launchAddGlobalDampingForceTorque
+    f.restype = None
+    f.argtypes = [C.c_void_p] * 4 + [C.c_double, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p]
+
+    def hook(time, v):
+        f(v.force, v.torque, v.velocity, v.angular_velocity, 0.3, v.num, v.grid_dim, v.block_dim, v.stream)
+    rng = np.random.default_rng(4)
+    nonzero = 0
+    for i, n in enumerate(CHUNKS):
+        for s in (fu, se):
+            s.set_external_force(hook if i in (1, 2, 5) else None)
+        if i == 3:
+            force = rng.normal(0.0, 1e-3, (sc.n_particles, 3))
+            torque = rng.normal(0.0, 1e-6, (sc.n_particles, 3))
+            for s in (fu, se):
+                s.set_particles(force=force, torque=torque)
+        fu.step(n)
+        se.step(n)
+        _same(fu, se, f"call {i} ({n} steps)")
+        nonzero += int(np.count_nonzero(fu.get_particles()["force"]))
+    assert nonzero > 0
