@@ -59,7 +59,7 @@ typedef struct lsdem_config {
   int32_t no_sdf_cull;    /* 1: cull node clusters by bounding spheres / grid boxes only (no SDF
                              bound); for measuring what the SDF and surrogate culls save */
   double skin;            /* candidate pairs are kept for reuse within this distance and rebuilt
-                             when a body has moved half of it (< 0: 4 % of the largest bounding
+                             when a body has moved half of it (< 0: 8 % of the largest bounding
                              radius; 0: rebuilt every step).  Contacts do not depend on it. */
   int32_t no_certificates; /* 1: evaluate every candidate pair every step (no separation
                               certificates); contacts do not depend on it */

@@ -898,7 +898,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.rMax = rMax;
   P.boxRMax = boxRMax;
   // skin of the reused candidate pairs (rebuilt when a body has moved half of it)
-  const double skin = c->cfg.skin >= 0.0 ? c->cfg.skin : 0.04 * rMax;
+  const double skin = c->cfg.skin >= 0.0 ? c->cfg.skin : 0.08 * rMax;
   P.halfSkin = 0.5 * skin;
   c->forceRebuild = true;
   {
