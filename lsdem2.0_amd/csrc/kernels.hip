@@ -285,14 +285,37 @@ __device__ __forceinline__ void stepHead(const Params& P, const StepHead& H, int
 // the rebuild test and the motion record of particle i.  kicks = 2 applies the previous step's
 // second half-kick and this step's first, the same increment twice (the force of a step is used by
 // both), exactly as two separate half-kicks round.
-__device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, int kicks, int buf) {
-  d3 vel = P.vel[i], w = P.angvel[i];
-  const q4 q0 = P.quat[i];
-  const double invM = P.invM[i];
+// Everything the kick and drift of particle i read, loaded before any of its stores (the stores
+// could alias later loads as far as the compiler knows, which would serialise the round trips).
+struct BodyIn {
+  d3 vel, w, pos, xBuild, xBuildW;
+  q4 quat, qBuild;
+  s6 invI;
+  double invM;
+  int imgBuild;
+};
+__device__ __forceinline__ BodyIn loadBody(const Params& P, int i) {
+  BodyIn b;
+  b.vel = P.vel[i];
+  b.w = P.angvel[i];
+  b.quat = P.quat[i];
+  b.invM = P.invM[i];
+  b.invI = P.invI[i];
+  b.pos = P.pos[i];
+  b.xBuild = P.xBuild[i];
+  b.xBuildW = P.xBuildW[i];
+  b.qBuild = P.qBuild[i];
+  b.imgBuild = (P.periodicX || P.periodicY) ? P.imgBuild[i] : 0;
+  return b;
+}
+__device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& in, d3 F, d3 T, int kicks, int buf) {
+  d3 vel = in.vel, w = in.w;
+  const q4 q0 = in.quat;
+  const double invM = in.invM;
   if (!isZero(invM)) {
     const double h = 0.5 * P.dt;
     const d3 dv = (F * invM + P.g) * h;
-    const d3 dw = (mul(rotInvInertia(q0, P.invI[i]), T)) * h;
+    const d3 dw = (mul(rotInvInertia(q0, in.invI), T)) * h;
     vel += dv;
     w += dw;
     if (kicks == 2) {
@@ -302,7 +325,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
     P.vel[i] = vel;
     P.angvel[i] = w;
   }
-  d3 p = P.pos[i];
+  d3 p = in.pos;
   p += P.dt * vel;
   const q4 qn = qintegrate(q0, w, P.dt);
   P.quat[i] = qn;
@@ -315,6 +338,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
     if (P.periodicY) { if (p.y < G.mn.y) p += oy; else if (p.y >= G.mx.y) p -= oy; }
   }
   P.posDet[i] = p;
+  const d3 pDet = p;
   const int keyRef = refKey(G, p);
   int gk[3] = {-1, -1, -1};  // image keys; an image's position is stored only when it exists
   int unshifted = 0;  // valid images at the particle's own position (particle outside the domain)
@@ -359,18 +383,18 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
   // rebuild the candidate pairs when a body has moved half the skin since the last build (a
   // periodic wrap always does)
   const double hs2 = P.halfSkin * P.halfSkin;
-  const d3 dd = P.posDet[i] - P.xBuild[i], dw = p - P.xBuildW[i];
+  const d3 dd = pDet - in.xBuild, dw = p - in.xBuildW;
   const bool moved = len2(dd) > hs2 || len2(dw) > hs2;
   if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[rebuildFlagIdx(buf)], 1);
   // motion since the build for the separation certificates (a step that wraps rebuilds, so posDet
   // and pos move alike on every step that reads this)
   BodyStep st;
-  const q4 qb = P.qBuild[i];
+  const q4 qb = in.qBuild;
   st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qn)));
   P.qrel[i] = relQuat(qb, qn);
   st.key = make_int4(keyRef, -1, -1, -1);
   if (periodic) {
-    const int flip = imgShiftMask(P, p) ^ P.imgBuild[i];
+    const int flip = imgShiftMask(P, p) ^ in.imgBuild;
     int* sk = &st.key.y;
     for (int g = 0; g < 3; ++g) sk[g] = gk[g] >= 0 ? (gk[g] | (((flip >> g) & 1) << kFlipBit)) : -1;
   }
@@ -380,12 +404,19 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, d3 F, d3 T, in
 // One thread per particle, then one per wall.
 __global__ void k_kick_drift(Params P, int forceRebuild) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  BodyIn in;
+  d3 F, T;
+  if (i < P.nP) {
+    in = loadBody(P, i);
+    F = P.force[i];
+    T = P.torque[i];
+  }
   stepHead(P, StepHead{P.hashKey, P.hashSlot, P.curBuf, forceRebuild}, i, int(gridDim.x * blockDim.x));
   if (i >= P.nP) {
     if (i - P.nP < P.nW) wallDrift(P, i - P.nP, P.curBuf);
     return;
   }
-  kickDrift(P, i, P.force[i], P.torque[i], 1, P.curBuf);
+  kickDrift(P, i, in, F, T, 1, P.curBuf);
 }
 
 __device__ void wallDrift(const Params& P, int w, int buf) {
@@ -1160,12 +1191,10 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
     for (int k = threadIdx.x; k < sCount; k += 256) {
       const int p = sList[k], a = sBase + k;
       const PairView v = loadPair(P, p);
-      PairF32 f;
-      makeF32(P, v, f);
-      out[a] = f;
-      P.pairRec[p] = a;
       // the motion state of this evaluation, for the pair's certificate against later motion
+      // (loaded before the stores below)
       const BodyStep sa = P.step[v.A];
+      const float4 qA = P.qrel[v.A];
       float4 mB, qB;
       float flip = 0.f;
       if (v.type == PT_PW) {
@@ -1177,8 +1206,12 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         qB = P.qrel[v.partner];
         if (v.type >= PT_GX) flip = float(((&sb.key.y)[v.type - PT_GX] >> kFlipBit) & 1);
       }
+      PairF32 f;
+      makeF32(P, v, f);
+      out[a] = f;
+      P.pairRec[p] = a;
       P.certE[p] = make_float4(sa.motion.x - mB.x, sa.motion.y - mB.y, sa.motion.z - mB.z, flip);
-      P.certQA[p] = P.qrel[v.A];
+      P.certQA[p] = qA;
       P.certQB[p] = qB;
     }
     __syncthreads();
@@ -1405,11 +1438,6 @@ __device__ void exactSlot(const Params& P, int c, unsigned long long& statQuerie
     return;
   }
   statContacts++;
-  P.cInfo[c] = v.info;
-  P.cNode[c] = h.gnode;
-  P.cPoint[c] = h.point;
-  P.cNormal[c] = h.normal;
-  P.cOverlap[c] = h.ov;
   const uint32_t type = v.type;
   const int j = v.partner;
   const int A = v.A;
@@ -1438,6 +1466,12 @@ __device__ void exactSlot(const Params& P, int c, unsigned long long& statQuerie
   // contact goes into this step's table for the next one
   const unsigned long long key = histKey(type, h.gnode, j);
   d3 spring = histLookup(P, key);
+  // (the slot's stores after every load above)
+  P.cInfo[c] = v.info;
+  P.cNode[c] = h.gnode;
+  P.cPoint[c] = h.point;
+  P.cNormal[c] = h.normal;
+  P.cOverlap[c] = h.ov;
   P.hashSlot[c] = histInsert(P, key, c);
   const d3 vc = vA + cross(wA, h.point - riA) - (vB + cross(wB, h.point - xf));
   d3 F = mk3(0., 0., 0.);
@@ -1966,6 +2000,12 @@ __global__ void k_reduce_kick(Params P, int kick) {
 // and this step's flags and counters are closed by thread 0 before it clears the next step's.
 __global__ void k_reduce_drift(Params P, StepHead H) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  BodyIn in;
+  bool set = false;
+  if (b < P.nP) {
+    in = loadBody(P, b);
+    set = P.forceSet[b] != 0;
+  }
   if (b == 0) stepEnd(P);
   stepHead(P, H, b, int(gridDim.x * blockDim.x));
   if (b >= P.nP) {
@@ -1974,13 +2014,13 @@ __global__ void k_reduce_drift(Params P, StepHead H) {
   }
   d3 F, T;
   const bool any = reduceBody(P, b, F, T);
+  kickDrift(P, b, in, F, T, 2, H.buf);
   // the stored sums of a body without terms are already zero unless written otherwise since
-  if (any || P.forceSet[b]) {
+  if (any || set) {
     P.force[b] = F;
     P.torque[b] = T;
     P.forceSet[b] = any ? 1 : 0;
   }
-  kickDrift(P, b, F, T, 2, H.buf);
 }
 
 // Second half-kick from the stored sums (after an external-force hook has added to them).
