@@ -903,13 +903,14 @@ struct PairView {
   int keyB;
 };
 
-__device__ __forceinline__ PairView loadPair(const Params& P, int p) {
+// (info and A as in pairInfo[p], pairA[p], when the caller already holds them)
+__device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t info, int A) {
   PairView v;
   v.pair = p;
-  v.info = P.pairInfo[p];
+  v.info = info;
   v.type = v.info >> 28;
   v.partner = int(v.info & kPartnerMask);
-  v.A = P.pairA[p];
+  v.A = A;
   const bool ghost = v.type >= PT_GX;
   v.xA = ghost ? P.pos[v.A] : P.posDet[v.A];
   v.qA = P.quat[v.A];
@@ -939,6 +940,9 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p) {
   }
   v.valid = 1;
   return v;
+}
+__device__ __forceinline__ PairView loadPair(const Params& P, int p) {
+  return loadPair(P, p, P.pairInfo[p], P.pairA[p]);
 }
 
 constexpr int kContactChunk = 1024;  // candidate slots reserved per atomic
@@ -1088,7 +1092,8 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
 // order).
 constexpr int kCheckTile = 1024;
 __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
-  __shared__ int sList[kCheckTile];
+  __shared__ int sList[kCheckTile], sA[kCheckTile];
+  __shared__ uint32_t sInfo[kCheckTile];
   PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
   __shared__ int sCount, sBase;
   if (P.devInt[3] & 4) return;  // a grid barrier of the broad phase timed out: no pair list to read
@@ -1183,14 +1188,19 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       int first = 0;
       if (lane == 0 && m) first = atomicAdd(&sCount, __popcll(m));
       first = __shfl(first, 0, 64);
-      if (actv[r]) sList[first + __popcll(m & ltMask)] = pv[r];
+      if (actv[r]) {
+        const int slot = first + __popcll(m & ltMask);
+        sList[slot] = pv[r];
+        sInfo[slot] = (tv[r] << 28) | uint32_t(jv[r]);
+        sA[slot] = Av[r];
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) sBase = sCount ? atomicAdd(&P.devInt[9], sCount) : 0;
     __syncthreads();
     for (int k = threadIdx.x; k < sCount; k += 256) {
       const int p = sList[k], a = sBase + k;
-      const PairView v = loadPair(P, p);
+      const PairView v = loadPair(P, p, sInfo[k], sA[k]);
       // the motion state of this evaluation, for the pair's certificate against later motion
       // (loaded before the stores below)
       const BodyStep sa = P.step[v.A];
