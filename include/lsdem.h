@@ -169,8 +169,10 @@ int lsdem_get_counters(lsdem_ctx* ctx, lsdem_counters* out);
  * 0 integrate (half-kick + drift + walls), 1 broad-phase sort, 2 candidate pairs,
  * 3 reaction lists, 4 narrow phase + contact law, 5 history table, 6 bonds,
  * 7 reduction + second half-kick.  With the single-launch broad phase (the default) phases 1-3
- * are one launch, reported under 1; the history table is filled inside phase 4.  Work counters
- * are summed over the profiled steps.
+ * are one launch, reported under 1; the history table is filled inside phase 4.  When consecutive
+ * steps of one call share a launch at the step boundary (the default), the next step's kick and
+ * drift are reported under 7 and phase 0 holds only a call's first step.  Work counters are summed
+ * over the profiled steps.
  * No counterpart in the reference (it has no timers). */
 #define LSDEM_NUM_PHASES 8
 typedef struct lsdem_profile {
