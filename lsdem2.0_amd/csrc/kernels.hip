@@ -1431,17 +1431,18 @@ __device__ __forceinline__ bool nodeSdfF32(const PairF32& f, const float (&c)[8]
 
 // Exact test and contact law of one candidate slot: the reference's node-vs-SDF membership in
 // FP64, then (for contacts) tangential history and the linear spring-dashpot.
-__device__ void exactSlot(const Params& P, int c, unsigned long long& statQueries, unsigned long long& statContacts) {
-  const int p = P.cPair[c];
+// (p, node, pInfo, pA: the slot's pair and node, and that pair's pairInfo and pairA)
+__device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInfo, int pA,
+                          unsigned long long& statQueries, unsigned long long& statContacts) {
   if (p < 0) {
     P.cInfo[c] = kInvalidInfo;
     P.hashSlot[c] = -1;
     return;
   }
-  const PairView v = loadPair(P, p);
+  const PairView v = loadPair(P, p, pInfo, pA);
   NodeHit h;
   unsigned long long tests = 0;
-  const bool hit = exactTest(P, v, P.cNode[c], true, h, tests, statQueries);
+  const bool hit = exactTest(P, v, node, true, h, tests, statQueries);
   if (!hit) {
     P.cInfo[c] = kInvalidInfo;
     P.hashSlot[c] = -1;
@@ -1496,12 +1497,14 @@ __device__ void exactSlot(const Params& P, int c, unsigned long long& statQuerie
 
 constexpr int kBatch = 8;  // candidate pairs per wave batch
 constexpr int kMaxSub = 4;  // sub-clusters per cluster (64 / 16)
+constexpr int kSlotStage = 128;  // candidate slots of a batch handed to the exact test through LDS
 
 // Per-wave LDS of k_candidates: pair maps, surviving (pair, cluster) entries, surviving
 // (pair, sub-cluster) entries with their 16-bit node masks, per-pair counts, offsets and slacks.
 __host__ __device__ inline int candWaveBytes(int maxClusters) {
   const int nEnt = kBatch * maxClusters, nSub = kMaxSub * nEnt;
-  const int b = int(sizeof(PairF32)) * kBatch + 8 * nEnt + 8 * nSub + 4 * nSub + 12 * kBatch + 16;
+  const int b = int(sizeof(PairF32)) * kBatch + 8 * nEnt + 8 * nSub + 4 * nSub + 12 * kBatch + 16 + 8 * kBatch +
+                8 * kSlotStage;
   return (b + 15) & ~15;
 }
 size_t candidatesLdsBytes(int maxClusters) { return size_t(candWaveBytes(maxClusters)) * (kNarrowBlock / 64); }
@@ -1553,6 +1556,9 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   int* sOff = sCnt + kBatch;
   unsigned int* sSlack = reinterpret_cast<unsigned int*>(sOff + kBatch);  // bits of a float >= 0
   int* sBase4 = reinterpret_cast<int*>(sSlack + kBatch);  // write offsets of the 4 sub-entries of a pass
+  uint32_t* sPInfo = reinterpret_cast<uint32_t*>(sBase4 + 4);  // pairInfo and pairA of the batch's pairs
+  int* sPA = reinterpret_cast<int*>(sPInfo + kBatch);
+  int2* sStage = reinterpret_cast<int2*>(sPA + kBatch);  // (batch pair, node) of the first slots
   const int q = lane >> 4, b16 = lane & 15;
   const unsigned int lt16 = (1u << b16) - 1u;
   const float kInf = __builtin_huge_valf();
@@ -1584,6 +1590,13 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     __builtin_amdgcn_wave_barrier();
     const PairF32& f = sF[j];
     const int clusterOff = own ? f.clusterOff : 0, nCl = own ? f.nClusters : 0;
+    // the pair's list entries for the exact test, loaded now and first used at its end
+    uint32_t myInfo = 0;
+    int myA = 0;
+    if (own && k == 0) {
+      myInfo = P.pairInfo[f.pair];
+      myA = P.pairA[f.pair];
+    }
     int E = 0;
     float lmin = kInf;  // smallest slack of this lane's culled clusters
     for (int it = 0; it < iters; ++it) {
@@ -1775,13 +1788,27 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
             const int dst = sBase4[q] + __popc(mq & lt16);
             P.cPair[dst] = sF[se.y >> 16].pair;
             P.cNode[dst] = se.x + b16;  // cluster-ordered node index until the exact test
+            if (dst - first < kSlotStage) sStage[dst - first] = make_int2(se.y >> 16, se.x + b16);
           }
         }
         __builtin_amdgcn_wave_barrier();
       }
       // the exact test and contact law of this batch's candidates, by the whole wavefront
+      if (k == 0 && j < kBatch) {
+        sPInfo[j] = myInfo;
+        sPA[j] = myA;
+      }
       __threadfence_block();
-      for (int c = first + lane; c < first + total; c += 64) exactSlot(P, c, statQueries, statContacts);
+      __builtin_amdgcn_wave_barrier();
+      for (int c = first + lane; c < first + total; c += 64) {
+        if (c - first < kSlotStage) {
+          const int2 st = sStage[c - first];
+          exactSlot(P, c, sF[st.x].pair, st.y, sPInfo[st.x], sPA[st.x], statQueries, statContacts);
+        } else {
+          const int pc = P.cPair[c];
+          exactSlot(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], statQueries, statContacts);
+        }
+      }
     }
     statCand += (lane == 0) ? (unsigned long long)total : 0ull;
   }
