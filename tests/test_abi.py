@@ -58,3 +58,32 @@ def test_mass_properties_matches_oracle(L):
     assert a[0] == b[0]
     for x, y in zip(a[1:], b[1:]):
         assert np.array_equal(x, y)
+
+
+def test_struct_layouts_match_header(L, tmp_path):
+    """The ctypes mirrors of the public structs have the C header's size and field offsets
+    (a small C program built with gcc against include/lsdem.h prints them)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    mirrors = {"lsdem_config": L.Config, "lsdem_counters": L.Counters, "lsdem_profile": L.Profile,
+               "lsdem_surrogate_params": L.SurrogateParams, "lsdem_surrogate_report": L.SurrogateReport,
+               "lsdem_device_view": L.DeviceView}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lsdem.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, f"{cname}.{fname}"
