@@ -45,7 +45,7 @@ B_CONTACT = 16 * 4
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per step of the narrow phase, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a
 # 256 MiB copy (profiles/c2_200k_narrow_pmc_traffic.md); None when the kernels changed since.
-NARROW_TRAFFIC_BYTES = 84.1e6
+NARROW_TRAFFIC_BYTES = 84.3e6
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 
 
