@@ -96,3 +96,15 @@ def test_fused_after_hook_and_forces_set_between_calls(L, S):
         _same(fu, se, f"call {i} ({n} steps)")
         nonzero += int(np.count_nonzero(fu.get_particles()["force"]))
     assert nonzero > 0
+
+
+def test_block_size_not_a_wavefront_multiple(L, S):
+    # the body kernels round the app's block size up to whole wavefronts; results do not change
+    sc = S.scene_c1(n=36, lattice=10.5e-3, jitter=0.2e-3, box=(0.075, 0.075, 0.3))
+    sc.particles["pos"][:, 2] = 5.1e-3
+    a = sc.load(L.Sim({**sc.cfg, "block_size": 100}))
+    b = sc.load(L.Sim(sc.cfg))
+    for i, n in enumerate((1, 20, 40)):
+        a.step(n)
+        b.step(n)
+        _same(a, b, f"call {i} ({n} steps)")
