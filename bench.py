@@ -8,7 +8,8 @@ floor wall, linear spring-dashpot (kn 1e5, kt 0.4 kn, e 0.5, mu 0.5), dt 5e-6, s
 random-lattice start (layer pitch 0.88 x the layer's largest bounding diameter, orientations
 resampled until no two particles overlap).  W warm-up steps, then K timed steps; the host clock
 brackets the timed steps (lsdem_step returns after the stream is drained).  Per-phase GPU times
-come from HIP events recorded on the library's stream inside the timed region.
+come from HIP events recorded on the library's stream inside the timed region, on every 10th step
+(each event record costs GPU time of its own; timing every step would add ~13 us to each).
 
 Surrogate path: a second context from the same initial state trains the culling network on the
 GPU (not timed in the step rate; reported), then runs W + K steps with the network culling node
@@ -47,6 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # 256 MiB copy (profiles/c2_200k_narrow_pmc_traffic.md); None when the kernels changed since.
 NARROW_TRAFFIC_BYTES = 84.3e6
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
+PROFILE_EVERY = 10
 
 
 def log(msg):
@@ -95,7 +97,9 @@ def run(L, sc, caps, warmup, steps, label, surrogate=None):
         c = sim.counters()
         log(f"[bench] {label} warmup {done}/{warmup} ({time.perf_counter() - t:.1f}s) pairs={c['pairs']} "
             f"contacts={c['contacts']} node_tests={c['node_tests']}")
-    sim.profile_begin(steps)
+    # phases timed with HIP events on every PROFILE_EVERY-th step of the timed region (an event
+    # record costs GPU time of its own); work counters cover every step
+    sim.profile_begin((steps + PROFILE_EVERY - 1) // PROFILE_EVERY, every=PROFILE_EVERY)
     t0 = time.perf_counter()
     sim.step(steps)
     t1 = time.perf_counter()
@@ -156,12 +160,13 @@ def main():
     ms_step = 1e3 * wall / args.steps
     value = args.n * args.steps / wall
     K = max(prof["steps"], 1)
-    narrow_ms = prof["ms"]["narrow"] / K
+    KS = max(prof["sampled_steps"], 1)  # steps whose phases were timed
+    narrow_ms = prof["ms"]["narrow"] / KS
     per_step_bytes = narrow_bytes(prof) / K
     achieved = per_step_bytes / (narrow_ms * 1e-3) / 1e9
-    gpu_ms = sum(prof["ms"].values()) / K
+    gpu_ms = sum(prof["ms"].values()) / KS
     log(f"[bench] plain: {args.steps} steps in {wall:.2f}s ({ms_step:.3f} ms/step, GPU phases {gpu_ms:.3f} ms/step)")
-    log("[bench] phases ms/step: " + " ".join(f"{k}={v / K:.4f}" for k, v in prof["ms"].items()))
+    log("[bench] phases ms/step: " + " ".join(f"{k}={v / KS:.4f}" for k, v in prof["ms"].items()))
     log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} master_nodes={prof['master_nodes'] / K:.0f} "
         f"cluster_tests={prof['cluster_tests'] / K:.0f} node_tests={prof['node_tests'] / K:.0f} "
         f"sdf_queries={prof['sdf_queries'] / K:.0f} contacts={prof['contacts'] / K:.0f} "
@@ -173,10 +178,11 @@ def main():
         ssim, swall, sprof, extra = run(L, sc, caps, args.warmup, args.steps, "surrogate",
                                         surrogate={"iterations": args.sur_iters})
         sK = max(sprof["steps"], 1)
+        sKS = max(sprof["sampled_steps"], 1)
         rep = extra["report"]
         sur = {"value": args.n * args.steps / swall, "ms_per_step": 1e3 * swall / args.steps,
                "speedup_vs_plain": (args.n * args.steps / swall) / value,
-               "narrow_ms_per_step": sprof["ms"]["narrow"] / sK,
+               "narrow_ms_per_step": sprof["ms"]["narrow"] / sKS,
                "cluster_tests_per_step": sprof["cluster_tests"] / sK,
                "evaluated_pairs_per_step": sprof["evaluated_pairs"] / sK,
                "plain_evaluated_pairs_per_step": prof["evaluated_pairs"] / K,
@@ -187,7 +193,7 @@ def main():
                "missed_contacts": rep["missed_contacts"]}
         log(f"[bench] surrogate: {sur['ms_per_step']:.3f} ms/step ({sur['speedup_vs_plain']:.2f}x plain), "
             f"audits {rep['audits']} contacts {rep['audited_contacts']} missed {rep['missed_contacts']}")
-        log("[bench] surrogate phases ms/step: " + " ".join(f"{k}={v / sK:.4f}" for k, v in sprof["ms"].items()))
+        log("[bench] surrogate phases ms/step: " + " ".join(f"{k}={v / sKS:.4f}" for k, v in sprof["ms"].items()))
         del ssim
 
     base = None
@@ -217,7 +223,8 @@ def main():
                      "traffic_source": "profiles/c2_200k_narrow_pmc_traffic.md"},
         "cpu_baseline": base,
         "surrogate": sur,
-        "phases_ms_per_step": {k: v / K for k, v in prof["ms"].items()},
+        "phases_ms_per_step": {k: v / KS for k, v in prof["ms"].items()},
+        "phases_sampled_every": PROFILE_EVERY,
         "work_per_step": {k: prof[k] / K for k in ("pairs", "evaluated_pairs", "master_nodes", "cluster_tests",
                                                     "node_tests", "sdf_queries", "contacts", "rebuilds")},
         "node_sdf_queries_per_s": prof["sdf_queries"] / K / (ms_step * 1e-3),

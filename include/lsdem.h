@@ -184,10 +184,16 @@ typedef struct lsdem_profile {
   int64_t evaluated_pairs;     /* pairs the narrow phase evaluated (the rest held a separation
                                   certificate or had no partner), summed */
   int64_t rebuilds;            /* steps that rebuilt the candidate pairs */
-  int64_t reserved[4];
+  int64_t sampled_steps;       /* steps whose phases were timed (ms[] sums over these) */
+  int64_t reserved[3];
 } lsdem_profile;
-/* Start recording; at most max_steps steps are timed (events are preallocated). */
+/* Start recording; at most max_steps steps are timed (events are preallocated).  `steps` and the
+ * work counters cover every step until lsdem_profile_end. */
 int lsdem_profile_begin(lsdem_ctx* ctx, int32_t max_steps);
+/* The same, timing only every `every`-th step (the first one after this call, then every
+ * every-th), at most max_samples of them: the event records of a timed step cost GPU time of their
+ * own, which sampling spreads thin.  No counterpart in the reference. */
+int lsdem_profile_begin_sampled(lsdem_ctx* ctx, int32_t max_samples, int32_t every);
 /* Stop recording, wait for the work and return the totals. */
 int lsdem_profile_end(lsdem_ctx* ctx, lsdem_profile* out);
 

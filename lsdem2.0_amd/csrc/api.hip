@@ -304,6 +304,8 @@ struct lsdem_ctx {
   std::vector<hipEvent_t> events;
   std::vector<int> eventOf;  // per profiled step and phase mark: the event slot recorded for it
   int profMax = 0, profSteps = 0;
+  int profEvery = 1;
+  long long profSeen = 0;  // steps since lsdem_profile_begin
   bool profiling = false;
 
   ~lsdem_ctx() {
@@ -478,7 +480,11 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
   setCurrentBuffers(c);
   const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
   hipStream_t s = c->stream;
-  const bool prof = c->profiling && c->profSteps < c->profMax;
+  bool prof = false;
+  if (c->profiling) {
+    prof = c->profSteps < c->profMax && c->profSeen % c->profEvery == 0;
+    c->profSeen++;
+  }
   hipEvent_t* ev = prof ? &c->events[size_t(c->profSteps) * kPhaseEvents] : nullptr;
   int* evOf = prof ? &c->eventOf[size_t(c->profSteps) * kPhaseEvents] : nullptr;
   int mark = 0;
@@ -1268,9 +1274,12 @@ int lsdem_get_counters(lsdem_ctx* c, lsdem_counters* out) {
   return LSDEM_OK;
 }
 
-int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
+int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) { return lsdem_profile_begin_sampled(c, max_steps, 1); }
+
+int lsdem_profile_begin_sampled(lsdem_ctx* c, int32_t max_steps, int32_t every) {
   if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
   if (max_steps < 0) return fail(LSDEM_ERR_ARG, "negative max_steps");
+  if (every < 1) return fail(LSDEM_ERR_ARG, "sampling interval below 1");
   HIPCHK(hipSetDevice(c->cfg.device));
   const size_t need = size_t(max_steps) * kPhaseEvents;
   while (c->events.size() < need) {
@@ -1285,6 +1294,8 @@ int lsdem_profile_begin(lsdem_ctx* c, int32_t max_steps) {
   HIPCHK(hipMemset(c->dDevStat.p + 12, 0, 3 * sizeof(unsigned long long)));
   c->profMax = max_steps;
   c->profSteps = 0;
+  c->profEvery = every;
+  c->profSeen = 0;
   c->profiling = true;
   return LSDEM_OK;
 }
@@ -1295,7 +1306,8 @@ int lsdem_profile_end(lsdem_ctx* c, lsdem_profile* out) {
   HIPCHK(hipSetDevice(c->cfg.device));
   HIPCHK(hipStreamSynchronize(c->stream));
   std::memset(out, 0, sizeof(*out));
-  out->steps = c->profSteps;
+  out->steps = c->profSeen;
+  out->sampled_steps = c->profSteps;
   for (int st = 0; st < c->profSteps; ++st) {
     hipEvent_t* ev = &c->events[size_t(st) * kPhaseEvents];
     const int* of = &c->eventOf[size_t(st) * kPhaseEvents];

@@ -99,7 +99,8 @@ class Profile(C.Structure):
         ("master_nodes", C.c_int64),
         ("evaluated_pairs", C.c_int64),
         ("rebuilds", C.c_int64),
-        ("reserved", C.c_int64 * 4),
+        ("sampled_steps", C.c_int64),
+        ("reserved", C.c_int64 * 3),
     ]
 
 
@@ -376,13 +377,18 @@ class Sim:
                        _p(out["sigma_max"]), _p(out["tau_max"]))
         return out
 
-    def profile_begin(self, max_steps: int):
-        self._call("profile_begin", self.ctx, C.c_int32(max_steps))
+    def profile_begin(self, max_steps: int, every: int = 1):
+        """Time every `every`-th step's phases, at most max_steps of them; counters cover all steps."""
+        if every == 1:
+            self._call("profile_begin", self.ctx, C.c_int32(max_steps))
+        else:
+            self._call("profile_begin_sampled", self.ctx, C.c_int32(max_steps), C.c_int32(every))
 
     def profile_end(self) -> dict:
         p = Profile()
         self._call("profile_end", self.ctx, C.byref(p))
-        return {"steps": p.steps, "ms": dict(zip(PHASES, list(p.ms))), "pairs": p.pairs, "node_tests": p.node_tests,
+        return {"steps": p.steps, "sampled_steps": p.sampled_steps, "ms": dict(zip(PHASES, list(p.ms))),
+                "pairs": p.pairs, "node_tests": p.node_tests,
                 "sdf_queries": p.sdf_queries, "contacts": p.contacts, "cluster_tests": p.cluster_tests,
                 "master_nodes": p.master_nodes, "evaluated_pairs": p.evaluated_pairs, "rebuilds": p.rebuilds}
 

@@ -108,3 +108,18 @@ def test_block_size_not_a_wavefront_multiple(L, S):
         a.step(n)
         b.step(n)
         _same(a, b, f"call {i} ({n} steps)")
+
+
+def test_sampled_profile(L, S):
+    # phases timed on every third step; counters over all of them
+    sc = S.scene_c1(n=27, lattice=9.8e-3, jitter=0.2e-3)
+    sim = sc.load(L.Sim(sc.cfg))
+    sim.profile_begin(10, every=3)
+    sim.step(30)
+    p = sim.profile_end()
+    assert p["steps"] == 30 and p["sampled_steps"] == 10
+    assert all(v >= 0.0 for v in p["ms"].values()) and sum(p["ms"].values()) > 0.0
+    sim.profile_begin(40)
+    sim.step(30)
+    q = sim.profile_end()
+    assert q["steps"] == 30 and q["sampled_steps"] == 30
