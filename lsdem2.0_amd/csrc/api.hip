@@ -32,6 +32,8 @@ __global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, 
 __global__ void k_mark_build(Params P);
 __global__ void k_pairs_find(Params P);
 __global__ void k_pairs_emit(Params P);
+__global__ void k_backup_pairs(Params P);
+__global__ void k_inherit_certs(Params P);
 __global__ void k_rc_count(Params P);
 __global__ void k_rc_scatter(Params P);
 __global__ void k_gather_particles(Params P);
@@ -281,7 +283,9 @@ struct lsdem_ctx {
   DBuf<uint32_t> dPairSlab;
   DBuf<int> dPairCount, dPairStart, dPairA, dPairCoff, dPairCcnt, dRcCount, dRcStart, dRcPair, dRcRank, dRcTmp;
   DBuf<int> dSortTmp, dSortBig;
-  DBuf<uint32_t> dPairInfo;
+  DBuf<uint32_t> dPairInfo, dOldInfo;
+  DBuf<int> dOldStart;
+  DBuf<float4> dOldCert;
   DBuf<int> dCNode[2], dCPair;
   DBuf<d3> dCF, dCTA, dCTB;
   DBuf<uint32_t> dCInfo[2];
@@ -334,7 +338,7 @@ void lsdem_ctx::releaseAll() {
   dPairCount.release(); dPairStart.release(); dPairA.release(); dPairCoff.release(); dPairCcnt.release();
   dRcCount.release(); dRcStart.release(); dRcPair.release(); dRcRank.release(); dRcTmp.release();
   dSortTmp.release(); dSortBig.release();
-  dPairInfo.release();
+  dPairInfo.release(); dOldInfo.release(); dOldStart.release(); dOldCert.release();
   for (int b = 0; b < 2; ++b) {
     dCNode[b].release(); dCInfo[b].release(); dCSpring[b].release(); dCPoint[b].release();
     dCNormal[b].release(); dCOverlap[b].release(); dCEn[b].release(); dCEs[b].release();
@@ -454,9 +458,11 @@ int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
   hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
   phase();
   hipLaunchKernelGGL(k_pairs_find, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  hipLaunchKernelGGL(k_backup_pairs, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
   rc = scan(c, P.pairCount, P.nP, P.pairStart, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_pairs_emit, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
+  hipLaunchKernelGGL(k_inherit_certs, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
   hipLaunchKernelGGL(k_mark_build, dim3(blocksFor(std::max(P.nP, P.nW), 256)), dim3(256), 0, s, P);
   phase();
   // reaction CSR
@@ -972,6 +978,9 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dBodyHit.alloc(std::max(nP, 1)));
   HIPCHK(c->dForceSet.alloc(std::max(nP, 1)));
   HIPCHK(c->dCert.alloc(pairCap));
+  HIPCHK(c->dOldInfo.alloc(pairCap));
+  HIPCHK(c->dOldCert.alloc(pairCap));
+  HIPCHK(c->dOldStart.alloc(nP + 1));
   HIPCHK(c->dCertE.alloc(pairCap));
   HIPCHK(c->dCertQA.alloc(pairCap));
   HIPCHK(c->dCertQB.alloc(pairCap));
@@ -1061,7 +1070,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   P.nodeStart = c->dNodeStart.p; P.scell = c->dScell.p;
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
-  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.pairRec = c->dPairRec.p;
+  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.oldInfo = c->dOldInfo.p; P.oldCert = c->dOldCert.p; P.oldStart = c->dOldStart.p; P.pairRec = c->dPairRec.p;
   P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
   P.surSlack = c->dSurSlack.p;
   P.bodyHit = c->dBodyHit.p;

@@ -160,6 +160,9 @@ struct Params {
   int* pairCcnt;    // contacts of the pair
   // separation certificates: a pair whose nodes were all culled with slack s (world units, less
   // the motion bound at that step) is skipped while the motion bound since the build stays below s
+  int* oldStart;           // the previous pair list kept through a rebuild: per-master starts (nP + 1),
+  uint32_t* oldInfo;       // pair entries and certificates, for the certificates the new list inherits
+  float4* oldCert;
   float4* cert;     // pairCap: (s_build or -1, cB, cA, s or -1): s_build against the motion since the
                     // build, s against the motion since the pair's last evaluation (below)
   float4* certE;    // pairCap: at the last evaluation, dA - dB (x, y, z) and the image's flip state (w)

@@ -796,6 +796,58 @@ __device__ void pairsEmit(const Params& P, Span g) {
     }
   }
 }
+// Keep the pair list being replaced (per-master starts, entries, certificates) for the
+// certificates the new list inherits; runs before the new starts and entries are written.
+__device__ void backupPairs(const Params& P, Span g) {
+  const int n = min(P.pairStart[P.nP], (int)P.pairCap);
+  for (int i = g.tid; i <= P.nP; i += g.nth) P.oldStart[i] = P.pairStart[i];
+  for (int p = g.tid; p < n; p += g.nth) {
+    P.oldInfo[p] = P.pairInfo[p];
+    P.oldCert[p] = P.cert[p];
+  }
+}
+__global__ void k_backup_pairs(Params P) {
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
+  backupPairs(P, gridSpan());
+}
+// A particle-particle pair of the new list that was in the previous one keeps its certificate,
+// rebased to this build: the room its fast check leaves now (bound from the old motion records,
+// read before they restart), with the rotation coefficients of this build's poses (|d| + r_A and
+// r_A).  A node's motion since the old evaluation is at most the old bound now plus the new bound
+// later, so the new bound staying below that room keeps the pair separated.  Other pairs start
+// without a certificate.
+__device__ void inheritCerts(const Params& P, Span g) {
+  const int n = min(P.pairStart[P.nP], (int)P.pairCap);
+  for (int p = g.tid; p < n; p += g.nth) {
+    float4 nc = make_float4(-1.f, 0.f, 0.f, -1.f);
+    const uint32_t info = P.pairInfo[p];
+    if (P.useCert && (info >> 28) == PT_PP) {
+      const int A = P.pairA[p], j = int(info & kPartnerMask);
+      const int q1 = min(P.oldStart[A + 1], (int)P.pairCap);
+      for (int q = P.oldStart[A]; q < q1; ++q) {
+        if (P.oldInfo[q] != info) continue;
+        const float4 c = P.oldCert[q];
+        if (c.x > 0.f) {
+          const float4 mA = P.step[A].motion, mB = P.step[j].motion;
+          const float ex = mA.x - mB.x, ey = mA.y - mB.y, ez = mA.z - mB.z;
+          const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB.w * c.y + mA.w * fabsf(c.z)) * (1.f + 1e-5f);
+          const float rem = (c.x - bound) * (1.f - 1e-5f) - 1e-6f * c.x;  // the roundings of both checks
+          if (rem > 0.f) {
+            const double cA = P.scale[A] * P.shapes[P.shape[A]].nodeR * (1.0 + 1e-6);
+            const double cB = (len(P.posDet[A] - P.posDet[j]) + cA) * (1.0 + 1e-6);
+            nc = make_float4(rem, float(cB * (1.0 + 1e-6)), copysignf(float(cA * (1.0 + 1e-6)), c.z), -1.f);
+          }
+        }
+        break;
+      }
+    }
+    P.cert[p] = nc;
+  }
+}
+__global__ void k_inherit_certs(Params P) {
+  LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
+  inheritCerts(P, gridSpan());
+}
 __global__ void k_pairs_emit(Params P) {
   LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   pairsEmit(P, gridSpan());
@@ -1098,8 +1150,8 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   __shared__ int sCount, sBase;
   if (P.devInt[3] & 4) return;  // a grid barrier of the broad phase timed out: no pair list to read
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
-  const bool rebuilt = P.devInt[rebuildFlagIdx(P.curBuf)] != 0;  // a new pair list: no certificate exists yet
-  const bool fresh = rebuilt || !P.useCert;
+  const bool rebuilt = P.devInt[rebuildFlagIdx(P.curBuf)] != 0;  // a new pair list this step
+  const bool fresh = !P.useCert;  // (a new list holds the certificates it inherited)
   const int lane = threadIdx.x & 63;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
@@ -1178,7 +1230,8 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         else if (rebuilt) P.cert[p] = make_float4(-1.f, 0.f, 0.f, -1.f);
         // a pair skipped under a certificate has had no contacts since the evaluation that issued
         // it (a pair with candidate nodes gets no certificate); others drop last step's count
-        if (!act && !(cv[r].x > 0.f || cv[r].w > 0.f)) P.pairCcnt[p] = 0;
+        // (on a new list every count is the old list's)
+        if (!act && (rebuilt || !(cv[r].x > 0.f || cv[r].w > 0.f))) P.pairCcnt[p] = 0;
       }
       actv[r] = act && p < nPairs;
     }
@@ -2177,6 +2230,7 @@ __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
   grid.sync();
   // particle data in sorted order; candidate pairs per master, prefix, pair list
   for (int k = g.tid; k < P.nP; k += g.nth) gatherParticle(P, k);
+  backupPairs(P, g);
   if (g.tid == 0) P.devInt[2] = 0;  // large-bucket list of the reaction sort
   grid.sync();
   pairsFind(P, g);
@@ -2186,6 +2240,9 @@ __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
   scanApplyTiles(P.pairCount, P.nP, P.scanTiles, P.pairStart, lds);
   grid.sync();
   pairsEmit(P, g);
+  grid.sync();
+  inheritCerts(P, g);  // reads the motion records before they restart
+  grid.sync();
   markBuild(P, g);
   grid.sync();
   // reaction CSR: pairs by partner, in pair order

@@ -84,3 +84,30 @@ def test_same_as_full_evaluation(L, S):
           prof["rebuilds"], "contacts at checkpoints", total_contacts)
     assert total_contacts > 0
     assert frac < 0.5
+
+
+def test_certificates_through_rebuilds(L, S):
+    # A rebuild every ~10 steps (small skin, fast lateral motion, periodic wraps): particle pairs of
+    # each new list inherit what is left of their certificates.  Against full evaluation every
+    # step, contacts and states stay bit-identical.
+    sc = S.scene_c2(n=1500, n_types=3, n_grid=24, n_nodes=128, cell_xy=0.05, gap=-0.1, seed=17)
+    rng = np.random.default_rng(21)
+    sc.particles["vel"][:, :2] = rng.normal(0.0, 2.0, (sc.n_particles, 2))
+    sc.particles["angvel"][:] = rng.normal(0.0, 50.0, (sc.n_particles, 3))
+    sc.cfg["skin"] = 2e-4
+    cert = sc.load(L.Sim(sc.cfg))
+    full = sc.load(L.Sim({**sc.cfg, "no_certificates": 1}))
+    cert.profile_begin(150)
+    full.profile_begin(150)
+    for block in range(15):
+        cert.step(10)
+        full.step(10)
+        for l in range(5):
+            ca, cb = cert.contacts(l), full.contacts(l)
+            for k in ca:
+                assert np.array_equal(ca[k], cb[k]), f"step {10 * block + 9} list {LISTS[l]}: {k} differs"
+        _state_equal(cert.get_particles(), full.get_particles(), f"step {10 * block + 9}")
+    pc, pf = cert.profile_end(), full.profile_end()
+    print("rebuilds", pc["rebuilds"], "pairs", pc["pairs"], "evaluated", pc["evaluated_pairs"])
+    assert pc["rebuilds"] == pf["rebuilds"] and pc["rebuilds"] >= 8
+    assert pc["evaluated_pairs"] < 0.5 * pc["pairs"]
