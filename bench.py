@@ -109,6 +109,60 @@ def run(L, sc, caps, warmup, steps, label, surrogate=None):
     return sim, t1 - t0, prof, extra
 
 
+def drums(args):
+    """Independent rotating drums (particle count lognormal, median 4,000, capped by what the drum
+    holds; superellipsoids with exponent U[2,6] and semi-axes 3-5 mm, SDF 32^3, 300 nodes; drum radius
+    0.1 m, depth 0.05 m, 1-30 rpm; mu U[0.1,0.8], e U[0.3,0.9]; dt 1e-5) stepped concurrently from one
+    host thread each (each context has its own stream), then one after another."""
+    import threading
+    import __graft_entry__ as g
+    g.load_package()
+    import lsdem_amd.lsdem as L
+    import lsdem_amd.scenes as S
+    rng = np.random.default_rng(args.seed)
+    scs = []
+    for k in range(args.drums):
+        n = int(np.clip(rng.lognormal(np.log(4000.0), 0.8), 500, 20000))
+        scs.append(S.scene_drum(n=n, seed=args.seed + k, rpm=float(rng.uniform(1, 30)), mu=float(rng.uniform(0.1, 0.8)),
+                                restitution=float(rng.uniform(0.3, 0.9))))
+    counts = [sc.n_particles for sc in scs]
+    log(f"[bench] drums: {args.drums}, particles {counts}")
+    # separate broad-phase launches: concurrent contexts never wait on each other's grid barriers
+    make = lambda sc: sc.load(L.Sim({**sc.cfg, "broad_phase_launches": 1}))
+    sims = [make(sc) for sc in scs]
+    for sim in sims:
+        sim.step(10)  # first steps (builds) outside the timing
+    errs = []
+
+    def run(sim):
+        try:
+            sim.step(args.drum_steps)
+        except Exception as ex:
+            errs.append(ex)
+    th = [threading.Thread(target=run, args=(sim,)) for sim in sims]
+    t = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    conc = time.perf_counter() - t
+    if errs:
+        raise errs[0]
+    del sims
+    sims = [make(sc) for sc in scs]
+    for sim in sims:
+        sim.step(10)
+    t = time.perf_counter()
+    for sim in sims:
+        sim.step(args.drum_steps)
+    seq = time.perf_counter() - t
+    total = sum(counts) * args.drum_steps
+    print(json.dumps({"metric": "aggregate particle-steps/s of independent rotating drums, one GPU",
+                      "drums": args.drums, "particles": counts, "steps": args.drum_steps,
+                      "concurrent": total / conc, "sequential": total / seq, "speedup": seq / conc,
+                      "unit": "particle-steps/s", "dtype": "f64", "data": "synthetic"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +180,10 @@ def main():
     ap.add_argument("--gap", type=float, default=-0.12,
                     help="lattice pitch relative to the layer's largest bounding diameter, minus 1 "
                          "(negative: denser than bounding spheres; overlapping orientations are resampled)")
+    ap.add_argument("--drums", type=int, default=0, metavar="K",
+                    help="instead of the main line: K independent rotating drums stepped concurrently from K host "
+                         "threads, then one after another; prints their aggregate particle-steps/s")
+    ap.add_argument("--drum-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -135,6 +193,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"[bench] copied {args.calibrate_copy} MiB x 10")
+        return
+    if args.drums:
+        drums(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",

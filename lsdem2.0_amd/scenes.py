@@ -251,6 +251,83 @@ def scene_c1(n: int = 1000, seed: int = 20261203, n_grid: int = 40, spacing: flo
 
 
 # ----------------------------------------------------------------------------------------------
+# Config 5 building block: one rotating drum (the reference's CylinderWall with the level set
+# positive inside, LevelSetObjectGenerator.h:1877-2136, here as an analytic grid)
+# ----------------------------------------------------------------------------------------------
+def drum_wall_shape(radius: float, depth: float, spacing: float, margin_cells: int = 3):
+    """Closed cylinder with axis y, centred at the origin, as a wall grid (float32, [z, y, x]):
+    inside the cavity the distance to the nearest face (mantle or end cap), outside minus the
+    distance to the cylinder."""
+    h = float(spacing)
+    nr = 2 * (int(np.ceil(radius / h)) + margin_cells) + 1
+    ny = 2 * (int(np.ceil(0.5 * depth / h)) + margin_cells) + 1
+    o = np.array([-0.5 * (nr - 1) * h, -0.5 * (ny - 1) * h, -0.5 * (nr - 1) * h])
+    z, y, x = np.meshgrid(o[2] + h * np.arange(nr), o[1] + h * np.arange(ny), o[0] + h * np.arange(nr), indexing="ij")
+    r, a = np.sqrt(x * x + z * z), np.abs(y)
+    inside = (r <= radius) & (a <= 0.5 * depth)
+    d_in = np.minimum(radius - r, 0.5 * depth - a)
+    d_out = -np.sqrt(np.maximum(r - radius, 0.0) ** 2 + np.maximum(a - 0.5 * depth, 0.0) ** 2)
+    return np.where(inside, d_in, d_out).astype(np.float32), o, h
+
+
+def scene_drum(n: int = 4000, seed: int = 20261206, radius: float = 0.1, depth: float = 0.05, rpm: float = 10.0,
+               mu: float = 0.5, restitution: float = 0.5, n_types: int = 4, n_grid: int = 32, n_nodes: int = 300,
+               semi_axes=(3e-3, 5e-3), exponents=(2.0, 6.0), wall_spacing: float = 1e-3, dt: float = 1e-5,
+               kn: float = 1e5, density: float = 2500.0, gravity: float = 9.81, fill: float = 0.5) -> Scene:
+    """A rotating drum (axis y, speed rpm) holding superellipsoids (exponent and semi-axes uniform in
+    the given ranges, SDF n_grid^3, n_nodes nodes) on a jittered lattice in its lower part, at most
+    `fill` of the cavity's lattice sites.  n is capped by what fits."""
+    rng = np.random.default_rng(seed)
+    shapes: list = []
+    types = []
+    for _ in range(n_types):
+        ax = rng.uniform(semi_axes[0], semi_axes[1], 3)
+        p1, p2 = rng.uniform(exponents[0], exponents[1], 2)
+        ee, en = 2.0 / p1, 2.0 / p2
+        h = 2.0 * ax.max() / (n_grid - 5)
+        sdf, org = superellipsoid_grid(*ax, ee, en, n_grid, h)
+        sid = _add_shape(shapes, sdf, org, h, superellipsoid_nodes(*ax, ee, en, n_nodes), density)
+        types.append((sid, float(np.linalg.norm(shapes[sid]["nodes"], axis=1).max())))
+    rb = max(t[1] for t in types)  # largest bounding radius
+    pitch = 2.0 * rb * 1.01
+    g = np.arange(-radius + 1.01 * rb, radius - rb, pitch)
+    gy = np.arange(-0.5 * depth + 1.01 * rb, 0.5 * depth - rb, pitch)
+    X, Y, Z = np.meshgrid(g, gy, g, indexing="ij")
+    site = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    site = site[np.sqrt(site[:, 0] ** 2 + site[:, 2] ** 2) <= radius - 1.01 * rb]
+    site = site[np.argsort(site[:, 2], kind="stable")]  # lowest first
+    n = int(min(n, int(fill * len(site))))
+    P = _particles(n)
+    kind = rng.integers(0, n_types, n)
+    P["pos"] = site[:n] + rng.uniform(-0.002 * pitch, 0.002 * pitch, (n, 3))
+    P["quat"] = random_quaternions(rng, n)
+    for t, (sid, _) in enumerate(types):
+        m = kind == t
+        S = shapes[sid]
+        P["shape"][m] = sid
+        P["inv_mass"][m] = 1.0 / S["mass"]
+        P["inv_inertia"][m] = S["inv_inertia"]
+    P["kn"][:] = kn
+    P["ks"][:] = 0.4 * kn
+    P["mu"][:] = mu
+    P["e"][:] = restitution
+    sdf, o, h = drum_wall_shape(radius, depth, wall_spacing)
+    wid = _add_shape(shapes, sdf, o, h, np.zeros((0, 3)))
+    w = _walls_dict(1)
+    w["shape"][0] = wid
+    w["radius"][0] = np.sqrt(radius ** 2 + (0.5 * depth) ** 2)
+    w["quat"][0] = (1.0, 0.0, 0.0, 0.0)
+    w["angvel"][0] = (0.0, rpm * 2.0 * np.pi / 60.0, 0.0)
+    w["mu"][0] = mu
+    w["e"][0] = restitution
+    m = radius + 4 * wall_spacing
+    cfg = {"domain_min": (-m, -0.5 * depth - 4 * wall_spacing, -m), "domain_max": (m, 0.5 * depth + 4 * wall_spacing, m),
+           "gravity": (0.0, 0.0, -gravity), "dt": dt}
+    return Scene(cfg=cfg, shapes=shapes, particles=P, walls=w,
+                 meta={"name": "drum", "seed": seed, "rpm": rpm, "mu": mu, "e": restitution})
+
+
+# ----------------------------------------------------------------------------------------------
 # Config 2: 200,000 superquadrics of 10 types, periodic x/y (0.3 m cell), floor wall
 # ----------------------------------------------------------------------------------------------
 def unit_superquadric_types(rng, n_types: int, n_grid: int, n_nodes: int):
