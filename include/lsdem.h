@@ -139,6 +139,12 @@ int lsdem_initialize(lsdem_ctx* ctx);
  * (particleIntegrationKernel.cu:42,72; mySpatialGrid.h:33; LSParticleContactDetectionKernel.cuh:79-215;
  * buildHashStartEnd.cu:69,139; contactKernel.cuh:201-333). */
 int lsdem_step(lsdem_ctx* ctx, int32_t n_steps);
+/* The same n_steps (even, >= 2) replayed from a HIP graph captured on the first call with that
+ * count: one launch per call instead of one per kernel, for simulations small enough to be
+ * launch-bound.  Results are identical to lsdem_step.  Not with an external-force hook, profiling
+ * or the surrogate (LSDEM_ERR_STATE); a pending rebuild request runs the call eagerly.  No
+ * counterpart in the reference. */
+int lsdem_step_graph(lsdem_ctx* ctx, int32_t n_steps);
 
 /* Copy particle state to / from the host; NULL pointers are skipped.  force/torque are the
  * totals of the last step.  Replaces LSParticle::finalize / copyHostToDevice

@@ -311,8 +311,20 @@ struct lsdem_ctx {
   int profEvery = 1;
   long long profSeen = 0;  // steps since lsdem_profile_begin
   bool profiling = false;
+  // steps captured as one HIP graph (lsdem_step_graph), for graphN steps per launch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  int graphN = 0;
+  void dropGraph() {
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+    graphN = 0;
+  }
 
   ~lsdem_ctx() {
+    dropGraph();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     releaseAll();
     if (stream) (void)hipStreamDestroy(stream);
@@ -1100,12 +1112,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   return LSDEM_OK;
 }
 
-int lsdem_step(lsdem_ctx* c, int32_t n) {
-  if (!c) return fail(LSDEM_ERR_ARG, "null context");
-  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
-  if (n < 0) return fail(LSDEM_ERR_ARG, "negative step count");
-  HIPCHK(hipSetDevice(c->cfg.device));
-  if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
+// Launches n steps on the context's stream (no wait).
+int enqueueSteps(lsdem_ctx* c, int n) {
   // consecutive steps share one launch at each boundary (reduce + next kick/drift) unless an
   // external-force hook runs between the second half-kick's sums and the kick
   const bool fuse = !c->hookFn && !c->cfg.no_fused_steps;
@@ -1113,6 +1121,11 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
     int rc = runStep(c, fuse && k > 0, fuse && k + 1 < n);
     if (rc) return rc;
   }
+  return LSDEM_OK;
+}
+
+// Waits for the stream and reports the device's error flags.
+int finishSteps(lsdem_ctx* c) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   int flags[8];
@@ -1126,6 +1139,56 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
     return fail(LSDEM_ERR_CAPACITY, "contact buffer overflow (peak " + std::to_string(flags[5]) + " > capacity " +
                                         std::to_string(c->P.contactCap) + ")");
   return LSDEM_OK;
+}
+
+int lsdem_step(lsdem_ctx* c, int32_t n) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (n < 0) return fail(LSDEM_ERR_ARG, "negative step count");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
+  const int rc = enqueueSteps(c, n);
+  if (rc) return rc;
+  return finishSteps(c);
+}
+
+int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (n < 2 || n % 2) return fail(LSDEM_ERR_ARG, "graph step count must be even and at least 2");
+  if (c->hookFn || c->profiling || c->sur.enabled)
+    return fail(LSDEM_ERR_STATE, "graph steps run without an external-force hook, profiling or the surrogate");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
+  // a pending rebuild request is a launch argument of the first step: those steps run eagerly
+  if (c->forceRebuild) return lsdem_step(c, n);
+  if (c->graphN != n) {
+    c->dropGraph();
+    // capture the launches of n steps; the host's per-step bookkeeping is applied per replay
+    // instead (n is even, so the contact buffers come back to the same parity)
+    const int cur0 = c->cur;
+    const long long step0 = c->stepCount;
+    const double time0 = c->time;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueueSteps(c, n);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    c->cur = cur0;
+    c->stepCount = step0;
+    c->time = time0;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(LSDEM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    c->graph = g;
+    HIPCHK(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
+    c->graphN = n;
+  }
+  HIPCHK(hipGraphLaunch(c->graphExec, c->stream));
+  c->stepCount += n;
+  for (int k = 0; k < n; ++k) c->time += c->P.dt;  // as the eager steps add it
+  return finishSteps(c);
 }
 
 int lsdem_get_particles(lsdem_ctx* c, double* pos, double* quat, double* vel, double* angvel, double* force,

@@ -260,6 +260,10 @@ class Sim:
     def step(self, n: int = 1):
         self._call("step", self.ctx, C.c_int32(n))
 
+    def step_graph(self, n: int):
+        """n steps (even) replayed from a HIP graph captured on the first call with that count."""
+        self._call("step_graph", self.ctx, C.c_int32(n))
+
     def set_external_force(self, fn):
         """fn(time: float, view: DeviceView) runs after the contact and bond forces of every step and
         before the second half-kick; None removes it."""

@@ -123,3 +123,23 @@ def test_sampled_profile(L, S):
     sim.step(30)
     q = sim.profile_end()
     assert q["steps"] == 30 and q["sampled_steps"] == 30
+
+
+@pytest.mark.parametrize("launches", [0, 1])
+def test_graph_steps_match_eager(L, S, launches):
+    # the same calls replayed from a captured graph, with the program's own reads between the
+    # replays; every replay against eager steps (single-launch and separate-launch broad phase)
+    sc = S.scene_c2(n=600, n_types=3, n_grid=24, n_nodes=128, cell_xy=0.05, gap=-0.1, seed=23)
+    rng = np.random.default_rng(2)
+    sc.particles["vel"][:, :2] = rng.normal(0.0, 2.0, (sc.n_particles, 2))
+    sc.cfg["skin"] = 5e-4
+    cfg = {**sc.cfg, "broad_phase_launches": launches}
+    gr = sc.load(L.Sim(cfg))
+    ea = sc.load(L.Sim(cfg))
+    gr.step(2)  # the first steps build the pairs (a pending rebuild request runs eagerly)
+    ea.step(2)
+    for i in range(6):
+        gr.step_graph(20)
+        ea.step(20)
+        _same(gr, ea, f"replay {i}")
+        assert gr.time() == ea.time()
