@@ -179,12 +179,39 @@ def drums(args):
     for sim in sims:
         sim.step(args.drum_steps)
     seq = time.perf_counter() - t
+    del sims
     total = sum(counts) * args.drum_steps
+    tuned = {}
+    if args.autotune:
+        # one setting per size bucket, measured on the bucket's largest drum stepped alone, then the
+        # concurrent run again with each drum under its bucket's setting
+        import lsdem_amd.autotune as A
+        base = {"broad_phase_launches": 1}
+        t = time.perf_counter()
+        chosen, report = A.tune_buckets(L, scs, edges=tuple(args.tune_edges), base=base)
+        tune_s = time.perf_counter() - t
+        log(f"[bench] autotune: {tune_s:.1f} s, " + "; ".join(
+            f"bucket {b}: drum {r['representative']} ({r['particles']} particles) -> {r['best']}" for b, r in report.items()))
+        sims = [sc.load(L.Sim(A.apply(sc, base, c))) for sc, c in zip(scs, chosen)]
+        for sim in sims:
+            sim.step(10)
+        th = [threading.Thread(target=run, args=(sim,)) for sim in sims]
+        t = time.perf_counter()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        conc_tuned = time.perf_counter() - t
+        if errs:
+            raise errs[0]
+        del sims
+        tuned = {"concurrent_tuned": total / conc_tuned, "tuning_s": tune_s,
+                 "tuned": {str(b): r["best"] for b, r in report.items()}}
     print(json.dumps({"metric": "aggregate particle-steps/s of independent rotating drums, one GPU",
                       "drums": args.drums, "particles": counts, "steps": args.drum_steps,
                       "concurrent": total / conc, "sequential": total / seq, "speedup": seq / conc,
                       "concurrent_graph": (sum(counts) * (args.drum_steps // chunk) * chunk) / conc_graph,
-                      "unit": "particle-steps/s", "dtype": "f64", "data": "synthetic"}))
+                      **tuned, "unit": "particle-steps/s", "dtype": "f64", "data": "synthetic"}))
 
 
 def main():
@@ -208,6 +235,10 @@ def main():
                     help="instead of the main line: K independent rotating drums stepped concurrently from K host "
                          "threads, then one after another; prints their aggregate particle-steps/s")
     ap.add_argument("--drum-steps", type=int, default=2000)
+    ap.add_argument("--autotune", action="store_true",
+                    help="with --drums: tune block size, narrow batch and skin per size bucket, then run again")
+    ap.add_argument("--tune-edges", type=int, nargs="*", default=[450],
+                    help="particle counts that split the drums into size buckets for --autotune")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
