@@ -119,6 +119,7 @@ def drums(args):
     g.load_package()
     import lsdem_amd.lsdem as L
     import lsdem_amd.scenes as S
+    legs = set(args.drum_legs.split(","))
     rng = np.random.default_rng(args.seed)
     scs = []
     for k in range(args.drums):
@@ -127,8 +128,11 @@ def drums(args):
                                 restitution=float(rng.uniform(0.3, 0.9))))
     counts = [sc.n_particles for sc in scs]
     log(f"[bench] drums: {args.drums}, particles {counts}")
-    # separate broad-phase launches: concurrent contexts never wait on each other's grid barriers
-    make = lambda sc: sc.load(L.Sim({**sc.cfg, "broad_phase_launches": 1}))
+    # separate broad-phase launches (concurrent contexts never wait on each other's grid barriers), or
+    # the single-launch broad phase on a grid small enough that every context's blocks fit at once
+    bp = ({"broad_phase_launches": 1} if args.drum_broad_blocks < 0 else
+          {"broad_phase_launches": 0, "broad_phase_blocks": args.drum_broad_blocks})
+    make = lambda sc: sc.load(L.Sim({**sc.cfg, **bp}))
     sims = [make(sc) for sc in scs]
     for sim in sims:
         sim.step(10)  # first steps (builds) outside the timing
@@ -149,44 +153,47 @@ def drums(args):
     if errs:
         raise errs[0]
     del sims
-    # the same, each drum's steps replayed from a captured HIP graph of 100 steps
-    chunk = 100
-    sims = [make(sc) for sc in scs]
-    for sim in sims:
-        sim.step(10)
-        sim.step_graph(chunk)  # capture (and first replay) outside the timing
+    conc_graph, chunk = None, 100
+    if "graph" in legs:
+        # the same, each drum's steps replayed from a captured HIP graph of `chunk` steps
+        sims = [make(sc) for sc in scs]
+        for sim in sims:
+            sim.step(10)
+            sim.step_graph(chunk)  # capture (and first replay) outside the timing
 
-    def run_graph(sim):
-        try:
-            for _ in range(args.drum_steps // chunk):
-                sim.step_graph(chunk)
-        except Exception as ex:
-            errs.append(ex)
-    th = [threading.Thread(target=run_graph, args=(sim,)) for sim in sims]
-    t = time.perf_counter()
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
-    conc_graph = time.perf_counter() - t
-    if errs:
-        raise errs[0]
-    del sims
-    sims = [make(sc) for sc in scs]
-    for sim in sims:
-        sim.step(10)
-    t = time.perf_counter()
-    for sim in sims:
-        sim.step(args.drum_steps)
-    seq = time.perf_counter() - t
-    del sims
+        def run_graph(sim):
+            try:
+                for _ in range(args.drum_steps // chunk):
+                    sim.step_graph(chunk)
+            except Exception as ex:
+                errs.append(ex)
+        th = [threading.Thread(target=run_graph, args=(sim,)) for sim in sims]
+        t = time.perf_counter()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        conc_graph = time.perf_counter() - t
+        if errs:
+            raise errs[0]
+        del sims
+    seq = None
+    if "sequential" in legs:
+        sims = [make(sc) for sc in scs]
+        for sim in sims:
+            sim.step(10)
+        t = time.perf_counter()
+        for sim in sims:
+            sim.step(args.drum_steps)
+        seq = time.perf_counter() - t
+        del sims
     total = sum(counts) * args.drum_steps
     tuned = {}
     if args.autotune:
         # one setting per size bucket, measured on the bucket's largest drum stepped alone, then the
         # concurrent run again with each drum under its bucket's setting
         import lsdem_amd.autotune as A
-        base = {"broad_phase_launches": 1}
+        base = bp
         t = time.perf_counter()
         chosen, report = A.tune_buckets(L, scs, edges=tuple(args.tune_edges), base=base)
         tune_s = time.perf_counter() - t
@@ -209,9 +216,10 @@ def drums(args):
                  "tuned": {str(b): r["best"] for b, r in report.items()}}
     print(json.dumps({"metric": "aggregate particle-steps/s of independent rotating drums, one GPU",
                       "drums": args.drums, "particles": counts, "steps": args.drum_steps,
-                      "concurrent": total / conc, "sequential": total / seq, "speedup": seq / conc,
-                      "concurrent_graph": (sum(counts) * (args.drum_steps // chunk) * chunk) / conc_graph,
-                      **tuned, "unit": "particle-steps/s", "dtype": "f64", "data": "synthetic"}))
+                      "concurrent": total / conc, "sequential": total / seq if seq else None,
+                      "speedup": seq / conc if seq else None,
+                      "concurrent_graph": (sum(counts) * (args.drum_steps // chunk) * chunk) / conc_graph if conc_graph else None,
+                      "broad_phase": bp, **tuned, "unit": "particle-steps/s", "dtype": "f64", "data": "synthetic"}))
 
 
 def main():
@@ -235,6 +243,13 @@ def main():
                     help="instead of the main line: K independent rotating drums stepped concurrently from K host "
                          "threads, then one after another; prints their aggregate particle-steps/s")
     ap.add_argument("--drum-steps", type=int, default=2000)
+    ap.add_argument("--drum-broad-blocks", type=int, default=0,
+                    help="with --drums: each drum's broad phase as one launch of B blocks (0: the library's choice, "
+                         "8 blocks at these sizes; grid barriers, so all drums' blocks must fit on the GPU "
+                         "together), or as separate launches (-1)")
+    ap.add_argument("--drum-legs", default="concurrent,graph,sequential",
+                    help="with --drums: which runs besides the concurrent one to make (graph: concurrent replays of "
+                         "captured HIP graphs; sequential: one drum after another)")
     ap.add_argument("--autotune", action="store_true",
                     help="with --drums: tune block size, narrow batch and skin per size bucket, then run again")
     ap.add_argument("--tune-edges", type=int, nargs="*", default=[450],

@@ -66,8 +66,8 @@ typedef struct lsdem_config {
   int32_t broad_phase_launches; /* 1: run the broad phase as separate launches gated on the
                               device instead of one launch with grid barriers; results do not
                               depend on it */
-  int32_t broad_phase_blocks; /* grid of the single-launch broad phase (0: automatic); capped at
-                              what can be resident together */
+  int32_t broad_phase_blocks; /* grid of the single-launch broad phase, blocks of 1024 threads (0:
+                              one per 512 particles, at least 8); capped at the CU count */
   int32_t narrow_batch;   /* candidate pairs per wavefront batch of the narrow phase: 1, 2, 4 or 8
                              (0: chosen each step from the number of pairs it evaluates) */
   int32_t no_fused_steps; /* 1: launch each step's kick/drift and its closing sums separately

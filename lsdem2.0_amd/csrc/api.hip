@@ -970,7 +970,10 @@ int lsdem_initialize(lsdem_ctx* c) {
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k_broad_phase),
                                                         kBroadThreadsHost, 0));
     if (perCU < 1) return fail(LSDEM_ERR_HIP, "the single-launch broad phase does not fit a CU");
-    c->coopBlocks = c->numCU;  // one block of 1024 threads per CU
+    // one block of 1024 threads per 512 particles, at least 8 and at most one per CU: a small
+    // simulation leaves the rest of the GPU to others stepped concurrently, and its rebuilds meet
+    // at barriers of few blocks
+    c->coopBlocks = int(std::min<long long>(c->numCU, std::max<long long>(8, (nP + 511) / 512)));
     if (c->cfg.broad_phase_blocks > 0) c->coopBlocks = std::min(c->coopBlocks, int(c->cfg.broad_phase_blocks));
   }
   // candidate nodes (slots of the exact test); reserved per batch at their exact count

@@ -34,12 +34,15 @@ def test_drum_matches_oracle(L, S):
     assert np.array_equal(wa["quat"], wb["quat"])  # the drum turned the same way
 
 
-def test_drums_concurrently(L, S):
+@pytest.mark.parametrize("bp", [{"broad_phase_launches": 1}, {"broad_phase_launches": 0, "broad_phase_blocks": 8}],
+                         ids=["separate_launches", "single_launch_8_blocks"])
+def test_drums_concurrently(L, S, bp):
     cfgs = [dict(seed=11 + k, rpm=10.0 + 15.0 * k, mu=0.2 + 0.2 * k, e=0.3 + 0.2 * k) for k in range(4)]
     scs = [_small_drum(S, **c) for c in cfgs]
-    # separate broad-phase launches: concurrent contexts never wait on each other's grid barriers
-    conc = [sc.load(L.Sim({**sc.cfg, "broad_phase_launches": 1})) for sc in scs]
-    seq = [sc.load(L.Sim({**sc.cfg, "broad_phase_launches": 1})) for sc in scs]
+    # separate broad-phase launches (concurrent contexts never wait on each other's grid barriers), or
+    # single launches of 8 blocks (the four contexts' blocks fit on the GPU together)
+    conc = [sc.load(L.Sim({**sc.cfg, **bp})) for sc in scs]
+    seq = [sc.load(L.Sim({**sc.cfg, **bp})) for sc in scs]
     errs = []
 
     def run(sim):
