@@ -1625,8 +1625,10 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   // batch bi takes the evaluated pairs bi, bi + NB, bi + 2 NB, ...: pairs near contact come in
   // spatial runs, and spreading them over the batches keeps the slowest wavefront short
   // pairs per batch: when few pairs are evaluated, small batches spread the slowest pairs (those
-  // near contact, with many surviving clusters) over more wavefronts
-  const int nb = P.narrowBatch > 0 ? P.narrowBatch : (nAct <= nWaves / 2 ? 2 : (nAct <= nWaves ? 4 : kBatch));
+  // near contact, with many surviving clusters) over more wavefronts (one pair per wavefront when
+  // there are at least eight wavefronts per pair, as in small simulations)
+  const int nb = P.narrowBatch > 0 ? P.narrowBatch
+                                   : (nAct <= nWaves / 8 ? 1 : (nAct <= nWaves / 2 ? 2 : (nAct <= nWaves ? 4 : kBatch)));
   const int NB = (nAct + nb - 1) / nb;
   for (int bi = wave; bi < NB; bi += nWaves) {
     // ---- level 1: clusters ----
