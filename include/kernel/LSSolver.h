@@ -10,7 +10,6 @@
 //  - a later solve() continues from the state of the previous one; if bodies or bonds were added
 //    in between, or the domain/time step/gravity/periodicity changed, the library context is
 //    rebuilt from the current state and the tangential contact history starts empty;
-//  - frame 0 of a phase holds no contacts (the library reports the contacts of the last step).
 // Output goes to <dir>_phase<k>/{LSParticle,LSParticleInteraction,Wall,LSParticle-WallInteraction}
 // as in the reference (LSSolver.h:598-660), plus one .pvd collection per series.
 #pragma once
@@ -232,6 +231,12 @@ class LSSolver {
 
     lsdem_set_time(ctx_, 0.0);
     size_t iStep = 0, iFrame = 0;
+    // the pre-loop contact build and periodic wrap of LSSolver::compute* (LSSolver.h:996-998,
+    // 1061-1066); frame 0 shows its contacts
+    if (lsdem_begin_phase(ctx_) != LSDEM_OK) {
+      std::cerr << "[Solver] " << lsdem_last_error() << std::endl;
+      return;
+    }
     output(iFrame, iStep, 0.0);
     while (iStep <= numStep) {
       // steps until the next frame (or the end of the loop)

@@ -139,6 +139,12 @@ int lsdem_initialize(lsdem_ctx* ctx);
  * (particleIntegrationKernel.cu:42,72; mySpatialGrid.h:33; LSParticleContactDetectionKernel.cuh:79-215;
  * buildHashStartEnd.cu:69,139; contactKernel.cuh:201-333). */
 int lsdem_step(lsdem_ctx* ctx, int32_t n_steps);
+/* The pre-loop build that opens each LSSolver::compute* call (LSSolver.h:996-998, 1029-1031,
+ * 1061-1066, 1097-1104): contact lists at the current poses with springs carried over, then the
+ * periodic wrap and the ghost images; no integration, no forces, no step counted.  It runs by
+ * itself before the first lsdem_step after lsdem_initialize; the drop-in's solve() calls it at
+ * the start of every phase.  Contacts are readable after it (the reference's frame 0). */
+int lsdem_begin_phase(lsdem_ctx* ctx);
 /* The same n_steps (even, >= 2) replayed from a HIP graph captured on the first call with that
  * count: one launch per call instead of one per kernel, for simulations small enough to be
  * launch-bound.  Results are identical to lsdem_step.  Not with an external-force hook, profiling

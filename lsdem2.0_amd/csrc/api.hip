@@ -45,6 +45,7 @@ size_t candidatesLdsBytes(int maxClusters);
 __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P, int kick);
 __global__ void k_kick(Params P);
+__global__ void k_prime_end(Params P);
 }  // namespace lsd
 
 using namespace lsd;
@@ -244,6 +245,8 @@ struct lsdem_ctx {
   DBuf<unsigned char> dBodyHit, dForceSet;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
+  bool primePending = false;  // the pre-loop build runs before the next step
+  bool haveContacts = false;  // a detection has filled the contact buffers
   DBuf<int> dNodeCluster;
   // learned culling surrogate
   struct Sur {
@@ -583,6 +586,38 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
   return LSDEM_OK;
 }
 
+// The pre-loop build of a solve phase (LSSolver.h:996-998, 1029-1031, 1061-1066, 1097-1104):
+// keys, candidate pairs and contact lists at the current poses, the periodic wrap and the ghost
+// images, with springs carried over; no integration, no forces, no step counted.
+int runPrime(lsdem_ctx* c) {
+  Params& P = c->P;
+  setCurrentBuffers(c);
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  hipStream_t s = c->stream;
+  P.prime = 1;
+  c->forceRebuild = false;
+  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, 1);
+  int rc = LSDEM_OK;
+  if (c->coopBlocks > 0) {
+    hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(kBroadThreadsHost), 0, s, P);
+  } else {
+    auto none = [](bool = true) {};
+    rc = broadPhaseLaunches(c, P.devInt + rebuildFlagIdx(P.curBuf), none);
+  }
+  if (rc == LSDEM_OK) {
+    hipLaunchKernelGGL(k_pair_check, dim3(unsigned((c->numCU * 4 + 7) / 8 * 8)), dim3(256), 0, s, P, 0);
+    hipLaunchKernelGGL(k_candidates<false>, dim3(unsigned(c->narrowBlocks)), dim3(256),
+                       unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
+    hipLaunchKernelGGL(k_prime_end, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  }
+  P.prime = 0;
+  if (rc) return rc;
+  c->cur = 1 - c->cur;
+  c->primePending = false;
+  c->haveContacts = true;
+  return LSDEM_OK;
+}
+
 d3 rd3(const double* p, size_t i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 void wr3(double* p, size_t i, d3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
 q4 rq4(const double* p, size_t i) { return q4{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]}; }
@@ -758,13 +793,14 @@ int lsdem_add_bond(lsdem_ctx* c, int32_t i, int32_t j, const double point[3], co
   c->bActive.push_back(1);
   const double cst[9] = {B1, B2, B3, B4, r, L, sigmaT, coh, bmu};
   c->bConst.insert(c->bConst.end(), cst, cst + 9);
-  c->bLocal.push_back(normalize(rotInv(N1i, qi)));
-  c->bLocal.push_back(normalize(rotInv(N2i, qi)));
-  c->bLocal.push_back(normalize(rotInv(N3i, qi)));
+  // local frames stored as rotated, without renormalising (VBondedInteraction.h:161-166)
+  c->bLocal.push_back(rotInv(N1i, qi));
+  c->bLocal.push_back(rotInv(N2i, qi));
+  c->bLocal.push_back(rotInv(N3i, qi));
   c->bLocal.push_back(rotInv(gpi - c->pPos[i], qi));
-  c->bLocal.push_back(normalize(rotInv(N1j, qj)));
-  c->bLocal.push_back(normalize(rotInv(N2j, qj)));
-  c->bLocal.push_back(normalize(rotInv(N3j, qj)));
+  c->bLocal.push_back(rotInv(N1j, qj));
+  c->bLocal.push_back(rotInv(N2j, qj));
+  c->bLocal.push_back(rotInv(N3j, qj));
   c->bLocal.push_back(rotInv(gpj - c->pPos[j], qj));
   c->bPoint.push_back(pt);
   return LSDEM_OK;
@@ -1112,11 +1148,17 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipGetLastError());
   c->initialized = true;
+  c->primePending = true;  // the first step is preceded by the pre-loop build, as in LSSolver::compute*
+  c->haveContacts = false;
   return LSDEM_OK;
 }
 
 // Launches n steps on the context's stream (no wait).
 int enqueueSteps(lsdem_ctx* c, int n) {
+  if (n > 0 && c->primePending) {
+    const int rc = runPrime(c);
+    if (rc) return rc;
+  }
   // consecutive steps share one launch at each boundary (reduce + next kick/drift) unless an
   // external-force hook runs between the second half-kick's sums and the kick
   const bool fuse = !c->hookFn && !c->cfg.no_fused_steps;
@@ -1124,6 +1166,7 @@ int enqueueSteps(lsdem_ctx* c, int n) {
     int rc = runStep(c, fuse && k > 0, fuse && k + 1 < n);
     if (rc) return rc;
   }
+  if (n > 0) c->haveContacts = true;
   return LSDEM_OK;
 }
 
@@ -1155,6 +1198,16 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
   return finishSteps(c);
 }
 
+int lsdem_begin_phase(lsdem_ctx* c) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (c->P.nP == 0) { c->primePending = false; return LSDEM_OK; }
+  const int rc = runPrime(c);
+  if (rc) return rc;
+  return finishSteps(c);
+}
+
 int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
   if (!c) return fail(LSDEM_ERR_ARG, "null context");
   if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
@@ -1163,8 +1216,9 @@ int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
     return fail(LSDEM_ERR_STATE, "graph steps run without an external-force hook, profiling or the surrogate");
   HIPCHK(hipSetDevice(c->cfg.device));
   if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
-  // a pending rebuild request is a launch argument of the first step: those steps run eagerly
-  if (c->forceRebuild) return lsdem_step(c, n);
+  // a pending rebuild request is a launch argument of the first step, and a pending pre-loop build
+  // is launched before it: those steps run eagerly
+  if (c->forceRebuild || c->primePending) return lsdem_step(c, n);
   if (c->graphN != n) {
     c->dropGraph();
     // capture the launches of n steps; the host's per-step bookkeeping is applied per replay
@@ -1266,7 +1320,7 @@ int lsdem_get_contacts(lsdem_ctx* c, int32_t list, int64_t capacity, int64_t* co
   HIPCHK(hipSetDevice(c->cfg.device));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (count) *count = 0;
-  if (c->stepCount == 0 || c->P.nP == 0) return LSDEM_OK;
+  if (!c->haveContacts || c->P.nP == 0) return LSDEM_OK;
   int di[8];
   HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
   const long long n = std::min<long long>(di[1], c->P.contactCap);

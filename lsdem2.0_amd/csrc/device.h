@@ -71,6 +71,9 @@ struct Params {
   double dt;
   int quirks, periodicX, periodicY, sdfCull;
   int narrowBatch;    // pairs per wavefront batch of k_candidates (0: chosen on the device)
+  int prime;          // 1: the pre-loop build of a solve phase (detection at the current poses: no
+                      // kick, no drift, no contact law; springs carried over as found)
+  int pad1;
   GridGeom gridP, gridW;
   SearchGrid sg;
   double rMax;        // max particle bounding radius

@@ -312,7 +312,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
   d3 vel = in.vel, w = in.w;
   const q4 q0 = in.quat;
   const double invM = in.invM;
-  if (!isZero(invM)) {
+  if (!isZero(invM) && !P.prime) {
     const double h = 0.5 * P.dt;
     const d3 dv = (F * invM + P.g) * h;
     const d3 dw = (mul(rotInvInertia(q0, in.invI), T)) * h;
@@ -326,9 +326,12 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
     P.angvel[i] = w;
   }
   d3 p = in.pos;
-  p += P.dt * vel;
-  const q4 qn = qintegrate(q0, w, P.dt);
-  P.quat[i] = qn;
+  q4 qn = q0;
+  if (!P.prime) {  // (the pre-loop build detects at the current poses)
+    p += P.dt * vel;
+    qn = qintegrate(q0, w, P.dt);
+    P.quat[i] = qn;
+  }
   const bool periodic = P.periodicX || P.periodicY;
   const bool wrapAfter = (P.quirks & 4) != 0;
   const GridGeom& G = P.gridP;
@@ -421,10 +424,13 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
 
 __device__ void wallDrift(const Params& P, int w, int buf) {
   d3 p = P.wpos[w];
-  p += P.dt * P.wvel[w];
-  P.wpos[w] = p;
-  const q4 q = qintegrate(P.wquat[w], P.wangvel[w], P.dt);
-  P.wquat[w] = q;
+  q4 q = P.wquat[w];
+  if (!P.prime) {
+    p += P.dt * P.wvel[w];
+    P.wpos[w] = p;
+    q = qintegrate(q, P.wangvel[w], P.dt);
+    P.wquat[w] = q;
+  }
   P.wkey[w] = refKey(P.gridW, p);
   // displacement bound of any point of the wall's grid box since the last build: translation plus
   // 2 R sin(angle / 2) for the rotation
@@ -1539,6 +1545,17 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
   P.hashSlot[c] = histInsert(P, key, c);
   const d3 vc = vA + cross(wA, h.point - riA) - (vB + cross(wB, h.point - xf));
   d3 F = mk3(0., 0., 0.);
+  if (P.prime) {
+    // the pre-loop build only lists the contact with the spring it found
+    // (LSParticleContactDetectionKernel.cu:280-289); the force kernels do not run
+    P.cSpring[c] = spring;
+    P.cEn[c] = 0.;
+    P.cEs[c] = 0.;
+    P.cF[c] = F;
+    P.cTA[c] = F;
+    P.cTB[c] = F;
+    return;
+  }
   linearContact(F, spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
   P.cSpring[c] = spring;
   P.cEn[c] = 0.5 * h.ov * h.ov * kn;
@@ -2113,6 +2130,14 @@ __global__ void k_reduce_drift(Params P, StepHead H) {
     P.torque[b] = T;
     P.forceSet[b] = any ? 1 : 0;
   }
+}
+
+// Close of the pre-loop build: step bookkeeping, and the contact flags of the candidate kernel
+// cleared (no sums follow; the stored forces stay those of the last step).
+__global__ void k_prime_end(Params P) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) stepEnd(P);
+  if (b < P.nP) P.bodyHit[b] = 0;
 }
 
 // Second half-kick from the stored sums (after an external-force hook has added to them).

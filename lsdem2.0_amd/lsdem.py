@@ -260,6 +260,12 @@ class Sim:
     def step(self, n: int = 1):
         self._call("step", self.ctx, C.c_int32(n))
 
+    def begin_phase(self):
+        """The pre-loop build of a solve phase (LSSolver.h:996-998, 1061-1066): contacts at the
+        current poses, periodic wrap, ghost images; no step is taken.  Runs by itself before the
+        first step after initialize()."""
+        self._call("begin_phase", self.ctx)
+
     def step_graph(self, n: int):
         """n steps (even) replayed from a HIP graph captured on the first call with that count."""
         self._call("step_graph", self.ctx, C.c_int32(n))

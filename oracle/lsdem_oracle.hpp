@@ -338,6 +338,7 @@ class Solver {
     gridP.set(domMin, domMax, P.n() ? cellP * 2.0 : 0.);
     gridW.set(domMin, domMax, W.n() ? cellW * 2.0 : 0.);
     for (auto& l : lists) l.clear();
+    primePending = true;
   }
 
   // ---- per-body geometry of the scaled shape ----
@@ -736,12 +737,13 @@ class Solver {
     const V3 N2i = normalize(cross(ref, N1i));
     const V3 N3i = normalize(cross(N1i, N2i));
     const V3 N1j = -N1i, N2j = N2i, N3j = N3i;
-    b.n1i = normalize(rotateInv(N1i, P.q[i]));
-    b.n2i = normalize(rotateInv(N2i, P.q[i]));
-    b.n3i = normalize(rotateInv(N3i, P.q[i]));
-    b.n1j = normalize(rotateInv(N1j, P.q[j]));
-    b.n2j = normalize(rotateInv(N2j, P.q[j]));
-    b.n3j = normalize(rotateInv(N3j, P.q[j]));
+    // the rotated frames are stored as they come out of the rotation (VBondedInteraction.h:161-166)
+    b.n1i = rotateInv(N1i, P.q[i]);
+    b.n2i = rotateInv(N2i, P.q[i]);
+    b.n3i = rotateInv(N3i, P.q[i]);
+    b.n1j = rotateInv(N1j, P.q[j]);
+    b.n2j = rotateInv(N2j, P.q[j]);
+    b.n3j = rotateInv(N3j, P.q[j]);
     b.i = i; b.j = j; b.active = 1; b.point = point; b.radius = r; b.L0 = L;
     b.sigmaT = sigmaT; b.cohesion = coh; b.mu = mu;
     bonds.push_back(b);
@@ -799,11 +801,11 @@ class Solver {
     G.vel = P.vel;
   }
 
-  // ---- one step: LSSolver::compute* loop body (LSSolver.h:994-1132) ----
-  void step() {
-    kick(0.5 * dt);
-    drift(P, dt);
-    drift(W, dt);
+  // ---- detection: keys, particle-particle and wall lists, periodic wrap, ghost lists ----
+  // The part of a step between the drift and the force kernels (LSSolver.h:1003-1004,
+  // 1071-1074): contact lists rebuilt at the current poses, each carrying its springs over
+  // from the list it replaces.
+  void detect() {
     std::vector<int> keyP(P.n()), keyW(W.n());
     for (size_t i = 0; i < P.n(); ++i) keyP[i] = gridP.key(P.pos[i]);
     for (size_t i = 0; i < W.n(); ++i) keyW[i] = gridW.key(W.pos[i]);
@@ -840,6 +842,26 @@ class Solver {
         buildFixed(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
       }
     }
+  }
+
+  // ---- the pre-loop build of each LSSolver::compute* call (LSSolver.h:996-998, 1029-1031,
+  // 1061-1066, 1097-1104): detection at the current poses with no integration and no forces.
+  // Its effects: particles outside a periodic cell are wrapped before the first kick, and the
+  // first step finds its springs in lists built at the phase's starting poses.
+  bool primePending = false;  // set by initialize: the first step is preceded by it
+  void prime() {
+    detect();
+    primePending = false;
+  }
+
+  // ---- one step: LSSolver::compute* loop body (LSSolver.h:994-1132) ----
+  void step() {
+    if (primePending) prime();
+    kick(0.5 * dt);
+    drift(P, dt);
+    drift(W, dt);
+    detect();
+    const bool periodic = perX || perY;
     for (size_t i = 0; i < P.n(); ++i) { P.force[i] = v3(0, 0, 0); P.torque[i] = v3(0, 0, 0); }
     forcesPP(lists[LIST_PP]);
     forcesPW(lists[LIST_PW]);

@@ -119,6 +119,29 @@ int lsdem_oracle_step(void* ctx, int32_t n) {
   return LSDEM_OK;
 }
 
+// The pre-loop build of LSSolver::compute* now (lsdem_begin_phase).
+int lsdem_oracle_begin_phase(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  c->s.counters = Counters();
+  c->s.prime();
+  return LSDEM_OK;
+}
+
+// Ghost images of set g (0: +x, 1: +y, 2: +x+y) of the last detection: positions (n x 3) and
+// keys (-1: the image does not exist), PeriodicBoundary.h:215-289.  n must be the particle count.
+int lsdem_oracle_ghosts(void* ctx, int32_t set, int32_t n, double* pos, int32_t* key) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (set < 0 || set > 2) return fail(LSDEM_ERR_ARG, "bad ghost set");
+  const Ghost& G = c->s.ghost[set];
+  if (n != int(c->s.P.n()) || int(G.pos.size()) != n) return fail(LSDEM_ERR_STATE, "no ghost images of that size");
+  for (int i = 0; i < n; ++i) {
+    if (pos) wr3(pos, i, G.pos[i]);
+    if (key) key[i] = G.key[i];
+  }
+  return LSDEM_OK;
+}
+
 int lsdem_oracle_get_particles(void* ctx, double* pos, double* quat, double* vel, double* angvel, double* force,
                                double* torque) {
   Ctx* c = static_cast<Ctx*>(ctx);
