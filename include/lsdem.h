@@ -72,7 +72,17 @@ typedef struct lsdem_config {
                              (0: chosen each step from the number of pairs it evaluates) */
   int32_t no_fused_steps; /* 1: launch each step's kick/drift and its closing sums separately
                              instead of one launch per step boundary; results do not depend on it */
-  int32_t reserved[3];
+  int32_t no_recovery;    /* 1: a call that runs out of pair or contact room (or whose broad-phase
+                             grid barrier gives up) returns the error with the state undefined;
+                             0: the call is restored to its start, the buffers grown (or the broad
+                             phase switched to separate launches) and the call run again, with
+                             results identical to a call that had the room from the start */
+  int32_t barrier_spin;   /* polls of a broad-phase grid barrier before it gives up (0: ~1 s; -1:
+                             none, which makes the barriers give up: a test of the recovery) */
+  int32_t broad_phase_coop; /* 1: launch the single-launch broad phase with hipLaunchCooperativeKernel
+                             (the runtime checks that its blocks are resident together); 0: a plain
+                             launch sized by the occupancy query, the bounded barriers and the
+                             recovery covering the case where they are not */
 } lsdem_config;
 
 typedef struct lsdem_counters {
@@ -83,7 +93,8 @@ typedef struct lsdem_counters {
   int64_t contacts;       /* contacts written (all lists), last step */
   int64_t pair_capacity;
   int64_t contact_capacity;
-  int64_t reserved[9];
+  int64_t recoveries;     /* calls run again after growing a buffer or switching the broad phase */
+  int64_t reserved[8];
 } lsdem_counters;
 
 /* Fill *cfg with defaults (quirks all on, gravity 0, dt 1e-5, automatic capacities). */

@@ -26,9 +26,9 @@ __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, 
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
                                  int* bigCount, const int* gate);
 __global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
-                              const int* gate);
-__global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
-                                  const int* bigCount, const int* gate);
+                              const int* gate, int nItems);
+__global__ void k_bucket_sort_big(const int* start, int nb, const int* tmp, int* items, const int* bigList,
+                                  const int* bigCount, const int* gate, int nItems);
 __global__ void k_mark_build(Params P);
 __global__ void k_pairs_find(Params P);
 __global__ void k_pairs_emit(Params P);
@@ -46,6 +46,9 @@ __global__ void k_bonds(Params P);
 __global__ void k_reduce_kick(Params P, int kick);
 __global__ void k_kick(Params P);
 __global__ void k_prime_end(Params P);
+__global__ void k_snap_save(Params P, SnapDev S);
+__global__ void k_snap_restore(Params P, SnapDev S);
+__global__ void k_hist_reinsert(Params P, SnapDev S, unsigned long long* key, int* val, int* slotOf);
 }  // namespace lsd
 
 using namespace lsd;
@@ -247,6 +250,26 @@ struct lsdem_ctx {
   bool forceRebuild = true;
   bool primePending = false;  // the pre-loop build runs before the next step
   bool haveContacts = false;  // a detection has filled the contact buffers
+  // the state at the start of the running call (see SnapDev), with the host's own step bookkeeping
+  struct Snap {
+    DBuf<d3> pos, vel, ang, force, torque, wpos, hSpring;
+    DBuf<q4> quat, wquat;
+    DBuf<int> bActive, hNode, meta;
+    DBuf<uint32_t> hInfo;
+    DBuf<unsigned char> forceSet;
+    DBuf<unsigned long long> devStat;
+    long long hCap = 0;
+    long long stepCount = 0, audits = 0, profSeen = 0;
+    double time = 0.0;
+    int cur = 0, profSteps = 0;
+    bool primePending = false, forceRebuild = false, haveContacts = false;
+    void release() {
+      pos.release(); vel.release(); ang.release(); force.release(); torque.release(); wpos.release(); hSpring.release();
+      quat.release(); wquat.release(); bActive.release(); hNode.release(); meta.release(); hInfo.release();
+      forceSet.release(); devStat.release();
+    }
+  } snap;
+  int recoveries = 0;  // calls run again after an error (buffers grown or broad phase switched)
   DBuf<int> dNodeCluster;
   // learned culling surrogate
   struct Sur {
@@ -371,6 +394,7 @@ void lsdem_ctx::releaseAll() {
   dQrel.release(); dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
   dWMotion.release(); dCert.release(); dSurSlack.release(); dBodyHit.release(); dForceSet.release();
   sur.release();
+  snap.release();
 }
 
 namespace {
@@ -427,9 +451,9 @@ int bucketSort(lsdem_ctx* c, const int* bucket, int n, int nb, int* count, int* 
     hipLaunchKernelGGL(k_bucket_scatter, dim3(blocksFor(n, bs)), dim3(bs), 0, c->stream, bucket, n,
                        (const int*)start, (const int*)rank, c->dSortTmp.p, bigCount, gate);
     hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(nb, bs)), dim3(bs), 0, c->stream, (const int*)start, nb,
-                       (const int*)c->dSortTmp.p, out, c->dSortBig.p, bigCount, gate);
-    hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, c->stream, (const int*)start,
-                       (const int*)c->dSortTmp.p, out, (const int*)c->dSortBig.p, (const int*)bigCount, gate);
+                       (const int*)c->dSortTmp.p, out, c->dSortBig.p, bigCount, gate, n);
+    hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, c->stream, (const int*)start, nb,
+                       (const int*)c->dSortTmp.p, out, (const int*)c->dSortBig.p, (const int*)bigCount, gate, n);
   }
   return LSDEM_OK;
 }
@@ -487,18 +511,23 @@ int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
   rc = scan(c, P.rcCount, P.nP, P.rcStart, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_rc_scatter, dim3(gsPairs), dim3(256), 0, s, P);
+  const int rcItems = int(std::min<long long>(P.pairCap, (1ll << 31) - 1));
   hipLaunchKernelGGL(k_bucket_sort, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, (const int*)P.rcStart, P.nP,
-                     (const int*)P.rcTmp, P.rcPair, c->dSortBig.p, P.devInt + 2, gate);
-  hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, s, (const int*)P.rcStart,
-                     (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2), gate);
+                     (const int*)P.rcTmp, P.rcPair, c->dSortBig.p, P.devInt + 2, gate, rcItems);
+  hipLaunchKernelGGL(k_bucket_sort_big, dim3(unsigned(c->numCU)), dim3(256), 0, s, (const int*)P.rcStart, P.nP,
+                     (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2), gate,
+                     rcItems);
   return LSDEM_OK;
 }
 
+int launchBroadPhase(lsdem_ctx* c);
+
 // One step.  headDone: the previous launch already ran this step's kick and drift.  fuseNext: the
 // step ends with the next step's kick and drift (another step follows with no host work between).
-int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
+int runStep(lsdem_ctx* c, bool headDone, bool fuseNext, int callStep) {
   Params& P = c->P;
   setCurrentBuffers(c);
+  P.callStep = callStep;
   const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
   hipStream_t s = c->stream;
   bool prof = false;
@@ -532,7 +561,8 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
   phase(!headDone);
   if (c->coopBlocks > 0) {
     // one launch with grid barriers; all of its blocks return at once on steps that keep the pairs
-    hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(kBroadThreadsHost), 0, s, P);
+    const int rc = launchBroadPhase(c);
+    if (rc) return rc;
     phase();        // sort (the whole broad phase)
     phase(false);   // candidate pairs
   } else {
@@ -586,6 +616,23 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext) {
   return LSDEM_OK;
 }
 
+// The single-launch broad phase (its blocks meet at grid barriers).
+int launchBroadPhase(lsdem_ctx* c) {
+  Params& P = c->P;
+  if (c->cfg.broad_phase_coop) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(c->stream, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+      void* args[] = {&P};
+      HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_broad_phase), dim3(unsigned(c->coopBlocks)),
+                                        dim3(kBroadThreadsHost), args, 0, c->stream));
+      return LSDEM_OK;
+    }
+  }
+  hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(kBroadThreadsHost), 0, c->stream, P);
+  return LSDEM_OK;
+}
+
 // The pre-loop build of a solve phase (LSSolver.h:996-998, 1029-1031, 1061-1066, 1097-1104):
 // keys, candidate pairs and contact lists at the current poses, the periodic wrap and the ghost
 // images, with springs carried over; no integration, no forces, no step counted.
@@ -595,11 +642,12 @@ int runPrime(lsdem_ctx* c) {
   const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
   hipStream_t s = c->stream;
   P.prime = 1;
+  P.callStep = 0;
   c->forceRebuild = false;
   hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, 1);
   int rc = LSDEM_OK;
   if (c->coopBlocks > 0) {
-    hipLaunchKernelGGL(k_broad_phase, dim3(unsigned(c->coopBlocks)), dim3(kBroadThreadsHost), 0, s, P);
+    rc = launchBroadPhase(c);
   } else {
     auto none = [](bool = true) {};
     rc = broadPhaseLaunches(c, P.devInt + rebuildFlagIdx(P.curBuf), none);
@@ -630,6 +678,160 @@ s6 inverse6(s6 A) {
   return s6{(A.yy * A.zz - A.yz * A.yz) * di,  (A.xx * A.zz - A.xz * A.xz) * di,
             (A.xx * A.yy - A.xy * A.xy) * di,  -(A.xy * A.zz - A.xz * A.yz) * di,
             (A.xy * A.yz - A.xz * A.yy) * di,  -(A.xx * A.yz - A.xz * A.xy) * di};
+}
+
+// Pair-sized arrays (candidate pairs, certificates, map records, reaction lists).  Contents are
+// not kept: a new capacity comes with a rebuild of the pair list.
+int allocPairs(lsdem_ctx* c, long long pairCap) {
+  HIPCHK(c->dPairA.alloc(pairCap));
+  HIPCHK(c->dPairInfo.alloc(pairCap));
+  HIPCHK(c->dPairCoff.alloc(pairCap));
+  HIPCHK(c->dPairCcnt.alloc(pairCap));
+  HIPCHK(c->dSurMask.alloc(pairCap));
+  HIPCHK(c->dSurSlack.alloc(pairCap));
+  HIPCHK(c->dCert.alloc(pairCap));
+  HIPCHK(c->dOldInfo.alloc(pairCap));
+  HIPCHK(c->dOldCert.alloc(pairCap));
+  HIPCHK(c->dCertE.alloc(pairCap));
+  HIPCHK(c->dCertQA.alloc(pairCap));
+  HIPCHK(c->dCertQB.alloc(pairCap));
+  HIPCHK(c->dPairRec.alloc(pairCap));
+  HIPCHK(c->dPairMap.alloc(pairCap * 10));
+  HIPCHK(c->dRcPair.alloc(pairCap));
+  HIPCHK(c->dRcTmp.alloc(pairCap));
+  HIPCHK(c->dRcRank.alloc(pairCap));
+  c->P.pairCap = pairCap;
+  return LSDEM_OK;
+}
+
+// Contact-slot arrays and the two history tables (at least twice the slots, a power of two).
+// Contents are not kept; the call snapshot restores the history of the step before the call.
+int allocContacts(lsdem_ctx* c, long long contactCap) {
+  hipStream_t s = c->stream;
+  HIPCHK(c->dCPair.alloc(contactCap));
+  HIPCHK(c->dCF.alloc(contactCap));
+  HIPCHK(c->dCTA.alloc(contactCap));
+  HIPCHK(c->dCTB.alloc(contactCap));
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(c->dCNode[b].alloc(contactCap));
+    HIPCHK(c->dCInfo[b].alloc(contactCap));
+    HIPCHK(c->dCSpring[b].alloc(contactCap));
+    HIPCHK(c->dCPoint[b].alloc(contactCap));
+    HIPCHK(c->dCNormal[b].alloc(contactCap));
+    HIPCHK(c->dCOverlap[b].alloc(contactCap));
+    HIPCHK(c->dCEn[b].alloc(contactCap));
+    HIPCHK(c->dCEs[b].alloc(contactCap));
+    HIPCHK(hipMemsetAsync(c->dCSpring[b].p, 0, contactCap * sizeof(d3), s));
+  }
+  long long hs = 1024;
+  while (hs < 2 * contactCap) hs <<= 1;
+  c->hashSize = hs;
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(c->dHashKey[b].alloc(hs));
+    HIPCHK(c->dHashVal[b].alloc(hs));
+    HIPCHK(c->dHashSlot[b].alloc(contactCap));
+    hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, c->dHashKey[b].p, hs, kHashEmpty);
+  }
+  c->P.hashMask = (unsigned long long)(hs - 1);
+  c->P.contactCap = contactCap;
+  return LSDEM_OK;
+}
+
+// Snapshot buffers: the bodies, walls and bonds, and room for one step's contact slots.
+int snapAlloc(lsdem_ctx* c) {
+  auto& S = c->snap;
+  const int nP = std::max(c->P.nP, 1), nW = std::max(c->P.nW, 1), nB = std::max(c->P.nBonds, 1);
+  if (S.pos.n != size_t(nP)) {
+    HIPCHK(S.pos.alloc(nP)); HIPCHK(S.vel.alloc(nP)); HIPCHK(S.ang.alloc(nP)); HIPCHK(S.force.alloc(nP));
+    HIPCHK(S.torque.alloc(nP)); HIPCHK(S.quat.alloc(nP)); HIPCHK(S.forceSet.alloc(nP));
+    HIPCHK(S.wpos.alloc(nW)); HIPCHK(S.wquat.alloc(nW)); HIPCHK(S.bActive.alloc(nB));
+    HIPCHK(S.meta.alloc(4)); HIPCHK(S.devStat.alloc(kDevStats));
+  }
+  if (S.hCap != c->P.contactCap) {
+    HIPCHK(S.hNode.alloc(c->P.contactCap)); HIPCHK(S.hInfo.alloc(c->P.contactCap));
+    HIPCHK(S.hSpring.alloc(c->P.contactCap));
+    S.hCap = c->P.contactCap;
+  }
+  return LSDEM_OK;
+}
+
+SnapDev snapDev(lsdem_ctx* c, int cur) {
+  auto& S = c->snap;
+  const int o = 1 - cur;
+  return SnapDev{S.pos.p, S.vel.p, S.ang.p, S.force.p, S.torque.p, S.wpos.p, S.quat.p, S.wquat.p, S.bActive.p,
+                 S.forceSet.p, S.devStat.p, S.meta.p, S.hNode.p, S.hInfo.p, S.hSpring.p, S.hCap,
+                 c->dCNode[o].p, c->dCInfo[o].p, c->dCSpring[o].p, o};
+}
+
+unsigned snapBlocks(lsdem_ctx* c) { return unsigned(c->numCU * 4); }
+
+// The state at the start of a call (enqueued on the context's stream).
+int snapSave(lsdem_ctx* c) {
+  auto& S = c->snap;
+  hipLaunchKernelGGL(k_snap_save, dim3(snapBlocks(c)), dim3(256), 0, c->stream, c->P, snapDev(c, c->cur));
+  HIPCHK(hipGetLastError());
+  S.stepCount = c->stepCount; S.time = c->time; S.cur = c->cur; S.audits = c->sur.audits;
+  S.profSteps = c->profSteps; S.profSeen = c->profSeen;
+  S.primePending = c->primePending; S.forceRebuild = c->forceRebuild; S.haveContacts = c->haveContacts;
+  return LSDEM_OK;
+}
+
+// Back to the state of the last snapSave (buffers may have been reallocated since); the next step
+// rebuilds the candidate pairs.
+int snapRestore(lsdem_ctx* c) {
+  auto& S = c->snap;
+  c->stepCount = S.stepCount; c->time = S.time; c->cur = S.cur; c->sur.audits = S.audits;
+  c->profSteps = S.profSteps; c->profSeen = S.profSeen;
+  c->primePending = S.primePending; c->haveContacts = S.haveContacts;
+  c->forceRebuild = true;
+  setCurrentBuffers(c);
+  hipStream_t s = c->stream;
+  for (int b = 0; b < 2; ++b)
+    hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, c->dHashKey[b].p,
+                       (long long)c->hashSize, kHashEmpty);
+  const SnapDev D = snapDev(c, c->cur);
+  hipLaunchKernelGGL(k_snap_restore, dim3(snapBlocks(c)), dim3(256), 0, s, c->P, D);
+  const int o = 1 - c->cur;
+  hipLaunchKernelGGL(k_hist_reinsert, dim3(snapBlocks(c)), dim3(256), 0, s, c->P, D, c->dHashKey[o].p,
+                     c->dHashVal[o].p, c->dHashSlot[o].p);
+  HIPCHK(hipGetLastError());
+  return LSDEM_OK;
+}
+
+// Device pointers of every context array into the Params table.
+void bindPointers(lsdem_ctx* c) {
+  Params& P = c->P;
+  P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
+  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.subs32 = c->dSubs32.p; P.nodeCluster = c->dNodeCluster.p;
+  P.cPair = c->dCPair.p; P.cF = c->dCF.p; P.cTA = c->dCTA.p; P.cTB = c->dCTB.p;
+  P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;
+  P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
+  P.radius = c->dRadius.p; P.kn = c->dKn.p; P.ks = c->dKs.p; P.mu = c->dMu.p; P.res = c->dRes.p;
+  P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
+  P.nodeStart = c->dNodeStart.p; P.scell = c->dScell.p;
+  P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
+  P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
+  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.oldInfo = c->dOldInfo.p; P.oldCert = c->dOldCert.p; P.oldStart = c->dOldStart.p; P.pairRec = c->dPairRec.p;
+  P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
+  P.surSlack = c->dSurSlack.p;
+  P.bodyHit = c->dBodyHit.p;
+  P.forceSet = c->dForceSet.p;
+  P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
+  P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
+  for (int g = 0; g < 3; ++g) P.gpos[g] = c->dGPos[g].p;
+  P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
+  P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
+  P.sBoxR = c->dSBoxR.p;
+  P.statSlots = c->dStatSlots.p;
+  P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
+  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p; P.surMask = c->dSurMask.p;
+  P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
+  P.rcTmp = c->dRcTmp.p; P.sortTmp = c->dSortTmp.p; P.sortBig = c->dSortBig.p; P.scanTiles = c->dScanBlocks.p;
+  P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
+  P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
+  P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
+  P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
+  setCurrentBuffers(c);
 }
 
 }  // namespace
@@ -1011,61 +1213,38 @@ int lsdem_initialize(lsdem_ctx* c) {
     // at barriers of few blocks
     c->coopBlocks = int(std::min<long long>(c->numCU, std::max<long long>(8, (nP + 511) / 512)));
     if (c->cfg.broad_phase_blocks > 0) c->coopBlocks = std::min(c->coopBlocks, int(c->cfg.broad_phase_blocks));
+    c->coopBlocks = std::min(c->coopBlocks, perCU * c->numCU);  // all resident together on an idle GPU
+  }
+  // the narrow phase's per-wavefront LDS grows with the largest node count of a shape
+  {
+    const size_t lds = candidatesLdsBytes(maxClusters);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->cfg.device));
+    if (lds > prop.sharedMemPerBlock) {
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_candidates<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess ||
+          hipFuncSetAttribute(reinterpret_cast<const void*>(k_candidates<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+        return fail(LSDEM_ERR_ARG, "shapes with " + std::to_string(64 * maxClusters) +
+                                       " boundary nodes need " + std::to_string(lds) +
+                                       " bytes of LDS per narrow-phase block, more than the device's " +
+                                       std::to_string(prop.sharedMemPerBlock));
+    }
   }
   // candidate nodes (slots of the exact test); reserved per batch at their exact count
   long long contactCap = c->cfg.contact_capacity > 0 ? c->cfg.contact_capacity : 64ll * nP + 16384;
   pairCap = std::min<long long>(pairCap, (1ll << 31) - 1);
   contactCap = std::min<long long>(contactCap, (1ll << 30));
-  P.pairCap = pairCap;
-  P.contactCap = contactCap;
   HIPCHK(c->dPairCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dPairStart.alloc(nP + 1));
-  HIPCHK(c->dPairA.alloc(pairCap));
-  HIPCHK(c->dPairInfo.alloc(pairCap));
-  HIPCHK(c->dPairCoff.alloc(pairCap));
-  HIPCHK(c->dPairCcnt.alloc(pairCap));
-  HIPCHK(c->dSurMask.alloc(pairCap));
-  HIPCHK(c->dSurSlack.alloc(pairCap));
   HIPCHK(c->dBodyHit.alloc(std::max(nP, 1)));
   HIPCHK(c->dForceSet.alloc(std::max(nP, 1)));
-  HIPCHK(c->dCert.alloc(pairCap));
-  HIPCHK(c->dOldInfo.alloc(pairCap));
-  HIPCHK(c->dOldCert.alloc(pairCap));
   HIPCHK(c->dOldStart.alloc(nP + 1));
-  HIPCHK(c->dCertE.alloc(pairCap));
-  HIPCHK(c->dCertQA.alloc(pairCap));
-  HIPCHK(c->dCertQB.alloc(pairCap));
-  HIPCHK(c->dPairRec.alloc(pairCap));
-  HIPCHK(c->dPairMap.alloc(pairCap * 10));
   HIPCHK(c->dRcCount.alloc(std::max(nP, 1)));
   HIPCHK(c->dRcStart.alloc(nP + 1));
-  HIPCHK(c->dRcPair.alloc(pairCap));
-  HIPCHK(c->dRcTmp.alloc(pairCap));
-  HIPCHK(c->dRcRank.alloc(pairCap));
-  HIPCHK(c->dCPair.alloc(contactCap));
-  HIPCHK(c->dCF.alloc(contactCap));
-  HIPCHK(c->dCTA.alloc(contactCap));
-  HIPCHK(c->dCTB.alloc(contactCap));
-  for (int b = 0; b < 2; ++b) {
-    HIPCHK(c->dCNode[b].alloc(contactCap));
-    HIPCHK(c->dCInfo[b].alloc(contactCap));
-    HIPCHK(c->dCSpring[b].alloc(contactCap));
-    HIPCHK(c->dCPoint[b].alloc(contactCap));
-    HIPCHK(c->dCNormal[b].alloc(contactCap));
-    HIPCHK(c->dCOverlap[b].alloc(contactCap));
-    HIPCHK(c->dCEn[b].alloc(contactCap));
-    HIPCHK(c->dCEs[b].alloc(contactCap));
-  }
-  long long hs = 1024;
-  while (hs < 2 * contactCap) hs <<= 1;
-  c->hashSize = hs;
-  for (int b = 0; b < 2; ++b) {
-    HIPCHK(c->dHashKey[b].alloc(hs));
-    HIPCHK(c->dHashVal[b].alloc(hs));
-    HIPCHK(c->dHashSlot[b].alloc(contactCap));
-  }
+  if ((rc = allocPairs(c, pairCap))) return rc;
+  if ((rc = allocContacts(c, contactCap))) return rc;
   HIPCHK(c->dStatSlots.alloc(5 * kStatSlots));
-  P.hashMask = (unsigned long long)(hs - 1);
 
   // ---- bonds (+ per-particle CSR of bond ends, in bond order) ----
   {
@@ -1104,47 +1283,13 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dBodyHit.p, 0, std::max(nP, 1), s));
   HIPCHK(hipMemsetAsync(c->dForceSet.p, 1, std::max(nP, 1), s));
-  HIPCHK(hipMemsetAsync(c->dCSpring[0].p, 0, contactCap * sizeof(d3), s));
-  HIPCHK(hipMemsetAsync(c->dCSpring[1].p, 0, contactCap * sizeof(d3), s));
-  for (int b = 0; b < 2; ++b)
-    hipLaunchKernelGGL(k_fill_u64, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, c->dHashKey[b].p, hs, kHashEmpty);
 
-  // ---- pointer table ----
-  P.shapes = c->dShapes.p; P.clusters = c->dClusters.p; P.nodes = c->dNodes.p; P.sdf = c->dSdf.p;
-  P.nodes32 = c->dNodes32.p; P.clusters32 = c->dClusters32.p; P.subs32 = c->dSubs32.p; P.nodeCluster = c->dNodeCluster.p;
   P.surSlot = nullptr; P.surBand = 0.f; P.sur = SurNet{}; P.maxClusters = maxClusters;
-  P.cPair = c->dCPair.p; P.cF = c->dCF.p; P.cTA = c->dCTA.p; P.cTB = c->dCTB.p;
-  P.pos = c->dPos.p; P.posDet = c->dPosDet.p; P.quat = c->dQuat.p; P.vel = c->dVel.p; P.angvel = c->dAng.p;
-  P.force = c->dForce.p; P.torque = c->dTorque.p; P.invI = c->dInvI.p; P.invM = c->dInvM.p;
-  P.radius = c->dRadius.p; P.kn = c->dKn.p; P.ks = c->dKs.p; P.mu = c->dMu.p; P.res = c->dRes.p;
-  P.scale = c->dScale.p; P.invSp = c->dInvSp.p; P.gorig = c->dGorig.p; P.shape = c->dShape.p;
-  P.nodeStart = c->dNodeStart.p; P.scell = c->dScell.p;
-  P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
-  P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
-  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.oldInfo = c->dOldInfo.p; P.oldCert = c->dOldCert.p; P.oldStart = c->dOldStart.p; P.pairRec = c->dPairRec.p;
-  P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
-  P.surSlack = c->dSurSlack.p;
-  P.bodyHit = c->dBodyHit.p;
-  P.forceSet = c->dForceSet.p;
-  P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
-  P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
-  for (int g = 0; g < 3; ++g) {
-    P.gpos[g] = c->dGPos[g].p;
-  }
-  P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
-  P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
-  P.sBoxR = c->dSBoxR.p;
-  P.statSlots = c->dStatSlots.p;
-  P.pairCount = c->dPairCount.p; P.pairStart = c->dPairStart.p; P.pairA = c->dPairA.p;
-  P.pairInfo = c->dPairInfo.p; P.pairCoff = c->dPairCoff.p; P.pairCcnt = c->dPairCcnt.p; P.pairMap = c->dPairMap.p; P.surMask = c->dSurMask.p;
-  P.rcCount = c->dRcCount.p; P.rcStart = c->dRcStart.p; P.rcPair = c->dRcPair.p; P.rcRank = c->dRcRank.p;
-  P.rcTmp = c->dRcTmp.p; P.sortTmp = c->dSortTmp.p; P.sortBig = c->dSortBig.p; P.scanTiles = c->dScanBlocks.p;
-  P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
-  P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
-  P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
-  P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
+  P.barrierSpin = c->cfg.barrier_spin > 0 ? c->cfg.barrier_spin : (c->cfg.barrier_spin < 0 ? 0 : (1 << 21));
   c->cur = 0;
-  setCurrentBuffers(c);
+  bindPointers(c);
+  HIPCHK(hipMemsetAsync(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int), s));
+  if ((rc = snapAlloc(c))) return rc;
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipGetLastError());
   c->initialized = true;
@@ -1163,7 +1308,7 @@ int enqueueSteps(lsdem_ctx* c, int n) {
   // external-force hook runs between the second half-kick's sums and the kick
   const bool fuse = !c->hookFn && !c->cfg.no_fused_steps;
   for (int k = 0; k < n; ++k) {
-    int rc = runStep(c, fuse && k > 0, fuse && k + 1 < n);
+    int rc = runStep(c, fuse && k > 0, fuse && k + 1 < n, k);
     if (rc) return rc;
   }
   if (n > 0) c->haveContacts = true;
@@ -1171,20 +1316,90 @@ int enqueueSteps(lsdem_ctx* c, int n) {
 }
 
 // Waits for the stream and reports the device's error flags.
-int finishSteps(lsdem_ctx* c) {
+// Waits for the stream and reads the error flags of the call (kErr*) and the failing step.
+int readFlags(lsdem_ctx* c, int& flags, int& step) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  int flags[8];
-  HIPCHK(hipMemcpy(flags, c->dDevInt.p, sizeof flags, hipMemcpyDeviceToHost));
-  if (flags[3] & 1)
-    return fail(LSDEM_ERR_CAPACITY, "candidate-pair buffer overflow (peak " + std::to_string(flags[4]) +
-                                        " > capacity " + std::to_string(c->P.pairCap) + ")");
-  if (flags[3] & 4)
-    return fail(LSDEM_ERR_STATE, "broad phase: a grid barrier timed out (blocks were not resident together)");
-  if (flags[3] & 2)
-    return fail(LSDEM_ERR_CAPACITY, "contact buffer overflow (peak " + std::to_string(flags[5]) + " > capacity " +
-                                        std::to_string(c->P.contactCap) + ")");
+  int di[kErrStepIdx + 1];
+  HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
+  flags = di[3];
+  step = di[kErrStepIdx];
   return LSDEM_OK;
+}
+
+int reportFlags(lsdem_ctx* c, int flags, int step) {
+  int di[8];
+  HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
+  const std::string at = " (step " + std::to_string(step) + " of the call)";
+  if (flags & kErrPairs)
+    return fail(LSDEM_ERR_CAPACITY, "candidate-pair buffer overflow (needed " + std::to_string(di[4]) +
+                                        " > capacity " + std::to_string(c->P.pairCap) + ")" + at);
+  if (flags & kErrBarrier)
+    return fail(LSDEM_ERR_STATE, "broad phase: a grid barrier timed out (blocks were not resident together)" + at);
+  if (flags & kErrContacts)
+    return fail(LSDEM_ERR_CAPACITY, "contact buffer overflow (needed " + std::to_string(di[5]) + " > capacity " +
+                                        std::to_string(c->P.contactCap) + ")" + at);
+  return fail(LSDEM_ERR_CAPACITY, "history table full" + at);
+}
+
+// Larger buffers (or the separate-launch broad phase) for what made the call halt.  Returns
+// LSDEM_ERR_CAPACITY when a buffer is already at its largest size.
+int growFor(lsdem_ctx* c, int flags) {
+  int di[8];
+  HIPCHK(hipMemcpy(di, c->dDevInt.p, sizeof di, hipMemcpyDeviceToHost));
+  if (flags & kErrBarrier) c->coopBlocks = 0;  // separate gated launches need no co-residency
+  int rc;
+  if (flags & kErrPairs) {
+    const long long maxCap = (1ll << 31) - 1;
+    const long long cap = std::min(maxCap, std::max(2 * c->P.pairCap, (long long)di[4] + di[4] / 4 + 4096));
+    if (cap <= c->P.pairCap) return reportFlags(c, flags, -1);
+    if ((rc = allocPairs(c, cap))) return rc;
+  }
+  if (flags & (kErrContacts | kErrHash)) {
+    const long long maxCap = 1ll << 30;
+    const long long cap = std::min(maxCap, std::max(2 * c->P.contactCap, (long long)di[5] + di[5] / 4 + 16384));
+    if (cap <= c->P.contactCap) return reportFlags(c, flags, -1);
+    if ((rc = allocContacts(c, cap))) return rc;
+  }
+  bindPointers(c);
+  c->dropGraph();
+  c->recoveries++;
+  return LSDEM_OK;
+}
+
+constexpr int kMaxRecoveries = 8;
+
+// Runs `work` (which enqueues the call's launches) until it completes without an error flag: a call
+// that halted is restored to its start, its buffers grown, and run again, so its results are those
+// of a call that had the room from the start.  The external-force hook runs again for the repeated
+// steps.  With cfg.no_recovery the error is returned at once and the state is undefined.
+extern "C++" template <class Work>
+int runCall(lsdem_ctx* c, Work&& work) {
+  const bool recover = !c->cfg.no_recovery;
+  int rc;
+  if (recover && (rc = snapSave(c))) return rc;
+  for (int attempt = 0;; ++attempt) {
+    if ((rc = work())) return rc;
+    int flags = 0, step = -1;
+    if ((rc = readFlags(c, flags, step))) return rc;
+    if (!flags) return LSDEM_OK;
+    if (!recover || attempt >= kMaxRecoveries) {
+      const int err = reportFlags(c, flags, step);
+      // leave the context able to step again (from whatever state the halted call left)
+      HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
+      HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
+      c->forceRebuild = true;
+      return err;
+    }
+    const long long hCap = c->snap.hCap;
+    if ((rc = growFor(c, flags))) return rc;
+    if ((rc = snapRestore(c))) return rc;
+    if (c->P.contactCap != hCap) {  // room for the history of the larger buffers, filled again
+      HIPCHK(hipStreamSynchronize(c->stream));  // (the restore still reads the old snapshot)
+      if ((rc = snapAlloc(c))) return rc;
+      if ((rc = snapSave(c))) return rc;
+    }
+  }
 }
 
 int lsdem_step(lsdem_ctx* c, int32_t n) {
@@ -1193,9 +1408,8 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
   if (n < 0) return fail(LSDEM_ERR_ARG, "negative step count");
   HIPCHK(hipSetDevice(c->cfg.device));
   if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
-  const int rc = enqueueSteps(c, n);
-  if (rc) return rc;
-  return finishSteps(c);
+  if (n == 0) return LSDEM_OK;
+  return runCall(c, [&]() { return enqueueSteps(c, n); });
 }
 
 int lsdem_begin_phase(lsdem_ctx* c) {
@@ -1203,9 +1417,7 @@ int lsdem_begin_phase(lsdem_ctx* c) {
   if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
   HIPCHK(hipSetDevice(c->cfg.device));
   if (c->P.nP == 0) { c->primePending = false; return LSDEM_OK; }
-  const int rc = runPrime(c);
-  if (rc) return rc;
-  return finishSteps(c);
+  return runCall(c, [&]() { return runPrime(c); });
 }
 
 int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
@@ -1242,10 +1454,29 @@ int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
     HIPCHK(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
     c->graphN = n;
   }
+  // a replay that halts on an error is restored and run again eagerly (with larger buffers)
+  const bool recover = !c->cfg.no_recovery;
+  int rc;
+  if (recover && (rc = snapSave(c))) return rc;
   HIPCHK(hipGraphLaunch(c->graphExec, c->stream));
   c->stepCount += n;
   for (int k = 0; k < n; ++k) c->time += c->P.dt;  // as the eager steps add it
-  return finishSteps(c);
+  c->haveContacts = true;
+  int flags = 0, step = -1;
+  if ((rc = readFlags(c, flags, step))) return rc;
+  if (!flags) return LSDEM_OK;
+  if (!recover) {
+    const int err = reportFlags(c, flags, step);
+    HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
+    HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
+    c->forceRebuild = true;
+    return err;
+  }
+  if ((rc = growFor(c, flags))) return rc;
+  if ((rc = snapRestore(c))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));  // (the restore still reads the old snapshot)
+  if ((rc = snapAlloc(c))) return rc;
+  return lsdem_step(c, n);
 }
 
 int lsdem_get_particles(lsdem_ctx* c, double* pos, double* quat, double* vel, double* angvel, double* force,
@@ -1387,6 +1618,7 @@ int lsdem_get_counters(lsdem_ctx* c, lsdem_counters* out) {
   out->steps = c->stepCount;
   out->pair_capacity = c->P.pairCap;
   out->contact_capacity = c->P.contactCap;
+  out->recoveries = c->recoveries;
   if (!c->initialized || c->P.nP == 0) return LSDEM_OK;
   HIPCHK(hipSetDevice(c->cfg.device));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -73,6 +73,8 @@ struct Params {
   int narrowBatch;    // pairs per wavefront batch of k_candidates (0: chosen on the device)
   int prime;          // 1: the pre-loop build of a solve phase (detection at the current poses: no
                       // kick, no drift, no contact law; springs carried over as found)
+  int callStep;       // index of this step within the lsdem_step call (recorded with an error flag)
+  int barrierSpin;    // poll limit of a broad-phase grid barrier before it gives up (time-out error)
   int pad1;
   GridGeom gridP, gridW;
   SearchGrid sg;
@@ -217,12 +219,13 @@ struct Params {
   const int* bcItem;     // 2*nBonds: (bond << 1) | (0 = i end, 1 = j end)
 
   // ---- device counters ----
-  // devInt: [1] contact slots allocated this step, [3] overflow flags (1 pairs, 2 contacts),
+  // devInt: [1] contact slots allocated this step, [3] error flags (kErr*: any flag halts every later
+  //         kernel of the call),
   //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert,
   //         [7] images at their source's own position, [8] rebuild flag of the step,
   //         [9] pairs the narrow phase evaluates this step, [10 + b] contact slots of the
-  //         last step that filled history table b, [12] grid-barrier arrivals of the broad phase;
-  //         overflow flag 4: a broad-phase grid barrier timed out
+  //         last step that filled history table b, [12] grid-barrier arrivals of the broad phase,
+  //         [20] callStep of the step that raised the first error flag (-1: none)
   int* devInt;
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
@@ -252,6 +255,37 @@ struct StepHead {
   const int* hashSlot;
   int buf;
   int forceRebuild;
+};
+
+// Error flags in devInt[3].  Any of them halts the call: every kernel launched after the flag is up
+// returns at once (the gated broad-phase launches find their gate cleared), so the host can restore
+// the state of the call's start, grow the buffers and run the call again.
+constexpr int kErrPairs = 1;      // candidate pairs beyond pairCap
+constexpr int kErrContacts = 2;   // candidate slots beyond contactCap
+constexpr int kErrBarrier = 4;    // a grid barrier of the single-launch broad phase gave up waiting
+constexpr int kErrHash = 8;       // a history table probe found no free slot
+constexpr int kErrStepIdx = 20;   // devInt index of the failing step
+
+// Copy of the state at the start of an lsdem_step call, so that a call that halted on an error can
+// be run again from there (after the buffers grew): the bodies' state, the walls' poses, the bonds'
+// flags and the contact slots of the step before the call (their springs are the next step's
+// history; the table that finds them is rebuilt from the slots).
+struct SnapDev {
+  d3 *pos, *vel, *ang, *force, *torque, *wpos;
+  q4 *quat, *wquat;
+  int* bActive;
+  unsigned char* forceSet;
+  unsigned long long* devStat;  // kDevStats counters
+  int* meta;                    // [0] history slots kept
+  int* hNode;                   // history slots: node, info, spring
+  uint32_t* hInfo;
+  d3* hSpring;
+  long long hCap;
+  // the live contact buffer of the step before the call (1 - cur at the call's start)
+  int* prevNode;
+  uint32_t* prevInfo;
+  d3* prevSpring;
+  int prevBuf;
 };
 
 constexpr int kStatSlots = 32;

@@ -110,6 +110,19 @@ __device__ __forceinline__ int ghostKey(const Params& P, d3 p) {
 #define LSD_GATE(g) \
   if ((g) != nullptr && *(g) == 0) return
 
+// Raise error flag `bit` (kErr*): the call halts (every later kernel returns at entry).  The rebuild
+// gates of both step buffers are cleared so that gated broad-phase launches skip as well; the index
+// of the failing step is kept for the host.
+__device__ __forceinline__ void raiseError(const Params& P, int bit) {
+  atomicOr(&P.devInt[3], bit);
+  atomicMax(&P.devInt[kErrStepIdx], P.callStep);
+  atomicExch(&P.devInt[rebuildFlagIdx(0)], 0);
+  atomicExch(&P.devInt[rebuildFlagIdx(1)], 0);
+}
+// An error flag is up: this kernel has nothing to do.
+#define LSD_HALT(P) \
+  if ((P).devInt[3] != 0) return
+
 __global__ void k_fill_int(int* a, long long n, int v, const int* gate) {
   LSD_GATE(gate);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -406,6 +419,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
 
 // One thread per particle, then one per wall.
 __global__ void k_kick_drift(Params P, int forceRebuild) {
+  LSD_HALT(P);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   BodyIn in;
   d3 F, T;
@@ -483,23 +497,25 @@ __device__ void bucketCount(const int* bucket, int n, int* count, int* rank, Spa
 __device__ void bucketScatter(const int* bucket, int n, const int* start, const int* rank, int* out, Span g) {
   for (int i = g.tid; i < n; i += g.nth) {
     const int b = bucket[i];
-    if (b >= 0) out[start[b] + rank[i]] = i;
+    const int d = b >= 0 ? start[b] + rank[i] : -1;
+    if (d >= 0 && d < n) out[d] = i;  // (always in range when the counts and the scan are complete)
   }
 }
 // Buckets of up to 16 items are sorted in registers (odd-even transposition network) from
 // `tmp` into `items`; larger ones are listed for bucketSortBig (*bigCount starts at 0).
 __device__ void bucketSortSmall(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
-                                Span g) {
+                                Span g, int nItems) {
   for (int b = g.tid; b < nb; b += g.nth) {
     const int s = start[b], e = start[b + 1];
     const int n = e - s;
-    if (n == 0) continue;
+    if (n <= 0 || s < 0 || e > nItems) continue;  // (never with a complete scan of the counts)
     if (n == 1) {
       items[s] = tmp[s];
       continue;
     }
     if (n > 16) {
-      bigList[atomicAdd(bigCount, 1)] = b;
+      const int q = atomicAdd(bigCount, 1);
+      if (q < nb) bigList[q] = b;
       continue;
     }
     int v[16];
@@ -522,12 +538,13 @@ __device__ void bucketSortSmall(const int* start, int nb, const int* tmp, int* i
 // One wavefront per listed bucket: each item's place is the number of smaller items (items are
 // distinct indices).  `wave` / `nWaves`: this wavefront's index and the number of wavefronts.
 __device__ void bucketSortBig(const int* start, const int* tmp, int* items, const int* bigList, const int* bigCount,
-                              int wave, int nWaves) {
+                              int wave, int nWaves, int nb, int nItems) {
   const int lane = threadIdx.x & 63;
-  const int nBig = *bigCount;
+  const int nBig = min(*bigCount, nb);
   for (int q = wave; q < nBig; q += nWaves) {
     const int b = bigList[q];
     const int s = start[b], e = start[b + 1];
+    if (s < 0 || e > nItems || s >= e) continue;  // (never with a complete scan of the counts)
     for (int c0 = s; c0 < e; c0 += 64) {
       const int v = c0 + lane < e ? tmp[c0 + lane] : 0x7fffffff;
       int rank = 0;
@@ -550,15 +567,15 @@ __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, con
   bucketScatter(bucket, n, start, rank, out, gridSpan());
 }
 __global__ void k_bucket_sort(const int* start, int nb, const int* tmp, int* items, int* bigList, int* bigCount,
-                              const int* gate) {
+                              const int* gate, int nItems) {
   LSD_GATE(gate);
-  bucketSortSmall(start, nb, tmp, items, bigList, bigCount, gridSpan());
+  bucketSortSmall(start, nb, tmp, items, bigList, bigCount, gridSpan(), nItems);
 }
-__global__ void k_bucket_sort_big(const int* start, const int* tmp, int* items, const int* bigList,
-                                  const int* bigCount, const int* gate) {
+__global__ void k_bucket_sort_big(const int* start, int nb, const int* tmp, int* items, const int* bigList,
+                                  const int* bigCount, const int* gate, int nItems) {
   LSD_GATE(gate);
   const Span g = gridSpan();
-  bucketSortBig(start, tmp, items, bigList, bigCount, g.tid >> 6, g.nth >> 6);
+  bucketSortBig(start, tmp, items, bigList, bigCount, g.tid >> 6, g.nth >> 6, nb, nItems);
 }
 
 // Search-grid cell of a particle's post-wrap centre: ghost images are searched through their
@@ -854,8 +871,19 @@ __global__ void k_inherit_certs(Params P) {
   LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
   inheritCerts(P, gridSpan());
 }
+// The new list does not fit: raise the error (the rest of the call halts) instead of writing it.
+__device__ __forceinline__ bool pairOverflow(const Params& P) {
+  const int np = P.pairStart[P.nP];
+  if ((long long)np <= P.pairCap) return false;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicMax(&P.devInt[4], np);
+    raiseError(P, kErrPairs);
+  }
+  return true;
+}
 __global__ void k_pairs_emit(Params P) {
   LSD_GATE(P.devInt + rebuildFlagIdx(P.curBuf));
+  if (pairOverflow(P)) return;
   pairsEmit(P, gridSpan());
 }
 __global__ void k_mark_build(Params P) {
@@ -881,7 +909,8 @@ __device__ void rcScatterPairs(const Params& P, Span g) {
     const int r = P.rcRank[p];
     if (r < 0) continue;
     const int partner = int(P.pairInfo[p] & kPartnerMask);
-    P.rcTmp[P.rcStart[partner] + r] = p;
+    const int d = P.rcStart[partner] + r;
+    if (d < P.pairCap) P.rcTmp[d] = p;
   }
 }
 __global__ void k_rc_count(Params P) {
@@ -910,9 +939,11 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
 }
 // Insert a contact of this step into this step's table (open addressing, linear probing); returns
 // the slot, recorded so that the entry can be removed when the table is reused two steps later.
+// The table holds at least twice the contact capacity, so a probe finds a free slot long before it
+// has seen every slot; a probe that does see every slot raises kErrHash and gives up.
 __device__ __forceinline__ int histInsert(const Params& P, unsigned long long key, int c) {
   unsigned long long slot = mix64(key) & P.hashMask;
-  for (;;) {
+  for (unsigned long long probe = 0; probe <= P.hashMask; ++probe) {
     const unsigned long long old = atomicCAS(&P.hashKey[slot], kHashEmpty, key);
     if (old == kHashEmpty || old == key) {
       P.hashVal[slot] = c;
@@ -920,17 +951,20 @@ __device__ __forceinline__ int histInsert(const Params& P, unsigned long long ke
     }
     slot = (slot + 1) & P.hashMask;
   }
+  raiseError(P, kErrHash);
+  return -1;
 }
 
 // The previous step's spring of a contact key (zero when it had none), from the previous table.
 __device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key) {
   unsigned long long slot = mix64(key) & P.hashMask;
-  for (;;) {
+  for (unsigned long long probe = 0; probe <= P.hashMask; ++probe) {
     const unsigned long long k = P.prevHashKey[slot];
     if (k == key) return P.prevSpring[P.prevHashVal[slot]];
     if (k == kHashEmpty) return mk3(0., 0., 0.);
     slot = (slot + 1) & P.hashMask;
   }
+  return mk3(0., 0., 0.);  // (a full table: the insert that filled it raised kErrHash)
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1154,13 +1188,12 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   __shared__ uint32_t sInfo[kCheckTile];
   PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
   __shared__ int sCount, sBase;
-  if (P.devInt[3] & 4) return;  // a grid barrier of the broad phase timed out: no pair list to read
+  LSD_HALT(P);  // (a failed broad phase leaves no pair list to read)
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
   const bool rebuilt = P.devInt[rebuildFlagIdx(P.curBuf)] != 0;  // a new pair list this step
   const bool fresh = !P.useCert;  // (a new list holds the certificates it inherited)
   const int lane = threadIdx.x & 63;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long masterNodes = 0;  // nodes of the masters: what a node-by-node scan would test
   // XCD-aware tiles: workgroups go to the 8 XCDs in turn (blockIdx mod 8), so each XCD takes one
   // contiguous eighth of the pair list and its partners' data stays in that XCD's L2
   const int nTiles = (nPairs + kCheckTile - 1) / kCheckTile;
@@ -1232,8 +1265,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         }
       }
       if (p < nPairs) {
-        if (live) masterNodes += (unsigned long long)(P.nodeStart[Av[r] + 1] - P.nodeStart[Av[r]]);
-        else if (rebuilt) P.cert[p] = make_float4(-1.f, 0.f, 0.f, -1.f);
+        if (!live && rebuilt) P.cert[p] = make_float4(-1.f, 0.f, 0.f, -1.f);
         // a pair skipped under a certificate has had no contacts since the evaluation that issued
         // it (a pair with candidate nodes gets no certificate); others drop last step's count
         // (on a new list every count is the old list's)
@@ -1285,8 +1317,6 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
     }
     __syncthreads();
   }
-  const unsigned long long t = (unsigned long long)wsum(double(masterNodes));
-  if (lane == 0 && t) atomicAdd(&P.statSlots[4 * kStatSlots + ((blockIdx.x + (threadIdx.x >> 6)) & (kStatSlots - 1))], t);
 }
 
 // The FP32 prefilters may contract a * b + c into one rounding: their margins bound the rounding
@@ -1632,6 +1662,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const int q = lane >> 4, b16 = lane & 15;
   const unsigned int lt16 = (1u << b16) - 1u;
   const float kInf = __builtin_huge_valf();
+  LSD_HALT(P);
   unsigned long long statCand = 0, statEnt = 0, statQueries = 0, statContacts = 0;
   const int nAct = P.devInt[9];
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
@@ -1797,8 +1828,11 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       if (lane == 0) first = atomicAdd(&P.devInt[1], total);
       first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
       if ((long long)first + total > P.contactCap) {
-        // the step fails with a capacity error; slots reserved below the capacity stay empty
-        if (lane == 0) atomicOr(&P.devInt[3], 2);
+        // the call halts with a capacity error; slots reserved below the capacity stay empty
+        if (lane == 0) {
+          atomicMax(&P.devInt[5], first + total);
+          raiseError(P, kErrContacts);
+        }
         for (int c = first + lane; c < (int)P.contactCap; c += 64) {
           if (c >= first + total) break;
           P.cInfo[c] = kInvalidInfo;
@@ -1902,6 +1936,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
 // of pair p.  Only clusters whose centre lies in a family partner's grid box are queried, packed
 // 16 per network pass.
 __global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
+  LSD_HALT(P);
   __shared__ int sQ[kNarrowBlock / 64][64];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int nAct = P.devInt[9];
@@ -1949,6 +1984,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
 // family must lie in a cluster that the surrogate keeps; a culled one is a missed contact.
 // One lane per contact slot, 64 per wave through the network.
 __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
+  LSD_HALT(P);
   __shared__ int sQ[kNarrowBlock / 64][64];
   const int lane = threadIdx.x & 63;
   const int n = min(P.devInt[1], (int)P.contactCap);
@@ -1990,6 +2026,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_audit(Params P) {
 // Bonds (one thread per bond)
 // ------------------------------------------------------------------------------------------
 __global__ void k_bonds(Params P) {
+  LSD_HALT(P);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.nBonds) return;
   double* out = P.bOut + 6ll * k;
@@ -2092,6 +2129,7 @@ __device__ __forceinline__ bool reduceBody(const Params& P, int b, d3& F, d3& T)
 
 // One thread per body.  Thread 0 also closes the step's bookkeeping.
 __global__ void k_reduce_kick(Params P, int kick) {
+  LSD_HALT(P);
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0) stepEnd(P);
   if (b >= P.nP) return;
@@ -2108,6 +2146,7 @@ __global__ void k_reduce_kick(Params P, int kick) {
 // wall).  H describes the next step; P is this step's.  Nothing here reads another body's state,
 // and this step's flags and counters are closed by thread 0 before it clears the next step's.
 __global__ void k_reduce_drift(Params P, StepHead H) {
+  LSD_HALT(P);
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   BodyIn in;
   bool set = false;
@@ -2135,13 +2174,88 @@ __global__ void k_reduce_drift(Params P, StepHead H) {
 // Close of the pre-loop build: step bookkeeping, and the contact flags of the candidate kernel
 // cleared (no sums follow; the stored forces stay those of the last step).
 __global__ void k_prime_end(Params P) {
+  LSD_HALT(P);
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0) stepEnd(P);
   if (b < P.nP) P.bodyHit[b] = 0;
 }
 
+// ------------------------------------------------------------------------------------------
+// Call snapshot (restart of a call that halted on an error)
+// ------------------------------------------------------------------------------------------
+__global__ void k_snap_save(Params P, SnapDev S) {
+  const Span g = gridSpan();
+  for (int i = g.tid; i < P.nP; i += g.nth) {
+    S.pos[i] = P.pos[i]; S.vel[i] = P.vel[i]; S.ang[i] = P.angvel[i];
+    S.force[i] = P.force[i]; S.torque[i] = P.torque[i]; S.quat[i] = P.quat[i]; S.forceSet[i] = P.forceSet[i];
+  }
+  for (int i = g.tid; i < P.nW; i += g.nth) { S.wpos[i] = P.wpos[i]; S.wquat[i] = P.wquat[i]; }
+  for (int i = g.tid; i < P.nBonds; i += g.nth) S.bActive[i] = P.bActive[i];
+  const int n = int(min((long long)P.devInt[10 + S.prevBuf], S.hCap));
+  for (int c = g.tid; c < n; c += g.nth) {
+    S.hNode[c] = S.prevNode[c]; S.hInfo[c] = S.prevInfo[c]; S.hSpring[c] = S.prevSpring[c];
+  }
+  if (g.tid == 0) S.meta[0] = n;
+  if (g.tid < kDevStats) S.devStat[g.tid] = P.devStat[g.tid];
+}
+
+// Back to the call's start; the history tables must be empty (filled by the host before).  The
+// previous pair list is dropped (no certificate is inherited from it) and the step flags cleared.
+__global__ void k_snap_restore(Params P, SnapDev S) {
+  const Span g = gridSpan();
+  for (int i = g.tid; i < P.nP; i += g.nth) {
+    P.pos[i] = S.pos[i]; P.posDet[i] = S.pos[i]; P.vel[i] = S.vel[i]; P.angvel[i] = S.ang[i];
+    P.force[i] = S.force[i]; P.torque[i] = S.torque[i]; P.quat[i] = S.quat[i]; P.forceSet[i] = S.forceSet[i];
+    P.bodyHit[i] = 0;
+  }
+  for (int i = g.tid; i <= P.nP; i += g.nth) P.pairStart[i] = 0;
+  for (int i = g.tid; i < P.nW; i += g.nth) { P.wpos[i] = S.wpos[i]; P.wquat[i] = S.wquat[i]; }
+  for (int i = g.tid; i < P.nBonds; i += g.nth) P.bActive[i] = S.bActive[i];
+  const int n = S.meta[0];
+  for (int c = g.tid; c < n; c += g.nth) {
+    S.prevNode[c] = S.hNode[c]; S.prevInfo[c] = S.hInfo[c]; S.prevSpring[c] = S.hSpring[c];
+  }
+  if (g.tid < kDevStats) P.devStat[g.tid] = S.devStat[g.tid];
+  if (g.tid == 0) {
+    P.devInt[10 + S.prevBuf] = n;  // the restored slots, cleared from their table two steps on
+    P.devInt[11 - S.prevBuf] = 0;  // the other table is empty
+    P.devInt[1] = 0; P.devInt[9] = 0; P.devInt[12] = 0;
+    P.devInt[rebuildFlagIdx(0)] = 0; P.devInt[rebuildFlagIdx(1)] = 0;
+    P.devInt[ghostCountIdx(0)] = 0; P.devInt[ghostCountIdx(1)] = 0;
+    P.devInt[3] = 0;
+    P.devInt[kErrStepIdx] = -1;
+  }
+  for (int k = g.tid; k < 5 * kStatSlots; k += g.nth) P.statSlots[k] = 0;
+}
+
+// The history table of the restored slots: each contact slot's key inserted again (the table was
+// emptied first), its table slot recorded for the clearing two steps on.
+__global__ void k_hist_reinsert(Params P, SnapDev S, unsigned long long* key, int* val, int* slotOf) {
+  const Span g = gridSpan();
+  const int n = S.meta[0];
+  for (int c = g.tid; c < n; c += g.nth) {
+    const uint32_t info = S.prevInfo[c];
+    int at = -1;
+    if (info != kInvalidInfo) {
+      const unsigned long long k = histKey(info >> 28, S.prevNode[c], int(info & kPartnerMask));
+      unsigned long long slot = mix64(k) & P.hashMask;
+      for (unsigned long long probe = 0; probe <= P.hashMask; ++probe) {
+        const unsigned long long old = atomicCAS(&key[slot], kHashEmpty, k);
+        if (old == kHashEmpty || old == k) {
+          val[slot] = c;
+          at = int(slot);
+          break;
+        }
+        slot = (slot + 1) & P.hashMask;
+      }
+    }
+    slotOf[c] = at;
+  }
+}
+
 // Second half-kick from the stored sums (after an external-force hook has added to them).
 __global__ void k_kick(Params P) {
+  LSD_HALT(P);
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= P.nP) return;
   halfKick(P, b, P.force[b], P.torque[b]);
@@ -2154,8 +2268,7 @@ __global__ void k_kick(Params P) {
 // cleared for the next step.
 __device__ void stepEnd(const Params& P) {
   {
-    const int np = P.pairStart[P.nP];
-    if ((long long)np > P.pairCap) P.devInt[3] |= 1;
+    const int np = P.pairStart[P.nP];  // (an overflowing list halted the call in the broad phase)
     P.devInt[4] = max(P.devInt[4], np);
     P.devInt[5] = max(P.devInt[5], P.devInt[1]);
     P.devInt[10 + P.curBuf] = min(P.devInt[1], (int)P.contactCap);  // slots to clear two steps on
@@ -2198,40 +2311,47 @@ __device__ void stepEnd(const Params& P) {
 constexpr int kBroadThreads = 1024;
 
 // Grid barrier on the counter devInt[12] (cleared by the drift kernel every step): each block
-// adds one arrival per barrier and waits for nBlocks x (barriers passed).  The wait is bounded:
-// after about a second it sets overflow flag 4 and lets the block go on (the step then reports
-// an error instead of hanging).
+// adds one arrival per barrier and waits for nBlocks x (barriers passed).  The wait is bounded
+// (P.barrierSpin polls, about a second by default): a block that gives up raises kErrBarrier.  A
+// block also stops waiting as soon as any error flag is up, and after every barrier all blocks
+// read the flags: sync() returns true when the launch must be abandoned, so every block leaves
+// at the same barrier and no phase runs on the incomplete results of an earlier one.
 struct GridBarrier {
-  int* devInt;
+  const Params& P;
   int nBlocks, passed;
-  __device__ void sync() {
+  __device__ bool sync() {
+    __shared__ int abandon;
     __syncthreads();
     if (threadIdx.x == 0) {
       ++passed;
       const int target = passed * nBlocks;
-      int* count = devInt + 12;
+      int* count = P.devInt + 12;
       __threadfence();  // this block's writes, before the arrival
       atomicAdd(count, 1);
       // relaxed polling (an acquire per poll would invalidate the caches every time), then one
       // acquire fence
       for (unsigned int it = 0;
            __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
-        if (it > (1u << 21)) {
-          atomicOr(devInt + 3, 4);
+        if (__hip_atomic_load(P.devInt + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (it >= unsigned(P.barrierSpin)) {
+          raiseError(P, kErrBarrier);
           break;
         }
         __builtin_amdgcn_s_sleep(20);  // ~1300 cycles: few polls of the shared counter line
       }
       __threadfence();  // the other blocks' writes, after the wait
+      abandon = __hip_atomic_load(P.devInt + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
     }
     __syncthreads();
+    return abandon != 0;
   }
 };
 
 __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
+  LSD_HALT(P);
   if (P.devInt[rebuildFlagIdx(P.curBuf)] == 0) return;  // written by the drift kernel before this launch: grid-uniform
   __shared__ int lds[kScanLds];
-  GridBarrier grid{P.devInt, int(gridDim.x), 0};
+  GridBarrier grid{P, int(gridDim.x), 0};
   const Span g = gridSpan();
   const int wave = g.tid >> 6, nWaves = g.nth >> 6;
   const int ncell = P.sg.ncell;
@@ -2242,48 +2362,51 @@ __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
     P.scell[i] = searchCellOf(P, i);
   }
   if (g.tid == 0) P.devInt[2] = 0;  // large-bucket list
-  grid.sync();
+  if (grid.sync()) return;
   bucketCount(P.scell, P.nP, P.cellCount, P.rank, g);
-  grid.sync();
+  if (grid.sync()) return;
   scanReduceTiles(P.cellCount, ncell, P.scanTiles, lds);
-  grid.sync();
+  if (grid.sync()) return;
   scanApplyTiles(P.cellCount, ncell, P.scanTiles, P.cellStart, lds);
-  grid.sync();
+  if (grid.sync()) return;
   bucketScatter(P.scell, P.nP, P.cellStart, P.rank, P.sortTmp, g);
-  grid.sync();
-  bucketSortSmall(P.cellStart, ncell, P.sortTmp, P.sorted, P.sortBig, P.devInt + 2, g);
-  grid.sync();
-  bucketSortBig(P.cellStart, P.sortTmp, P.sorted, P.sortBig, P.devInt + 2, wave, nWaves);
-  grid.sync();
+  if (grid.sync()) return;
+  bucketSortSmall(P.cellStart, ncell, P.sortTmp, P.sorted, P.sortBig, P.devInt + 2, g, P.nP);
+  if (grid.sync()) return;
+  bucketSortBig(P.cellStart, P.sortTmp, P.sorted, P.sortBig, P.devInt + 2, wave, nWaves, ncell, P.nP);
+  if (grid.sync()) return;
   // particle data in sorted order; candidate pairs per master, prefix, pair list
   for (int k = g.tid; k < P.nP; k += g.nth) gatherParticle(P, k);
   backupPairs(P, g);
   if (g.tid == 0) P.devInt[2] = 0;  // large-bucket list of the reaction sort
-  grid.sync();
+  if (grid.sync()) return;
   pairsFind(P, g);
-  grid.sync();
+  if (grid.sync()) return;
   scanReduceTiles(P.pairCount, P.nP, P.scanTiles, lds);
-  grid.sync();
+  if (grid.sync()) return;
   scanApplyTiles(P.pairCount, P.nP, P.scanTiles, P.pairStart, lds);
-  grid.sync();
+  if (grid.sync()) return;
+  // every block reads the same total after the barrier: all leave together when it does not fit
+  if (pairOverflow(P)) return;
   pairsEmit(P, g);
-  grid.sync();
+  if (grid.sync()) return;
   inheritCerts(P, g);  // reads the motion records before they restart
-  grid.sync();
+  if (grid.sync()) return;
   markBuild(P, g);
-  grid.sync();
+  if (grid.sync()) return;
   // reaction CSR: pairs by partner, in pair order
   rcCountPairs(P, g);
-  grid.sync();
+  if (grid.sync()) return;
   scanReduceTiles(P.rcCount, P.nP, P.scanTiles, lds);
-  grid.sync();
+  if (grid.sync()) return;
   scanApplyTiles(P.rcCount, P.nP, P.scanTiles, P.rcStart, lds);
-  grid.sync();
+  if (grid.sync()) return;
   rcScatterPairs(P, g);
-  grid.sync();
-  bucketSortSmall(P.rcStart, P.nP, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, g);
-  grid.sync();
-  bucketSortBig(P.rcStart, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, wave, nWaves);
+  if (grid.sync()) return;
+  const int nRc = min(P.rcStart[P.nP], int(P.pairCap));
+  bucketSortSmall(P.rcStart, P.nP, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, g, nRc);
+  if (grid.sync()) return;
+  bucketSortBig(P.rcStart, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, wave, nWaves, P.nP, nRc);
 }
 
 template __global__ void k_candidates<false>(Params P);

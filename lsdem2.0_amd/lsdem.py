@@ -45,7 +45,9 @@ class Config(C.Structure):
         ("broad_phase_blocks", C.c_int32),
         ("narrow_batch", C.c_int32),
         ("no_fused_steps", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("no_recovery", C.c_int32),
+        ("barrier_spin", C.c_int32),
+        ("broad_phase_coop", C.c_int32),
     ]
 
 
@@ -79,7 +81,8 @@ class Counters(C.Structure):
         ("contacts", C.c_int64),
         ("pair_capacity", C.c_int64),
         ("contact_capacity", C.c_int64),
-        ("reserved", C.c_int64 * 9),
+        ("recoveries", C.c_int64),
+        ("reserved", C.c_int64 * 8),
     ]
 
 
@@ -178,6 +181,9 @@ class Sim:
         c.broad_phase_blocks = int(cfg.get("broad_phase_blocks", 0))
         c.narrow_batch = int(cfg.get("narrow_batch", 0))
         c.no_fused_steps = int(cfg.get("no_fused_steps", 0))
+        c.no_recovery = int(cfg.get("no_recovery", 0))
+        c.barrier_spin = int(cfg.get("barrier_spin", 0))
+        c.broad_phase_coop = int(cfg.get("broad_phase_coop", 0))
         c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()
