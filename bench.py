@@ -15,14 +15,20 @@ Surrogate path: a second context from the same initial state trains the culling 
 GPU (not timed in the step rate; reported), then runs W + K steps with the network culling node
 clusters and an exact audit every 100 steps (audits are inside the timed steps).
 
-Roofline (the narrow gather phase: k_pair_check + k_candidates, one launch each per step).  Algorithmic bytes per step, 4 B per FP32/int32 scalar:
-    P x 18 x 4   candidate pairs (pair record, both poses, map)
-  + Q x 3 x 4    boundary nodes of the masters (a node-by-node scan reads every one)
-  + S x 8 x 4    SDF queries (8 corners)
-  + C x 16 x 4   contacts written and history probes
-with P, Q, S, C counted by the kernels.  achieved = bytes / narrow-phase time; peak 8 TB/s.
-The CPU baseline is the oracle (serial C++, FP64, 1 core) on a bounded sample of the same workload:
-the same generator at fewer particles with the periodic cell shrunk to keep the density.
+Roofline.  Algorithmic bytes per step, 4 B per FP32/int32 scalar, with P candidate pairs, Q the
+boundary nodes of the masters of the evaluated pairs (what a node-by-node scan of those pairs
+reads), S exact SDF queries, C contacts and N particles, all counted by the kernels:
+    k_pair_check     P x 18 x 4                         (pair record, both poses, map)
+    k_candidates     Q x 3 x 4 + S x 8 x 4 + C x 16 x 4  (nodes, 8 corners, contact + history)
+                     + C x 45 x 4                        (contact law: record, both bodies' state)
+    k_reduce_drift   N x (78 + 6) x 4                    (kick + drift + kick, force/torque sums)
+achieved = bytes / that kernel's time (HIP events on the library's stream on every 10th timed
+step); peak 8 TB/s.  `roofline` is the kernel with the largest time; `kernels` lists all three and
+`narrow_phase` the pair check and candidate kernel together.
+CPU baseline: the oracle (C++, FP64, pair broad phase with identical contact sets) on the GPU
+run's own state at the end of the timed steps (all 200k particles), one untimed step for the
+pre-loop build and history, then K steps with OpenMP over all host cores (the contract's value)
+and one step on one core.
 """
 from __future__ import annotations
 
@@ -43,10 +49,12 @@ B_PAIR = 18 * 4
 B_NODE = 3 * 4
 B_QUERY = 8 * 4
 B_CONTACT = 16 * 4
+B_LAW = 45 * 4
+B_BODY = (78 + 6) * 4
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per step of the narrow phase, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a
-# 256 MiB copy (profiles/c2_200k_narrow_pmc_traffic.md); None when the kernels changed since.
-NARROW_TRAFFIC_BYTES = 84.3e6
+# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a plain copy
+# (profiles/c2_200k_pmc_traffic.md); None for a kernel not measured at its current code.
+KERNEL_TRAFFIC_BYTES = {"k_pair_check": None, "k_candidates": None, "k_reduce_drift": None}
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 PROFILE_EVERY = 10
 
@@ -55,27 +63,86 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def narrow_bytes(prof: dict) -> float:
-    return (B_PAIR * prof["pairs"] + B_NODE * prof["master_nodes"] + B_QUERY * prof["sdf_queries"] +
-            B_CONTACT * prof["contacts"])
+def kernel_bytes(prof: dict, n: int, K: int) -> dict:
+    """Algorithmic bytes per step of the three per-step kernels (counts averaged over K steps)."""
+    P, Q = prof["pairs"] / K, prof["master_nodes"] / K
+    S, C = prof["sdf_queries"] / K, prof["contacts"] / K
+    return {"k_pair_check": B_PAIR * P,
+            "k_candidates": B_NODE * Q + B_QUERY * S + B_CONTACT * C + B_LAW * C,
+            "k_reduce_drift": B_BODY * n}
 
 
-def cpu_baseline(L, S, n_full: int, n_sample: int, steps: int, seed: int, gap: float):
+def rooflines(prof: dict, n: int) -> dict:
+    """Per-kernel rooflines from the sampled phase times (pair check: phase pair_check; node culls,
+    exact test and contact law: candidates; body pass: reduce)."""
+    K = max(prof["steps"], 1)
+    KS = max(prof["sampled_steps"], 1)
+    ms = {"k_pair_check": prof["ms"]["pair_check"] / KS, "k_candidates": prof["ms"]["candidates"] / KS,
+          "k_reduce_drift": prof["ms"]["reduce"] / KS}
+    by = kernel_bytes(prof, n, K)
+    out = {}
+    for k in ms:
+        a = by[k] / (ms[k] * 1e-3) / 1e9 if ms[k] > 0 else 0.0
+        out[k] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                  "bytes_per_launch": by[k], "ms_per_launch": ms[k], "traffic": KERNEL_TRAFFIC_BYTES[k]}
+    nb = by["k_pair_check"] + by["k_candidates"]
+    nms = ms["k_pair_check"] + ms["k_candidates"]
+    na = nb / (nms * 1e-3) / 1e9 if nms > 0 else 0.0
+    tr = [KERNEL_TRAFFIC_BYTES[k] for k in ("k_pair_check", "k_candidates")]
+    narrow = {"bound": "hbm", "achieved": na, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": na / HBM_PEAK_GBS,
+              "bytes_per_step": nb, "ms_per_step": nms, "traffic": None if None in tr else sum(tr)}
+    return {"kernels": out, "narrow_phase": narrow}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+
+
+def cpu_baseline(sc, state: dict, omp_steps: int, serial_steps: int):
+    """The oracle on the GPU run's state (all particles): one untimed step (pre-loop build and the
+    history it starts), then omp_steps steps on all host cores and serial_steps on one."""
+    import ctypes as C
     sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_binding import oracle_sim  # the checker, timed here as the CPU baseline
-    cell = 0.3 * float(np.sqrt(n_sample / n_full))
-    sc = S.scene_c2(n=n_sample, cell_xy=cell, seed=seed, gap=gap)
-    if gap < 0:  # set-up on the GPU library (not timed); only the oracle's steps are timed
-        S.remove_initial_overlaps(lambda c: L.Sim(c), sc, log=lambda m: None)
-    sim = sc.load(oracle_sim(sc.cfg))
-    sim.step(1)  # first step builds the history; time the following ones
-    t0 = time.perf_counter()
-    sim.step(steps)
-    dt = time.perf_counter() - t0
-    return {"value": n_sample * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++, FP64, pair broad phase), same generator at {n_sample} particles in a "
-                      f"{cell:.4f} m periodic cell (same density), {steps} steps after 1 untimed step, {dt:.1f} s",
-            "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+    from oracle_binding import oracle_sim, oracle_lib  # the checker, timed here as the CPU baseline
+    cores = host_cores()
+    n = sc.n_particles
+    sim = sc.load(oracle_sim(sc.cfg, threads=cores))
+    sim.set_particles(**state)
+    t = time.perf_counter()
+    sim.step(1)
+    first = time.perf_counter() - t
+    t = time.perf_counter()
+    sim.step(omp_steps)
+    t_omp = time.perf_counter() - t
+    serial = None
+    if serial_steps > 0:
+        oracle_lib().lsdem_oracle_set_threads(sim.ctx, C.c_int(1))
+        t = time.perf_counter()
+        sim.step(serial_steps)
+        t_ser = time.perf_counter() - t
+        serial = {"value": n * serial_steps / t_ser, "unit": "particle-steps/s", "cores": 1, "steps": serial_steps,
+                  "seconds": t_ser}
+    sim.close()
+    return {"value": n * omp_steps / t_omp, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (C++, FP64, pair broad phase: the reference's contact sets), OpenMP over master "
+                      f"particles on {cores} threads, the GPU run's state after its timed steps ({n} particles), "
+                      f"{omp_steps} steps in {t_omp:.1f} s after 1 untimed step ({first:.1f} s)",
+            "serial": serial, "cpu": cpu_model(), "nproc": os.cpu_count()}
 
 
 def run(L, sc, caps, warmup, steps, label, surrogate=None):
@@ -229,8 +296,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5000)
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--seed", type=int, default=20261204)
-    ap.add_argument("--cpu-sample", type=int, default=5_000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps timed on all host cores")
+    ap.add_argument("--cpu-serial-steps", type=int, default=1, help="oracle steps timed on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sur-iters", type=int, default=3000)
@@ -247,9 +314,11 @@ def main():
                     help="with --drums: each drum's broad phase as one launch of B blocks (0: the library's choice, "
                          "8 blocks at these sizes; grid barriers, so all drums' blocks must fit on the GPU "
                          "together), or as separate launches (-1)")
-    ap.add_argument("--drum-legs", default="concurrent,graph,sequential",
+    ap.add_argument("--drum-legs", default="concurrent,sequential",
                     help="with --drums: which runs besides the concurrent one to make (graph: concurrent replays of "
-                         "captured HIP graphs; sequential: one drum after another)")
+                         "captured HIP graphs, one host thread per drum, not run by default: a host-side fault was "
+                         "once seen in that mode under a profiler's kernel trace and its cause is not established; "
+                         "sequential: one drum after another)")
     ap.add_argument("--autotune", action="store_true",
                     help="with --drums: tune block size, narrow batch and skin per size bucket, then run again")
     ap.add_argument("--tune-edges", type=int, nargs="*", default=[450],
@@ -292,16 +361,19 @@ def main():
     value = args.n * args.steps / wall
     K = max(prof["steps"], 1)
     KS = max(prof["sampled_steps"], 1)  # steps whose phases were timed
-    narrow_ms = prof["ms"]["narrow"] / KS
-    per_step_bytes = narrow_bytes(prof) / K
-    achieved = per_step_bytes / (narrow_ms * 1e-3) / 1e9
+    roof = rooflines(prof, args.n)
     gpu_ms = sum(prof["ms"].values()) / KS
-    log(f"[bench] plain: {args.steps} steps in {wall:.2f}s ({ms_step:.3f} ms/step, GPU phases {gpu_ms:.3f} ms/step)")
+    log(f"[bench] plain: {args.steps} steps in {wall:.2f}s ({ms_step:.4f} ms/step, GPU phases {gpu_ms:.4f} ms/step)")
     log("[bench] phases ms/step: " + " ".join(f"{k}={v / KS:.4f}" for k, v in prof["ms"].items()))
     log(f"[bench] per step: pairs={prof['pairs'] / K:.0f} master_nodes={prof['master_nodes'] / K:.0f} "
         f"cluster_tests={prof['cluster_tests'] / K:.0f} node_tests={prof['node_tests'] / K:.0f} "
         f"sdf_queries={prof['sdf_queries'] / K:.0f} contacts={prof['contacts'] / K:.0f} "
         f"evaluated_pairs={prof['evaluated_pairs'] / K:.0f} rebuilds={prof['rebuilds']}")
+    log("[bench] rooflines: " + " ".join(f"{k}={v['frac']:.3f}" for k, v in roof["kernels"].items()) +
+        f" narrow={roof['narrow_phase']['frac']:.3f}")
+    state = {k: v for k, v in sim.get_particles().items()
+             if k in ("pos", "quat", "vel", "angvel", "force", "torque")}
+    sim.close()
     del sim
 
     sur = None
@@ -313,7 +385,7 @@ def main():
         rep = extra["report"]
         sur = {"value": args.n * args.steps / swall, "ms_per_step": 1e3 * swall / args.steps,
                "speedup_vs_plain": (args.n * args.steps / swall) / value,
-               "narrow_ms_per_step": sprof["ms"]["narrow"] / sKS,
+               "narrow_ms_per_step": (sprof["ms"]["pair_check"] + sprof["ms"]["candidates"]) / sKS,
                "cluster_tests_per_step": sprof["cluster_tests"] / sK,
                "evaluated_pairs_per_step": sprof["evaluated_pairs"] / sK,
                "plain_evaluated_pairs_per_step": prof["evaluated_pairs"] / K,
@@ -325,13 +397,16 @@ def main():
         log(f"[bench] surrogate: {sur['ms_per_step']:.3f} ms/step ({sur['speedup_vs_plain']:.2f}x plain), "
             f"audits {rep['audits']} contacts {rep['audited_contacts']} missed {rep['missed_contacts']}")
         log("[bench] surrogate phases ms/step: " + " ".join(f"{k}={v / sKS:.4f}" for k, v in sprof["ms"].items()))
+        ssim.close()
         del ssim
 
     base = None
     if not args.no_cpu:
         t = time.perf_counter()
-        base = cpu_baseline(L, S, args.n, args.cpu_sample, args.cpu_steps, args.seed, args.gap)
-        log(f"[bench] cpu baseline {base['value']:.1f} particle-steps/s ({time.perf_counter() - t:.1f}s)")
+        base = cpu_baseline(sc, state, args.cpu_steps, args.cpu_serial_steps)
+        log(f"[bench] cpu baseline {base['value']:.1f} particle-steps/s on {base['cores']} cores, serial "
+            f"{base['serial']['value'] if base['serial'] else float('nan'):.1f} ({time.perf_counter() - t:.1f}s)")
+    top = max(roof["kernels"], key=lambda k: roof["kernels"][k]["ms_per_launch"])
     out = {
         "metric": METRIC,
         "value": value,
@@ -348,10 +423,9 @@ def main():
         "config": {"workload": "200k superquadrics (10 types, SDF 50^3 FP32, 512 nodes), periodic x/y 0.3 m "
                                "+ floor, dt 5e-6; value = plain path", "particles": args.n, "seed": args.seed,
                    "lattice_gap": args.gap},
-        "roofline": {"bound": "hbm", "kernel": "narrow phase (k_pair_check + k_candidates)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": NARROW_TRAFFIC_BYTES, "bytes_per_launch": per_step_bytes, "ms_per_launch": narrow_ms,
-                     "traffic_source": "profiles/c2_200k_narrow_pmc_traffic.md"},
+        "roofline": {"kernel": top, **roof["kernels"][top]},
+        "kernels": roof["kernels"],
+        "narrow_phase": roof["narrow_phase"],
         "cpu_baseline": base,
         "surrogate": sur,
         "phases_ms_per_step": {k: v / KS for k, v in prof["ms"].items()},

@@ -190,9 +190,10 @@ int lsdem_get_counters(lsdem_ctx* ctx, lsdem_counters* out);
 
 /* Per-phase timing of lsdem_step with HIP events recorded on the context's stream.  Phases:
  * 0 integrate (half-kick + drift + walls), 1 broad-phase sort, 2 candidate pairs,
- * 3 reaction lists, 4 narrow phase + contact law, 5 history table, 6 bonds,
- * 7 reduction + second half-kick.  With the single-launch broad phase (the default) phases 1-3
- * are one launch, reported under 1; the history table is filled inside phase 4.  When consecutive
+ * 3 reaction lists, 4 pair check (liveness, separation certificates, map records of the
+ * evaluated pairs), 5 node culls + exact test + contact law + history table (and the surrogate's
+ * masks and audits), 6 bonds, 7 reduction + second half-kick.  With the single-launch broad phase
+ * (the default) phases 1-3 are one launch, reported under 1.  When consecutive
  * steps of one call share a launch at the step boundary (the default), the next step's kick and
  * drift are reported under 7 and phase 0 holds only a call's first step.  Work counters are summed
  * over the profiled steps.
@@ -203,7 +204,7 @@ typedef struct lsdem_profile {
   double ms[LSDEM_NUM_PHASES]; /* summed GPU time per phase */
   int64_t pairs, node_tests, sdf_queries, contacts; /* sums over the profiled steps */
   int64_t cluster_tests;       /* node clusters passed to the node prefilter, summed */
-  int64_t master_nodes;        /* boundary nodes of the masters of all candidate pairs, summed */
+  int64_t master_nodes;        /* boundary nodes of the masters of the evaluated pairs, summed */
   int64_t evaluated_pairs;     /* pairs the narrow phase evaluated (the rest held a separation
                                   certificate or had no partner), summed */
   int64_t rebuilds;            /* steps that rebuilt the candidate pairs */

@@ -576,6 +576,7 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext, int callStep) {
   const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
   // (a multiple of 8 workgroups: the kernel gives each XCD an eighth of the pairs)
   hipLaunchKernelGGL(k_pair_check, dim3(unsigned((c->numCU * 4 + 7) / 8 * 8)), dim3(256), 0, s, P, audit ? 1 : 0);
+  phase();
   if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
   if (useSur && !audit)
     hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
@@ -588,10 +589,7 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext, int callStep) {
     hipLaunchKernelGGL(k_sur_audit, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P);
     c->sur.audits++;
   }
-  phase();
-  // (history: this step's contacts went into this step's table inside k_candidates; the phase
-  // stays in the profile layout)
-  phase(false);
+  phase();  // (this step's contacts went into this step's history table inside k_candidates)
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
   phase(P.nBonds > 0);
   if (c->hookFn) {

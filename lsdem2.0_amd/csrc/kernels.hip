@@ -1199,6 +1199,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   const int nTiles = (nPairs + kCheckTile - 1) / kCheckTile;
   const int xcd = int(blockIdx.x) & 7, perX = (int(gridDim.x) - xcd + 7) >> 3;  // blocks on this XCD
   const int tLo = int((long long)nTiles * xcd / 8), tHi = int((long long)nTiles * (xcd + 1) / 8);
+  unsigned int evNodes = 0;  // boundary nodes of the masters of this thread's evaluated pairs
   for (int t = tLo + (int(blockIdx.x) >> 3); t < tHi; t += perX) {
     const int t0 = t * kCheckTile;
     if (threadIdx.x == 0) sCount = 0;
@@ -1311,12 +1312,15 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       makeF32(P, v, f);
       out[a] = f;
       P.pairRec[p] = a;
+      evNodes += unsigned(P.nodeStart[v.A + 1] - v.nodeBase);
       P.certE[p] = make_float4(sa.motion.x - mB.x, sa.motion.y - mB.y, sa.motion.z - mB.z, flip);
       P.certQA[p] = qA;
       P.certQB[p] = qB;
     }
     __syncthreads();
   }
+  // (only threads that wrote map records count: a few thousand atomics per step at 200k particles)
+  if (evNodes) atomicAdd(&P.statSlots[4 * kStatSlots + (blockIdx.x & (kStatSlots - 1))], (unsigned long long)evNodes);
 }
 
 // The FP32 prefilters may contract a * b + c into one rounding: their margins bound the rounding

@@ -17,6 +17,9 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liblsdem.so"
 
 LSDEM_OK = 0
+LSDEM_ERR_ARG = 1
+LSDEM_ERR_STATE = 2
+LSDEM_ERR_HIP = 3
 LSDEM_ERR_CAPACITY = 4
 QUIRK_INT_X_GRADIENT = 1
 QUIRK_GHOST_Z_VS_MAXY = 2
@@ -87,7 +90,7 @@ class Counters(C.Structure):
 
 
 NUM_PHASES = 8
-PHASES = ("integrate", "sort", "pairs", "reaction", "narrow", "history", "bonds", "reduce")
+PHASES = ("integrate", "sort", "pairs", "reaction", "pair_check", "candidates", "bonds", "reduce")
 
 
 class Profile(C.Structure):

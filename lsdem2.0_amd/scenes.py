@@ -154,6 +154,7 @@ def _add_shape(shapes, sdf, origin, spacing, nodes, density=None):
         s["origin"] = s["origin"] - c
         s["nodes"] = s["nodes"] - c
         s["mass"], s["inertia"], s["inv_inertia"] = m, I, Ii
+        s["density"] = float(density)
     shapes.append(s)
     return len(shapes) - 1
 
@@ -520,6 +521,61 @@ def sdf_trilinear(shape: dict, p: np.ndarray) -> np.ndarray:
         out += w * sdf[i0[:, 2] + dz, i0[:, 1] + dy, i0[:, 0] + dx]
     out[~ok] = np.inf
     return out
+
+
+def node_owner(sc: Scene, nodes: np.ndarray) -> np.ndarray:
+    """Particle owning each boundary node index (nodes are numbered particle by particle)."""
+    node0 = np.concatenate([[0], np.cumsum([len(sc.shapes[s]["nodes"]) for s in sc.particles["shape"]])])
+    return np.searchsorted(node0, nodes, side="right") - 1
+
+
+def macro_stats(sc: Scene, parts: dict, lists: list, z_band=(0.1, 0.6)) -> dict:
+    """Macroscopic state of a simulation, the quantities long runs are compared on (trajectories
+    of DEM diverge chaotically, these do not):
+    - solid fraction: volume of the particles whose centres lie in the height band
+      [z_band[0] H, z_band[1] H] above the domain floor, over the band's volume; H is the 95th
+      percentile of the centre heights and the band spans the domain's x-y extent;
+    - coordination number: distinct contacting particle pairs (particle-particle and periodic-image
+      lists), counted once for each of the two particles, per particle; wall contacts apart;
+    - kinetic energy: translational 1/2 m v^2 plus rotational 1/2 w.(R I R^T) w.
+    `parts` is get_particles(); `lists` the five contact lists (contacts(l) for l in 0..4)."""
+    P = sc.particles
+    cfg = sc.cfg
+    lo, hi = np.asarray(cfg["domain_min"], float), np.asarray(cfg["domain_max"], float)
+    m = 1.0 / P["inv_mass"]
+    vol = m / np.array([sc.shapes[t]["density"] for t in P["shape"]])
+    z = parts["pos"][:, 2] - lo[2]
+    H = float(np.percentile(z, 95))
+    z0, z1 = z_band[0] * H, z_band[1] * H
+    inband = (z >= z0) & (z < z1)
+    area = (hi[0] - lo[0]) * (hi[1] - lo[1])
+    phi = float(vol[inband].sum() / (area * (z1 - z0))) if z1 > z0 else 0.0
+    n = len(m)
+    pairs = set()
+    for l in (0, 2, 3, 4):
+        c = lists[l]
+        if len(c["master"]) == 0:
+            continue
+        a = node_owner(sc, c["master"])
+        b = c["slave"].astype(np.int64)
+        pairs.update(zip(np.minimum(a, b).tolist(), np.maximum(a, b).tolist()))
+    z_pp = 2.0 * sum(1 for a, b in pairs if a != b) / n
+    wall = set()
+    c = lists[1]
+    if len(c["master"]):
+        wall.update(zip(node_owner(sc, c["master"]).tolist(), c["slave"].tolist()))
+    v, w, q = parts["vel"], parts["angvel"], parts["quat"]
+    ke_t = 0.5 * float(np.sum(m * np.sum(v * v, axis=1)))
+    Ii = P["inv_inertia"]
+    I = np.array([inv_sym(x) for x in Ii])  # local inertia (xx, yy, zz, xy, xz, yz)
+    wl = quat_rotate(np.concatenate([q[:, :1], -q[:, 1:]], axis=1), w)  # body frame
+    Iw = np.stack([I[:, 0] * wl[:, 0] + I[:, 3] * wl[:, 1] + I[:, 4] * wl[:, 2],
+                   I[:, 3] * wl[:, 0] + I[:, 1] * wl[:, 1] + I[:, 5] * wl[:, 2],
+                   I[:, 4] * wl[:, 0] + I[:, 5] * wl[:, 1] + I[:, 2] * wl[:, 2]], axis=1)
+    ke_r = 0.5 * float(np.sum(wl * Iw))
+    return {"solid_fraction": phi, "coordination": z_pp, "wall_contacts_per_particle": len(wall) / n,
+            "kinetic_energy": ke_t + ke_r, "ke_translational": ke_t, "ke_rotational": ke_r,
+            "bed_height": H, "band": (z0, z1)}
 
 
 def surface_gap(sc: Scene, i: int, j: int):

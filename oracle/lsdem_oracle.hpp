@@ -355,7 +355,7 @@ class Solver {
 
   // Node-vs-SDF test (LSParticleContactDetectionKernel.cu:62-115 / :355-409).  Returns true
   // and fills the contact when overlap >= 0.  `rel` is the node minus partner position.
-  bool nodeVsSdf(const Bodies& B, int b, V3 xw, V3 xB, Q4 qB, Contact& c) {
+  bool nodeVsSdf(const Bodies& B, int b, V3 xw, V3 xB, Q4 qB, Contact& c, Counters& cn) {
     const V3 rel = xw - xB;
     const V3 loc = rotateInv(rel, qB);
     const V3 o = gridOrigin(B, b);
@@ -377,7 +377,7 @@ class Solver {
     p[5] = sdfAt(B, b, linearIndex3D(I3{i1, j0, k1}, n));
     p[6] = sdfAt(B, b, linearIndex3D(I3{i0, j1, k1}, n));
     p[7] = sdfAt(B, b, linearIndex3D(I3{i1, j1, k1}, n));
-    counters.sdfQueries++;
+    cn.sdfQueries++;
     const double ov = -interpValue(x, y, z, p);
     if (!(ov >= 0.)) return false;
     V3 nc = (quirks & QUIRK_INT_X_GRADIENT) ? interpGradient(int(x), y, z, p) : interpGradientExact(x, y, z, p);
@@ -440,8 +440,7 @@ class Solver {
   // Particle-particle contacts, node-centric literal traversal (LSParticleContactDetectionKernel.cu:124-297).
   void buildPP_literal(const std::vector<int>& keyP, const CellTable& ct, const History& hist,
                        std::vector<Contact>& out) {
-    out.clear();
-    for (size_t A = 0; A < P.n(); ++A) {
+    overMasters(out, [&](size_t A, std::vector<Contact>& out, Counters& cn) {
       for (int k = P.nodeStart[A]; k < P.nodeStart[A + 1]; ++k) {
         const V3 xw = rotate(P.q[A], nodeLocal(P, int(A), k - P.nodeStart[A])) + P.pos[A];
         I3 c, lo, hi;
@@ -454,18 +453,18 @@ class Solver {
               for (int s = ct.start[h]; s < ct.end[h]; ++s) {
                 const int B = ct.sortedIdx[s];
                 if (int(A) >= B) continue;
-                counters.nodeTests++;
+                cn.nodeTests++;
                 const V3 rel = xw - P.pos[B];
                 const double rB = P.radius[B];
                 if (lengthSquared(rel) > rB * rB) continue;
                 Contact ct2;
-                if (!nodeVsSdf(P, B, xw, P.pos[B], P.q[B], ct2)) continue;
-                finishContact(ct2, k, hist);
+                if (!nodeVsSdf(P, B, xw, P.pos[B], P.q[B], ct2, cn)) continue;
+                finishContact(ct2, k, hist, cn);
                 out.push_back(ct2);
               }
             }
       }
-    }
+    });
     (void)keyP;
   }
 
@@ -473,10 +472,9 @@ class Solver {
   // nodes of A.  Equivalent because a node within rB of xB lies in a cell within +-1 of B's
   // cell whenever the cell width is >= 2*max radius (the reference's choice).
   void buildPP_pairs(const std::vector<int>& keyP, const History& hist, std::vector<Contact>& out) {
-    out.clear();
     const CellTable ct = buildCells(keyP, gridP.ncell());
-    std::vector<int> cand;
-    for (size_t A = 0; A < P.n(); ++A) {
+    overMasters(out, [&](size_t A, std::vector<Contact>& out, Counters& cn) {
+      std::vector<int> cand;
       // candidate partners: B > A, valid key, bounding spheres overlap (with slack).  B lies
       // within rA + rB <= 2 max radius <= one cell width of A, so the clamped 27-cell window
       // around A's centre holds every candidate.
@@ -497,30 +495,60 @@ class Solver {
             }
           }
       std::sort(cand.begin(), cand.end());
-      counters.pairs += int64_t(cand.size());
+      cn.pairs += int64_t(cand.size());
       for (int k = P.nodeStart[A]; k < P.nodeStart[A + 1]; ++k) {
         const V3 xw = rotate(P.q[A], nodeLocal(P, int(A), k - P.nodeStart[A])) + P.pos[A];
         for (int B : cand) {
-          counters.nodeTests++;
+          cn.nodeTests++;
           const V3 rel = xw - P.pos[B];
           const double rB = P.radius[B];
           if (lengthSquared(rel) > rB * rB) continue;
           Contact c;
-          if (!nodeVsSdf(P, B, xw, P.pos[B], P.q[B], c)) continue;
-          finishContact(c, k, hist);
+          if (!nodeVsSdf(P, B, xw, P.pos[B], P.q[B], c, cn)) continue;
+          finishContact(c, k, hist, cn);
           out.push_back(c);
         }
       }
-    }
+    });
   }
 
-  void finishContact(Contact& c, int node, const History& hist) {
+  void finishContact(Contact& c, int node, const History& hist, Counters& cn) const {
     c.master = node;
     auto it = hist.find(hkey(node, c.slave));
     c.spring = (it == hist.end()) ? v3(0., 0., 0.) : it->second;
     c.En = 0.;
     c.Es = 0.;
-    counters.contacts++;
+    cn.contacts++;
+  }
+
+  // Runs fn(A, out, counters) for every master particle A and leaves in `out` the contacts in
+  // increasing A (the serial order).  With threads > 1 the masters are cut into contiguous
+  // chunks taken by OpenMP threads; each chunk fills its own list and counters, concatenated in
+  // chunk order afterwards, so lists and counters are the same for any thread count.
+  int threads = 1;
+  template <class Fn>
+  void overMasters(std::vector<Contact>& out, Fn fn) {
+    out.clear();
+    const size_t n = P.n();
+    if (threads <= 1 || n < 2) {
+      for (size_t A = 0; A < n; ++A) fn(A, out, counters);
+      return;
+    }
+    const int nChunks = int(std::min<size_t>(n, size_t(threads) * 16));
+    std::vector<std::vector<Contact>> part(nChunks);
+    std::vector<Counters> cnt(nChunks);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (int ch = 0; ch < nChunks; ++ch) {
+      const size_t lo = n * size_t(ch) / size_t(nChunks), hi = n * size_t(ch + 1) / size_t(nChunks);
+      for (size_t A = lo; A < hi; ++A) fn(A, part[ch], cnt[ch]);
+    }
+    for (int ch = 0; ch < nChunks; ++ch) {
+      out.insert(out.end(), part[ch].begin(), part[ch].end());
+      counters.pairs += cnt[ch].pairs;
+      counters.nodeTests += cnt[ch].nodeTests;
+      counters.sdfQueries += cnt[ch].sdfQueries;
+      counters.contacts += cnt[ch].contacts;
+    }
   }
 
   // Fixed-partner contacts (walls, ghosts): LSParticleContactDetectionKernel.cu:418-590.  No
@@ -528,9 +556,8 @@ class Solver {
   void buildFixed(const Bodies& F, const std::vector<V3>& fpos, const std::vector<Q4>& fq,
                   const std::vector<int>& fkey, const Grid& grid, const History& hist,
                   std::vector<Contact>& out) {
-    out.clear();
     const CellTable ct = buildCells(fkey, grid.ncell());
-    for (size_t A = 0; A < P.n(); ++A) {
+    overMasters(out, [&](size_t A, std::vector<Contact>& out, Counters& cn) {
       for (int k = P.nodeStart[A]; k < P.nodeStart[A + 1]; ++k) {
         const V3 xw = rotate(P.q[A], nodeLocal(P, int(A), k - P.nodeStart[A])) + P.pos[A];
         I3 c, lo, hi;
@@ -542,15 +569,95 @@ class Solver {
               if (ct.start[h] == -1) continue;
               for (int s = ct.start[h]; s < ct.end[h]; ++s) {
                 const int B = ct.sortedIdx[s];
-                counters.nodeTests++;
+                cn.nodeTests++;
                 Contact cc;
-                if (!nodeVsSdf(F, B, xw, fpos[B], fq[B], cc)) continue;
-                finishContact(cc, k, hist);
+                if (!nodeVsSdf(F, B, xw, fpos[B], fq[B], cc, cn)) continue;
+                finishContact(cc, k, hist, cn);
                 out.push_back(cc);
               }
             }
       }
+    });
+  }
+
+  // Largest distance from a body's centre to a corner of its (scaled) grid box: no node outside
+  // that sphere can lie inside the grid.
+  double gridBoxRadius(const Bodies& B, int b) const {
+    const Shape& s = shapes[B.shape[b]];
+    double r2 = 0.;
+    for (int k = 0; k < 8; ++k) {
+      const V3 c = s.origin + s.spacing * v3((k & 1) ? double(s.nx - 1) : 0., (k & 2) ? double(s.ny - 1) : 0.,
+                                             (k & 4) ? double(s.nz - 1) : 0.);
+      r2 = std::max(r2, lengthSquared(c));
     }
+    return std::sqrt(r2) * B.scale[b];
+  }
+
+  // The same contact set as buildFixed, from partner candidates per master: a fixed partner B
+  // can hold a node of A only if A's bounding sphere meets the sphere around B's grid box.  Each
+  // node then keeps the literal traversal's conditions (B's key in the node's clamped +-1 window,
+  // the node inside B's grid) and the same node-vs-SDF arithmetic, so contacts are identical;
+  // only their order within the list differs.
+  void buildFixed_pairs(const Bodies& F, const std::vector<V3>& fpos, const std::vector<Q4>& fq,
+                        const std::vector<int>& fkey, const Grid& grid, const History& hist,
+                        std::vector<Contact>& out) {
+    std::vector<double> R(F.n(), 0.);
+    double rA = 0., rF = 0.;
+    for (size_t A = 0; A < P.n(); ++A) rA = std::max(rA, P.radius[A]);
+    for (size_t B = 0; B < F.n(); ++B) {
+      if (fkey[B] < 0) continue;
+      R[B] = gridBoxRadius(F, int(B));
+      rF = std::max(rF, R[B]);
+    }
+    // hash of the partners with a key, on cells of the largest reach
+    const double h = (rA + rF) * (1.0 + 1e-9) + 1e-12;
+    auto cellKey = [&](int64_t x, int64_t y, int64_t z) {
+      return ((x + (1ll << 20)) << 42) | ((y + (1ll << 20)) << 21) | (z + (1ll << 20));
+    };
+    auto cellOf = [&](V3 p, int64_t& x, int64_t& y, int64_t& z) {
+      x = int64_t(std::floor(p.x / h)); y = int64_t(std::floor(p.y / h)); z = int64_t(std::floor(p.z / h));
+    };
+    std::unordered_map<int64_t, std::vector<int>> cells;
+    for (size_t B = 0; B < F.n(); ++B) {
+      if (fkey[B] < 0) continue;
+      int64_t x, y, z;
+      cellOf(fpos[B], x, y, z);
+      cells[cellKey(x, y, z)].push_back(int(B));
+    }
+    overMasters(out, [&](size_t A, std::vector<Contact>& out, Counters& cn) {
+      std::vector<int> cand;
+      int64_t x, y, z;
+      cellOf(P.pos[A], x, y, z);
+      for (int64_t dz = -1; dz <= 1; ++dz)
+        for (int64_t dy = -1; dy <= 1; ++dy)
+          for (int64_t dx = -1; dx <= 1; ++dx) {
+            auto it = cells.find(cellKey(x + dx, y + dy, z + dz));
+            if (it == cells.end()) continue;
+            for (int B : it->second) {
+              const double rr = (P.radius[A] + R[B]) * (1.0 + 1e-9) + 1e-12;
+              if (lengthSquared(P.pos[A] - fpos[B]) <= rr * rr) cand.push_back(B);
+            }
+          }
+      std::sort(cand.begin(), cand.end());
+      cn.pairs += int64_t(cand.size());
+      if (cand.empty()) return;
+      for (int k = P.nodeStart[A]; k < P.nodeStart[A + 1]; ++k) {
+        const V3 xw = rotate(P.q[A], nodeLocal(P, int(A), k - P.nodeStart[A])) + P.pos[A];
+        I3 c, lo, hi;
+        grid.window(xw, c, lo, hi);
+        for (int B : cand) {
+          const I3 kc = grid.decode(fkey[B]);
+          if (kc.x < c.x + lo.x || kc.x > c.x + hi.x || kc.y < c.y + lo.y || kc.y > c.y + hi.y ||
+              kc.z < c.z + lo.z || kc.z > c.z + hi.z)
+            continue;
+          cn.nodeTests++;
+          Contact cc;
+          if (!nodeVsSdf(F, B, xw, fpos[B], fq[B], cc, cn)) continue;
+          finishContact(cc, k, hist, cn);
+          out.push_back(cc);
+        }
+      }
+    });
   }
 
   // ---- contact law (contactKernel.cuh:4-77) ----
@@ -824,7 +931,8 @@ class Solver {
     }
     {
       History h = makeHistory(lists[LIST_PW]);
-      buildFixed(W, W.pos, W.q, keyW, gridW, h, lists[LIST_PW]);
+      if (literal) buildFixed(W, W.pos, W.q, keyW, gridW, h, lists[LIST_PW]);
+      else buildFixed_pairs(W, W.pos, W.q, keyW, gridW, h, lists[LIST_PW]);
     }
     if (periodic) {
       if (quirks & QUIRK_WRAP_AFTER_BUILD) {
@@ -839,7 +947,8 @@ class Solver {
         const bool on = (s == 0 && perX) || (s == 1 && perY) || (s == 2 && perX && perY);
         if (!on) { lists[LIST_GX + s].clear(); continue; }
         History h = makeHistory(lists[LIST_GX + s]);
-        buildFixed(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
+        if (literal) buildFixed(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
+        else buildFixed_pairs(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
       }
     }
   }

@@ -51,6 +51,13 @@ int lsdem_oracle_set_literal(void* ctx, int literal) {
   return LSDEM_OK;
 }
 
+// OpenMP threads of the detection loops (1: serial).  Lists and counters do not depend on it.
+int lsdem_oracle_set_threads(void* ctx, int threads) {
+  if (threads < 1) return fail(LSDEM_ERR_ARG, "threads < 1");
+  static_cast<Ctx*>(ctx)->s.threads = threads;
+  return LSDEM_OK;
+}
+
 int lsdem_oracle_add_shape(void* ctx, const float* sdf, int32_t nx, int32_t ny, int32_t nz,
                            const double origin[3], double spacing, const double* nodes, int32_t n_nodes,
                            int32_t* shape_id) {

@@ -15,10 +15,16 @@ def oracle_lib() -> C.CDLL:
     return C.CDLL(str(ORACLE_SO))
 
 
-def oracle_sim(cfg: dict, literal: bool = False):
+def oracle_sim(cfg: dict, literal: bool = False, threads: int = 1):
+    """The oracle behind the product's Python binding.  threads > 1: OpenMP over master particles
+    in the detection loops (same lists, counters and forces as serial)."""
     import lsdem_amd.lsdem as L
     lib = oracle_lib()
     s = L.Sim(cfg, lib=lib, prefix="lsdem_oracle_")
     if literal:
         lib.lsdem_oracle_set_literal(s.ctx, C.c_int(1))
+    if threads > 1:
+        rc = lib.lsdem_oracle_set_threads(s.ctx, C.c_int(threads))
+        if rc != 0:
+            raise RuntimeError("lsdem_oracle_set_threads failed")
     return s

@@ -380,6 +380,26 @@ def test_literal_equals_pairs(S, periodic):
         assert np.allclose(pa["force"], pb["force"], rtol=0, atol=1e-9 * np.abs(pb["force"]).max())
 
 
+@pytest.mark.parametrize("literal", [False, True])
+def test_threads_equal_serial(S, literal):
+    """The OpenMP variant of the detection loops (the all-cores CPU baseline) gives the serial
+    oracle's lists, springs, forces and counters bit for bit."""
+    sc = S.scene_c2(n=150, n_types=2, n_grid=20, n_nodes=96, cell_xy=0.04, gap=-0.12)
+    a = sc.load(oracle_sim(sc.cfg, literal=literal))
+    b = sc.load(oracle_sim(sc.cfg, literal=literal, threads=4))
+    for _ in range(3):
+        a.step(2)
+        b.step(2)
+        for l in (0, 1, 2, 3, 4):
+            ca, cb = a.contacts(l), b.contacts(l)
+            for k in ca:
+                assert np.array_equal(ca[k], cb[k]), (l, k)
+        pa, pb = a.get_particles(), b.get_particles()
+        for k in ("pos", "vel", "quat", "angvel", "force", "torque"):
+            assert np.array_equal(pa[k], pb[k]), k
+    assert a.counters() == b.counters()
+
+
 def test_ghost_third_law(S):
     """Without walls, particle-particle and ghost contacts only exchange internal forces: the
     total force on all particles vanishes (each ghost contact pushes A and the real source j)."""
