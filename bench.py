@@ -52,9 +52,9 @@ B_CONTACT = 16 * 4
 B_LAW = 45 * 4
 B_BODY = (78 + 6) * 4
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a plain copy
-# (profiles/c2_200k_pmc_traffic.md); None for a kernel not measured at its current code.
-KERNEL_TRAFFIC_BYTES = {"k_pair_check": None, "k_candidates": None, "k_reduce_drift": None}
+# HBM bytes per launch at the 200k workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a
+# plain copy (profiles/c2_200k_pmc_traffic.md); None for a kernel not measured at its current code.
+KERNEL_TRAFFIC_BYTES = {"k_pair_check": 73.73e6, "k_candidates": 9.83e6, "k_reduce_drift": 88.32e6}
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 PROFILE_EVERY = 10
 

@@ -51,6 +51,103 @@ class LSParticleDeviceView {
   lsdem_device_view v_{};
 };
 
+// Host copies of one contact list of the last step, with the names of the reference's
+// SolidInteraction host references (SolidInteraction.h:335-374); each call downloads the list,
+// sorted by (master node, slave).  Empty before the first solve().
+class LSContactListView {
+ public:
+  size_t numPair_device() const { return size_t(count()); }
+  std::vector<double3> contactPointHostRef() const { return vec3(kPoint); }
+  std::vector<double3> contactNormalHostRef() const { return vec3(kNormal); }
+  std::vector<double3> slidingSpringHostRef() const { return vec3(kSpring); }
+  std::vector<double> contactOverlapHostRef() const { return vec1(kOverlap); }
+  std::vector<double> normalElasticEnergyHostRef() const { return vec1(kEn); }
+  std::vector<double> slidingElasticEnergyHostRef() const { return vec1(kEs); }
+  std::vector<int> masterIDHostRef() const { return ids(true); }
+  std::vector<int> slaveIDHostRef() const { return ids(false); }
+  void bind(lsdem_ctx* const* ctx, int32_t list) {
+    ctx_ = ctx;
+    list_ = list;
+  }
+
+ private:
+  enum { kPoint, kNormal, kSpring, kOverlap, kEn, kEs };
+  lsdem_ctx* const* ctx_ = nullptr;
+  int32_t list_ = LSDEM_LIST_PP;
+  int64_t count() const {
+    int64_t n = 0;
+    if (ctx_ && *ctx_)
+      lsdem_get_contacts(*ctx_, list_, 0, &n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return n;
+  }
+  std::vector<double> raw(int what, int width) const {
+    int64_t n = count();
+    std::vector<double> d(static_cast<size_t>(n) * width);
+    if (n == 0) return d;
+    double* a[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    a[what] = d.data();
+    lsdem_get_contacts(*ctx_, list_, n, &n, nullptr, nullptr, a[kPoint], a[kNormal], a[kOverlap], a[kSpring], a[kEn],
+                       a[kEs]);
+    return d;
+  }
+  std::vector<double3> vec3(int what) const {
+    const std::vector<double> d = raw(what, 3);
+    std::vector<double3> v(d.size() / 3);
+    for (size_t k = 0; k < v.size(); ++k) v[k] = make_double3(d[3 * k], d[3 * k + 1], d[3 * k + 2]);
+    return v;
+  }
+  std::vector<double> vec1(int what) const { return raw(what, 1); }
+  std::vector<int> ids(bool master) const {
+    int64_t n = count();
+    std::vector<int32_t> m(static_cast<size_t>(n)), sl(static_cast<size_t>(n));
+    if (n) lsdem_get_contacts(*ctx_, list_, n, &n, m.data(), sl.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    const std::vector<int32_t>& r = master ? m : sl;
+    return std::vector<int>(r.begin(), r.end());
+  }
+};
+
+// Host copies of the bond state, with the names of the reference's VBondedInteraction
+// (VBondedInteraction.h:489-520): activated flag, centre point, energies and peak stresses.
+class LSBondListView {
+ public:
+  size_t numPair() const { return n_; }
+  size_t numPair_device() const { return n_; }
+  std::vector<int> activatedFlagHostRef() const {
+    std::vector<int32_t> a(n_);
+    if (n_) lsdem_get_bonds(*ctx_, a.data(), nullptr, nullptr, nullptr, nullptr);
+    return std::vector<int>(a.begin(), a.end());
+  }
+  std::vector<double3> centerPointHostRef() const {
+    std::vector<double> d(3 * n_);
+    if (n_) lsdem_get_bonds(*ctx_, nullptr, d.data(), nullptr, nullptr, nullptr);
+    std::vector<double3> v(n_);
+    for (size_t k = 0; k < n_; ++k) v[k] = make_double3(d[3 * k], d[3 * k + 1], d[3 * k + 2]);
+    return v;
+  }
+  // Un, Us, Ub, Ut per bond
+  std::vector<double> elasticEnergyHostRef(int which) const {
+    std::vector<double> e(4 * n_), out(n_);
+    if (n_) lsdem_get_bonds(*ctx_, nullptr, nullptr, e.data(), nullptr, nullptr);
+    for (size_t k = 0; k < n_; ++k) out[k] = e[4 * k + size_t(which & 3)];
+    return out;
+  }
+  std::vector<double> maxNormalStressHostRef() const { return stress(true); }
+  std::vector<double> maxShearStressHostRef() const { return stress(false); }
+  void bind(lsdem_ctx* const* ctx, size_t n) {
+    ctx_ = ctx;
+    n_ = (ctx && *ctx) ? n : 0;
+  }
+
+ private:
+  lsdem_ctx* const* ctx_ = nullptr;
+  size_t n_ = 0;
+  std::vector<double> stress(bool normal) const {
+    std::vector<double> s(n_), t(n_);
+    if (n_) lsdem_get_bonds(*ctx_, nullptr, nullptr, nullptr, s.data(), t.data());
+    return normal ? s : t;
+  }
+};
+
 class LSSolver {
  public:
   LSSolver(const std::string dir = "Problem", hipStream_t stream = 0, const size_t maxGPUThread = 256,
@@ -257,6 +354,20 @@ class LSSolver {
   virtual void addExternalForceTorque(const double time) { (void)time; }
 
   LSParticleDeviceView& getLSParticle() { return view_; }
+  // Contact lists and bonds of the last step, as host copies (the reference returns its
+  // SolidInteraction / VBondedInteraction objects, LSSolver.h:510-518).
+  LSContactListView& getLSParticleInteraction() {
+    ppView_.bind(&ctx_, LSDEM_LIST_PP);
+    return ppView_;
+  }
+  LSContactListView& getLSParticleWallInteraction() {
+    pwView_.bind(&ctx_, LSDEM_LIST_PW);
+    return pwView_;
+  }
+  LSBondListView& getLSParticleBondedInteraction() {
+    bondView_.bind(&ctx_, dirty_ ? 0 : bonds_.size());
+    return bondView_;
+  }
 
  private:
   struct Particle {
@@ -308,6 +419,8 @@ class LSSolver {
   double cfgDt_ = 0;
   bool cfgPX_ = false, cfgPY_ = false;
   LSParticleDeviceView view_;
+  LSContactListView ppView_, pwView_;
+  LSBondListView bondView_;
   std::map<std::string, std::vector<std::pair<double, std::string>>> pvd_;
 
   static bool validGrid(const char* who, const std::vector<double>& sdf, int3 n, double h) {

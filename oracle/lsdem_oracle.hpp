@@ -95,6 +95,35 @@ inline V3 rotateInv(V3 v, Q4 q) {  // reverseRotateVectorByQuaternion, myQua.h:1
   V3 t = cross(qv, v) * 2.0;
   return v + (t * q.q0) + cross(qv, t);
 }
+
+// v rotated by the rotation vector omega (Rodrigues), rotateVector, myVec.h:143-184 (host branch).
+inline V3 rotateVector(V3 v, V3 omega) {
+  const double theta2 = dot(omega, omega);
+  const V3 oxv = cross(omega, v);
+  const V3 oxoxv = cross(omega, oxv);
+  if (isZero(theta2)) return v + oxv + 0.5 * oxoxv;
+  const double invTheta = 1.0 / std::sqrt(theta2);
+  const double theta = theta2 * invTheta;
+  const double A = std::sin(theta) * invTheta;
+  const double B = (1.0 - std::cos(theta)) * (invTheta * invTheta);
+  return v + A * oxv + B * oxoxv;
+}
+// (rotation by omega) x q, normalised; rotateQuaternion, myQua.h:131-179 (host branch).
+inline Q4 rotateQuaternion(Q4 q, V3 omega) {
+  const double theta2 = omega.x * omega.x + omega.y * omega.y + omega.z * omega.z;
+  if (isZero(theta2)) return q;
+  const double theta = std::sqrt(theta2);
+  const double invTheta = 1.0 / theta;
+  const double ux = omega.x * invTheta, uy = omega.y * invTheta, uz = omega.z * invTheta;
+  const double c = std::cos(0.5 * theta), sn = std::sin(0.5 * theta);
+  const double r0 = c, r1 = sn * ux, r2 = sn * uy, r3 = sn * uz;
+  Q4 o;
+  o.q0 = r0 * q.q0 - r1 * q.q1 - r2 * q.q2 - r3 * q.q3;
+  o.q1 = r0 * q.q1 + r1 * q.q0 + r2 * q.q3 - r3 * q.q2;
+  o.q2 = r0 * q.q2 - r1 * q.q3 + r2 * q.q0 + r3 * q.q1;
+  o.q3 = r0 * q.q3 + r1 * q.q2 - r2 * q.q1 + r3 * q.q0;
+  return normalize(o);
+}
 inline V3 operator*(S6 m, V3 v) {  // myMat.h:80-87
   return v3(m.xx * v.x + m.xy * v.y + m.xz * v.z, m.xy * v.x + m.yy * v.y + m.yz * v.z,
             m.xz * v.x + m.yz * v.y + m.zz * v.z);
@@ -275,6 +304,10 @@ class Solver {
   V3 domMin{0, 0, 0}, domMax{1, 1, 1}, g{0, 0, 0};
   double dt = 1e-5;
   bool perX = false, perY = false;
+  // quarter-sector rotational periodicity about the z axis through the domain's lower corner
+  // (LSSolver::addPeriodicBoundarySector, PeriodicBoundary.h:313-420); its three image sets are
+  // rotated by +90, 180 and -90 degrees and use the lists of the x, y and xy image sets
+  bool perSector = false;
   int quirks = QUIRK_ALL;
   bool literal = false;  // true: node-centric 27-cell traversal exactly as the reference
   Grid gridP, gridW;
@@ -908,6 +941,51 @@ class Solver {
     G.vel = P.vel;
   }
 
+  // ---- quarter-sector periodicity (mySpatialGrid.cu:58-85, 154-212) ----
+  // Bodies that left the sector across x = min.x (with y >= min.y) turn by -90 degrees about the
+  // corner's z axis, across y = min.y (with x >= min.x) by +90: position, velocity and orientation
+  // (the angular velocity is left as it is).
+  void wrapSector() {
+    const V3 mn = gridP.mn;
+    const V3 rm = v3(0., 0., -0.5 * kPi), rp = v3(0., 0., 0.5 * kPi);
+    for (size_t i = 0; i < P.n(); ++i) {
+      V3& p = P.pos[i];
+      if (p.x < mn.x && p.y >= mn.y) {
+        P.vel[i] = rotateVector(P.vel[i], rm);
+        p = mn + rotateVector(p - mn, rm);
+        P.q[i] = rotateQuaternion(P.q[i], rm);
+      }
+      if (p.y < mn.y && p.x >= mn.x) {
+        P.vel[i] = rotateVector(P.vel[i], rp);
+        p = mn + rotateVector(p - mn, rp);
+        P.q[i] = rotateQuaternion(P.q[i], rp);
+      }
+    }
+  }
+  // Image sets: 0 turned by +90 degrees for bodies in the first y cell, 1 by 180 for bodies in the
+  // first x and y cells, 2 by -90 for bodies in the first x cell; other copies keep the body's pose.
+  void makeSectorGhosts() {
+    const V3 mn = gridP.mn;
+    const V3 ang[3] = {v3(0., 0., 0.5 * kPi), v3(0., 0., kPi), v3(0., 0., -0.5 * kPi)};
+    for (int set = 0; set < 3; ++set) {
+      Ghost& G = ghost[set];
+      G.pos = P.pos;
+      G.vel = P.vel;
+      G.q = P.q;
+      G.key.resize(P.n());
+      for (size_t i = 0; i < P.n(); ++i) {
+        const I3 c = gridP.cellOf(P.pos[i]);
+        const bool on = set == 0 ? c.y == 0 : (set == 1 ? (c.x == 0 && c.y == 0) : c.x == 0);
+        if (on) {
+          G.vel[i] = rotateVector(P.vel[i], ang[set]);
+          G.pos[i] = mn + rotateVector(P.pos[i] - mn, ang[set]);
+          G.q[i] = rotateQuaternion(P.q[i], ang[set]);
+        }
+        G.key[i] = ghostKey(G.pos[i]);
+      }
+    }
+  }
+
   // ---- detection: keys, particle-particle and wall lists, periodic wrap, ghost lists ----
   // The part of a step between the drift and the force kernels (LSSolver.h:1003-1004,
   // 1071-1074): contact lists rebuilt at the current poses, each carrying its springs over
@@ -919,9 +997,10 @@ class Solver {
 
     const bool periodic = perX || perY;
     // Contact lists built from pre-wrap positions (the wrap follows the main build).
-    if (periodic && !(quirks & QUIRK_WRAP_AFTER_BUILD)) {
+    if ((periodic || perSector) && !(quirks & QUIRK_WRAP_AFTER_BUILD)) {
       if (perX) wrapX();
       if (perY) wrapY();
+      if (perSector) wrapSector();
       for (size_t i = 0; i < P.n(); ++i) keyP[i] = gridP.key(P.pos[i]);
     }
     {
@@ -946,6 +1025,15 @@ class Solver {
       for (int s = 0; s < 3; ++s) {
         const bool on = (s == 0 && perX) || (s == 1 && perY) || (s == 2 && perX && perY);
         if (!on) { lists[LIST_GX + s].clear(); continue; }
+        History h = makeHistory(lists[LIST_GX + s]);
+        if (literal) buildFixed(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
+        else buildFixed_pairs(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
+      }
+    }
+    if (perSector) {
+      if (quirks & QUIRK_WRAP_AFTER_BUILD) wrapSector();
+      makeSectorGhosts();
+      for (int s = 0; s < 3; ++s) {
         History h = makeHistory(lists[LIST_GX + s]);
         if (literal) buildFixed(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
         else buildFixed_pairs(P, ghost[s].pos, ghost[s].q, ghost[s].key, gridP, h, lists[LIST_GX + s]);
@@ -979,6 +1067,8 @@ class Solver {
       if (perY) forcesGhost(lists[LIST_GY], ghost[1]);
       if (perX && perY) forcesGhost(lists[LIST_GXY], ghost[2]);
     }
+    if (perSector)
+      for (int s = 0; s < 3; ++s) forcesGhost(lists[LIST_GX + s], ghost[s]);
     if (!bonds.empty()) forcesBonds();
     if (external) external(time);
     kick(0.5 * dt);

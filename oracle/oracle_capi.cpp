@@ -51,6 +51,15 @@ int lsdem_oracle_set_literal(void* ctx, int literal) {
   return LSDEM_OK;
 }
 
+// Quarter-sector rotational periodicity about the z axis through the domain's lower corner
+// (LSSolver::addPeriodicBoundarySector); not combined with x/y periodicity.
+int lsdem_oracle_set_sector(void* ctx, int on) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (on && (c->s.perX || c->s.perY)) return fail(LSDEM_ERR_ARG, "sector periodicity with x/y periodicity");
+  c->s.perSector = on != 0;
+  return LSDEM_OK;
+}
+
 // OpenMP threads of the detection loops (1: serial).  Lists and counters do not depend on it.
 int lsdem_oracle_set_threads(void* ctx, int threads) {
   if (threads < 1) return fail(LSDEM_ERR_ARG, "threads < 1");

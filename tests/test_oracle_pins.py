@@ -134,15 +134,17 @@ def test_ghost_images_and_keys_bruteforce(S, quirks):
         assert n_corner == 0
 
 
-def _brute_ghost(sc, st, images, grid, margin=1e-10):
+def _brute_ghost(sc, st, images, grid, margin=1e-10, image_rot=None):
     """(node of A, j) for every node inside image j's grid whose +-1 cell window holds the image's key
-    and whose trilinear phi <= 0.  Returns (sure, borderline)."""
+    and whose trilinear phi <= 0.  image_rot: the images' rotation matrices (default: the sources').
+    Returns (sure, borderline)."""
     mn, mx, inv, size = grid
     P, shapes = sc.particles, sc.shapes
     pos, quat = st["pos"], st["quat"]
     n = len(pos)
     node0 = np.concatenate([[0], np.cumsum([len(shapes[s]["nodes"]) for s in P["shape"]])])
     R = np.array([rotmat(q) for q in quat])
+    Rimg = R if image_rot is None else image_rot
     img, key = images
     sure, border = set(), set()
     kc = np.stack([key % size[0], (key // size[0]) % size[1], key // (size[0] * size[1])], axis=1)
@@ -157,7 +159,7 @@ def _brute_ghost(sc, st, images, grid, margin=1e-10):
             win = np.all(np.abs(c - kc[j]) <= 1, axis=1)
             SB = shapes[P["shape"][j]]
             s = P["scale"][j]
-            loc = (xw - img[j]) @ R[j]
+            loc = (xw - img[j]) @ Rimg[j]
             g = (loc - SB["origin"] * s) / (SB["spacing"] * s)
             i0 = np.floor(g).astype(int)
             nz, ny, nx = SB["sdf"].shape
@@ -396,3 +398,73 @@ def test_preloop_wrap(S):
     for k in range(5):
         assert np.array_equal(ca[k]["master"], cb[k]["master"]) and np.array_equal(ca[k]["slave"], cb[k]["slave"])
     assert len(ca[0]["master"]) > 0 and len(ca[2]["master"]) > 0
+
+
+def _rz(deg):
+    c, s = {90: (0.0, 1.0), 180: (-1.0, 0.0), 270: (0.0, -1.0)}[deg]
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _sector_scene(S, pos, rng=None):
+    sc = _sphere_cloud_scene(S, pos, dom=((0.0, 0.0, 0.0), (0.06, 0.06, 0.06)), rng=rng)
+    sc.cfg["periodic_x"] = 0
+    sc.cfg["periodic_y"] = 0
+    return sc
+
+
+def _sector_sim(sc):
+    sim = sc.load(oracle_sim(sc.cfg))
+    assert oracle_lib().lsdem_oracle_set_sector(sim.ctx, C.c_int(1)) == 0
+    return sim
+
+
+def test_sector_wrap_images_and_contacts_bruteforce(S):
+    """Quarter-sector periodicity (mySpatialGrid.cu:58-85, 154-212): a body that left across x = min
+    turns by -90 degrees about the corner's z axis (position, velocity and orientation), one that left
+    across y = min by +90; image set 0 is the +90 turn of bodies in the first y cell, set 1 the 180 turn
+    of those in the first x and y cells, set 2 the -90 turn of those in the first x cell, with keys only
+    outside the box.  Checked with exact quarter-turn matrices in numpy, and the image contact sets by
+    brute force over all (node, image) pairs with the images' own orientations."""
+    rng = np.random.default_rng(5)
+    pts = []
+    for x in (0.003, 0.006, 0.03, 0.055):
+        for y in (0.002, 0.006, 0.03, 0.056):
+            pts.append((x, y, 0.03 + 0.004 * len(pts) % 0.02))
+    pos = np.array(pts) + rng.uniform(-4e-4, 4e-4, (len(pts), 3))
+    pos[0] = (-0.0015, 0.021, 0.02)   # left across x = min: turned by -90 degrees
+    pos[1] = (0.033, -0.0012, 0.04)   # left across y = min: turned by +90
+    sc = _sector_scene(S, pos, rng)
+    sc.particles["vel"][:2] = ((0.3, -0.2, 0.1), (-0.1, 0.25, 0.0))
+    q0 = sc.particles["quat"][:2].copy()
+    sim = _sector_sim(sc)
+    sim.begin_phase()
+    st = sim.get_particles()
+    mn, mx, inv, size = _grid(sim)
+    for i, deg in ((0, 270), (1, 90)):
+        Rz = _rz(deg)
+        assert np.allclose(st["pos"][i], mn + Rz @ (pos[i] - mn), rtol=0, atol=1e-15)
+        assert np.allclose(st["vel"][i], Rz @ sc.particles["vel"][i], rtol=0, atol=1e-15)
+        assert np.allclose(rotmat(st["quat"][i]), Rz @ rotmat(q0[i]), rtol=0, atol=1e-14)
+    p = st["pos"]
+    cell = np.floor((p - mn) * inv).astype(int)
+    sets = [(90, cell[:, 1] == 0), (180, (cell[:, 0] == 0) & (cell[:, 1] == 0)), (270, cell[:, 0] == 0)]
+    zmax = mx[1]
+    total = 0
+    for s, (deg, on) in enumerate(sets):
+        Rz = _rz(deg)
+        exp = np.where(on[:, None], mn + (p - mn) @ Rz.T, p)
+        gp, gk = _ghosts(sim, s, len(p))
+        assert np.allclose(gp, exp, rtol=0, atol=1e-15), f"image positions of set {s}"
+        outside = ((exp[:, 0] < mn[0]) | (exp[:, 0] >= mx[0]) | (exp[:, 1] < mn[1]) | (exp[:, 1] >= mx[1]) |
+                   (exp[:, 2] < mn[2]) | (exp[:, 2] >= zmax))
+        c = np.clip(np.floor((exp - mn) * inv), 0, size - 1).astype(int)
+        key = np.where(outside, c[:, 2] * size[1] * size[0] + c[:, 1] * size[0] + c[:, 0], -1)
+        assert np.array_equal(gk, key), f"image keys of set {s}"
+        assert np.any(gk >= 0)
+        rot = np.array([(Rz if on[j] else np.eye(3)) @ rotmat(st["quat"][j]) for j in range(len(p))])
+        sure, border = _brute_ghost(sc, st, (gp, gk), (mn, mx, inv, size), image_rot=rot)
+        got = sim.contacts(2 + s)
+        pairs = set(zip(got["master"].tolist(), got["slave"].tolist()))
+        assert sure <= pairs and pairs <= sure | border, f"set {s}"
+        total += len(pairs)
+    assert total > 0
