@@ -301,6 +301,20 @@ int lsdem_surrogate_eval(lsdem_ctx* ctx, int64_t n, const float* points, const i
  * weights), computed by the training kernels; n a multiple of 64. */
 int lsdem_surrogate_gradient(lsdem_ctx* ctx, int64_t n, const float* points, const int32_t* shape_ids,
                              const float* targets, float* grad, double* loss);
+/* lsdem_surrogate_gradient without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight training or serving loops).  No counterpart in the
+ * reference.  points (n x 3), shape_ids (n), targets (n) and the outputs grad (same layout as the
+ * weights) and loss (one double, may be NULL) are DEVICE pointers on the context's device; the
+ * results equal lsdem_surrogate_gradient's bit for bit.  Intermediate data (family slots, per-block
+ * partial gradients) live in a fixed 16 MiB array of device memory that the library holds on each
+ * GPU until the process ends: n must be a positive multiple of 64 and at most what fits
+ * (256 B + 4 n B + (n / 64) (19,778 + 16 x shapes) x 4 B; about 13,000 samples with 10 shapes),
+ * else LSDEM_ERR_CAPACITY.  All calls on one GPU share that array: calls that overlap in time on
+ * the same GPU (from several threads or contexts) race and must be serialised by the caller.  The
+ * call synchronises the context's stream (to report shape ids outside the family as
+ * LSDEM_ERR_ARG). */
+int lsdem_surrogate_gradient_noalloc(lsdem_ctx* ctx, int64_t n, const float* points, const int32_t* shape_ids,
+                                     const float* targets, float* grad, double* loss);
 
 /* Mass, centroid (local frame), inertia and inverse inertia from an SDF grid with a smoothed
  * Heaviside of half-width 1.5 cells; host computation used by the first addLSParticle

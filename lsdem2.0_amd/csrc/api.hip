@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -521,6 +522,7 @@ int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
 }
 
 int launchBroadPhase(lsdem_ctx* c);
+int scratchOn(int dev, unsigned char** out);
 
 // One step.  headDone: the previous launch already ran this step's kick and drift.  fuseNext: the
 // step ends with the next step's kick and drift (another step follows with no host work between).
@@ -1010,6 +1012,10 @@ int lsdem_initialize(lsdem_ctx* c) {
   if (!c) return fail(LSDEM_ERR_ARG, "null context");
   if (c->initialized) return fail(LSDEM_ERR_STATE, "already initialized");
   HIPCHK(hipSetDevice(c->cfg.device));
+  {  // device address of the allocation-free entry point's scratch on this device
+    unsigned char* sp = nullptr;
+    if (int rc = scratchOn(c->cfg.device, &sp)) return rc;
+  }
   hipStream_t s = c->stream;
   const int nP = int(c->pPos.size()), nW = int(c->wPos.size()), nS = int(c->shapes.size());
   const int nB = int(c->bI.size());
@@ -2128,6 +2134,115 @@ int lsdem_surrogate_gradient(lsdem_ctx* c, int64_t n, const float* points, const
   }
   for (int b = 0; b < blocks; ++b) l += h[size_t(b) * (nAll + 1) + nAll];
   if (loss) *loss = l;
+  return LSDEM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free variant of lsdem_surrogate_gradient
+// ------------------------------------------------------------------------------------------
+namespace {
+
+// Scratch of lsdem_surrogate_gradient_noalloc: an error word (first 256 bytes), the family slot
+// of each sample, then the per-block partial gradients and losses that k_sum_partials adds up in
+// block order.  Every GPU the library runs on holds its own copy until the process ends; host code
+// reaches a device's copy only through the address hipGetSymbolAddress gives on that device.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMutex;
+unsigned char* g_scratchPtr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on device `dev`, which must be current; looked up once per device.
+int scratchOn(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(LSDEM_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(LSDEM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratchPtr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratchPtr[dev];
+  return LSDEM_OK;
+}
+
+__global__ void k_clear_word(int* w) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *w = 0;
+}
+
+// Family slot of each sample's shape id; an id outside the family raises *bad and takes slot 0
+// (so the kernels that follow stay in bounds; the call then reports the error).
+__global__ void k_sur_slots(const int32_t* shapeIds, long long n, const int* slotOfShape, int nShapeIds, int* slot,
+                            int* bad) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int id = shapeIds[i];
+    int sl = (id >= 0 && id < nShapeIds) ? slotOfShape[id] : -1;
+    if (sl < 0) {
+      atomicOr(bad, 1);
+      sl = 0;
+    }
+    slot[i] = sl;
+  }
+}
+
+// grad[k] = sum over blocks of part[b][k] in block order, in FP64 then rounded to FP32 (k < nAll);
+// k = nAll: the loss, kept in FP64.  The host sum of lsdem_surrogate_gradient, term for term.
+__global__ void k_sum_partials(const float* part, int blocks, int nAll, float* grad, double* loss) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nAll) return;
+  double g = 0;
+  for (int b = 0; b < blocks; ++b) g += part[size_t(b) * (nAll + 1) + k];
+  if (k < nAll) grad[k] = float(g);
+  else if (loss) *loss = g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsdem_surrogate_gradient_noalloc(lsdem_ctx* c, int64_t n, const float* points, const int32_t* shape_ids,
+                                     const float* targets, float* grad, double* loss) {
+  if (!c || !points || !shape_ids || !targets || !grad) return fail(LSDEM_ERR_ARG, "null argument");
+  if (n <= 0 || n % kSurRows) return fail(LSDEM_ERR_ARG, "n must be a positive multiple of 64");
+  if (c->sur.nShapes == 0) return fail(LSDEM_ERR_STATE, "no surrogate");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  unsigned char* base = nullptr;
+  if (int rc = scratchOn(c->cfg.device, &base)) return rc;
+  auto& S = c->sur;
+  const int blocks = int(n / kSurRows);
+  const int nAll = kSurNParam + S.nShapes * kSurEmb;
+  const size_t slotOff = 256;
+  const size_t partOff = slotOff + (size_t(n) * sizeof(int) + 255) / 256 * 256;
+  const size_t perBlock = size_t(nAll + 1) * sizeof(float);
+  if (partOff + size_t(blocks) * perBlock > kNoAllocScratchBytes) {
+    // largest multiple of 64 samples that fits: 256 + 256 * b + b * perBlock <= size (b blocks)
+    const long long maxBlocks = (long long)((kNoAllocScratchBytes - 256) / (256 + perBlock));
+    return fail(LSDEM_ERR_CAPACITY, "n beyond the scratch of the allocation-free gradient (" +
+                                        std::to_string(kNoAllocScratchBytes >> 20) + " MiB): at most " +
+                                        std::to_string(maxBlocks * kSurRows) + " samples with " +
+                                        std::to_string(S.nShapes) + " shapes");
+  }
+  int* bad = reinterpret_cast<int*>(base);
+  int* slot = reinterpret_cast<int*>(base + slotOff);
+  float* part = reinterpret_cast<float*>(base + partOff);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, s, bad);
+  hipLaunchKernelGGL(k_sur_slots, dim3(unsigned(std::min<long long>((n + 255) / 256, 1024))), dim3(256), 0, s,
+                     shape_ids, (long long)n, (const int*)S.slotOfShape.p, int(c->shapes.size()), slot, bad);
+  SurTrainArgs T{surNet(c), SurSamples{const_cast<float*>(points), slot, const_cast<float*>(targets), n}, 0, 1,
+                 int(n), part};
+  hipLaunchKernelGGL(k_sur_train, dim3(unsigned(blocks)), dim3(256), 0, s, T);
+  hipLaunchKernelGGL(k_sum_partials, dim3(unsigned((nAll + 1 + 255) / 256)), dim3(256), 0, s, (const float*)part,
+                     blocks, nAll, grad, loss);
+  HIPCHK(hipGetLastError());
+  int badHost = 0;
+  HIPCHK(hipMemcpyAsync(&badHost, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (badHost) return fail(LSDEM_ERR_ARG, "shape outside the surrogate family");
   return LSDEM_OK;
 }
 
