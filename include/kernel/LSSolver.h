@@ -6,7 +6,7 @@
 //  - the stream argument of the constructor is accepted and ignored: the library runs on its
 //    own stream, and hooks receive it through getLSParticle().stream();
 //  - identical level-set grids are stored once (deduplicated by content);
-//  - addPeriodicBoundarySector() is not supported (prints and is ignored);
+//  - addPeriodicBoundarySector() cannot be combined with x/y periodicity (the library rejects it);
 //  - a later solve() continues from the state of the previous one; if bodies or bonds were added
 //    in between, or the domain/time step/gravity/periodicity changed, the library context is
 //    rebuilt from the current state and the tangential contact history starts empty;
@@ -298,12 +298,13 @@ class LSSolver {
 
   void addPeriodicBoundaryXD() { periodicX_ = true; }
   void addPeriodicBoundaryYD() { periodicY_ = true; }
-  void addPeriodicBoundarySector() {
-    std::cerr << "[Solver] Sector periodicity is not supported by this build; ignored." << std::endl;
-  }
+  // Quarter-sector rotational periodicity about the z axis through the domain's lower corner
+  // (LSSolver.h:443-451).
+  void addPeriodicBoundarySector() { periodicSector_ = true; }
   void removePeriodicBoundary() {
     periodicX_ = false;
     periodicY_ = false;
+    periodicSector_ = false;
   }
 
   // Runs floor(T/dt) + 2 steps and writes numFrame + 2 frames (LSSolver.h:478-505, 994-1024).
@@ -405,7 +406,7 @@ class LSSolver {
   size_t maxGPUThread_;
   int device_;
   int phase_ = 0;
-  bool periodicX_ = false, periodicY_ = false;
+  bool periodicX_ = false, periodicY_ = false, periodicSector_ = false;
   std::vector<Shape> shapes_;
   std::map<std::string, int> shapeKey_;
   std::vector<Particle> parts_;
@@ -417,7 +418,7 @@ class LSSolver {
   bool dirty_ = true;
   double3 cfgMin_{}, cfgMax_{}, cfgG_{};
   double cfgDt_ = 0;
-  bool cfgPX_ = false, cfgPY_ = false;
+  bool cfgPX_ = false, cfgPY_ = false, cfgPS_ = false;
   LSParticleDeviceView view_;
   LSContactListView ppView_, pwView_;
   LSBondListView bondView_;
@@ -556,7 +557,7 @@ class LSSolver {
   bool sameConfig(double3 mn, double3 mx, double3 g, double dt) const {
     auto eq = [](double3 a, double3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; };
     return eq(mn, cfgMin_) && eq(mx, cfgMax_) && eq(g, cfgG_) && dt == cfgDt_ && periodicX_ == cfgPX_ &&
-           periodicY_ == cfgPY_;
+           periodicY_ == cfgPY_ && periodicSector_ == cfgPS_;
   }
 
   bool upload(double3 mn, double3 mx, double3 g, double dt) {
@@ -574,6 +575,7 @@ class LSSolver {
     cfg.dt = dt;
     cfg.periodic_x = periodicX_;
     cfg.periodic_y = periodicY_;
+    cfg.periodic_sector = periodicSector_;
     cfg.block_size = int32_t(maxGPUThread_);
     cfg.device = device_;
     auto check = [&](int rc) {
@@ -635,7 +637,7 @@ class LSSolver {
     lsdem_set_external_force(ctx_, &LSSolver::hookTrampoline, this);
     lsdem_device_view v;
     if (lsdem_get_device_view(ctx_, &v) == LSDEM_OK) view_.set(v);
-    cfgMin_ = mn; cfgMax_ = mx; cfgG_ = g; cfgDt_ = dt; cfgPX_ = periodicX_; cfgPY_ = periodicY_;
+    cfgMin_ = mn; cfgMax_ = mx; cfgG_ = g; cfgDt_ = dt; cfgPX_ = periodicX_; cfgPY_ = periodicY_; cfgPS_ = periodicSector_;
     dirty_ = false;
     return true;
   }

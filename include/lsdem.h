@@ -83,6 +83,11 @@ typedef struct lsdem_config {
                              (the runtime checks that its blocks are resident together); 0: a plain
                              launch sized by the occupancy query, the bounded barriers and the
                              recovery covering the case where they are not */
+  int32_t periodic_sector; /* 1: quarter-sector rotational periodicity about the z axis through
+                             domain_min (LSSolver::addPeriodicBoundarySector, PeriodicBoundary.h:313-420):
+                             a body that leaves across x = min (y = min) turns by -90 (+90) degrees
+                             about that axis; images turned by +90, 180 and -90 degrees fill the lists
+                             LSDEM_LIST_GX, _GY and _GXY.  Not combined with periodic_x / periodic_y. */
 } lsdem_config;
 
 typedef struct lsdem_counters {

@@ -304,6 +304,9 @@ struct lsdem_ctx {
   DBuf<double> dWMu, dWRes;
   DBuf<int> dWShape, dWKey;
   DBuf<d3> dGPos[3];
+  DBuf<q4> dGQuat[3];  // sector images: orientations and velocities
+  DBuf<d3> dGVel[3];
+  DBuf<q4> dQuatDet;   // sector periodicity: detection orientation of the particles
   DBuf<int> dCellCount, dCellStart, dSorted, dRank;
   DBuf<float4> dSPR, dSPRw;
   DBuf<float> dSBoxR;
@@ -370,6 +373,8 @@ void lsdem_ctx::releaseAll() {
   dWShape.release(); dWKey.release();
   for (int s = 0; s < 3; ++s) {
     dGPos[s].release();
+    dGQuat[s].release();
+    dGVel[s].release();
   }
   dCellCount.release(); dCellStart.release(); dSorted.release(); dRank.release();
   dSPR.release(); dSPRw.release(); dPairSlab.release(); dStatSlots.release();
@@ -798,6 +803,23 @@ int snapRestore(lsdem_ctx* c) {
   return LSDEM_OK;
 }
 
+// Rotation by `angle` about the z axis as the reference builds it from the rotation vector
+// (0, 0, angle): the Rodrigues coefficients of rotateVector (myVec.h:143-184) and the unit
+// quaternion of rotateQuaternion (myQua.h:131-175), host branches, so that the device applies the
+// same numbers with the same expressions.
+void sectorRotation(double angle, d3& w, double& A, double& B, q4& r) {
+  w = mk3(0.0, 0.0, angle);
+  const double theta2 = dot(w, w);
+  const double invTheta = 1.0 / std::sqrt(theta2);
+  const double theta = theta2 * invTheta;
+  A = std::sin(theta) * invTheta;
+  B = (1.0 - std::cos(theta)) * (invTheta * invTheta);
+  const double t = std::sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+  const double it = 1.0 / t;
+  const double c = std::cos(0.5 * t), sn = std::sin(0.5 * t);
+  r = q4{c, sn * (w.x * it), sn * (w.y * it), sn * (w.z * it)};
+}
+
 // Device pointers of every context array into the Params table.
 void bindPointers(lsdem_ctx* c) {
   Params& P = c->P;
@@ -818,7 +840,12 @@ void bindPointers(lsdem_ctx* c) {
   P.forceSet = c->dForceSet.p;
   P.wpos = c->dWPos.p; P.wquat = c->dWQuat.p; P.wvel = c->dWVel.p; P.wangvel = c->dWAng.p;
   P.wmu = c->dWMu.p; P.wres = c->dWRes.p; P.wshape = c->dWShape.p; P.wkey = c->dWKey.p;
-  for (int g = 0; g < 3; ++g) P.gpos[g] = c->dGPos[g].p;
+  for (int g = 0; g < 3; ++g) {
+    P.gpos[g] = c->dGPos[g].p;
+    P.gquat[g] = c->dGQuat[g].p;
+    P.gvel[g] = c->dGVel[g].p;
+  }
+  P.quatDet = c->dQuatDet.p;
   P.cellCount = c->dCellCount.p; P.cellStart = c->dCellStart.p; P.sorted = c->dSorted.p; P.rank = c->dRank.p;
   P.sPR = c->dSPR.p; P.sPRw = c->dSPRw.p; P.pairSlab = c->dPairSlab.p;
   P.sBoxR = c->dSBoxR.p;
@@ -859,6 +886,8 @@ int lsdem_create(const lsdem_config* cfg, lsdem_ctx** out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LSDEM_ERR_HIP, "no HIP device available");
   if (cfg->device < 0 || cfg->device >= ndev) return fail(LSDEM_ERR_ARG, "device ordinal out of range");
+  if (cfg->periodic_sector && (cfg->periodic_x || cfg->periodic_y))
+    return fail(LSDEM_ERR_ARG, "sector periodicity cannot be combined with x/y periodicity");
   HIPCHK(hipSetDevice(cfg->device));
   lsdem_ctx* c = new lsdem_ctx();
   c->cfg = *cfg;
@@ -1034,6 +1063,9 @@ int lsdem_initialize(lsdem_ctx* c) {
   }
   P.periodicX = c->cfg.periodic_x != 0;
   P.periodicY = c->cfg.periodic_y != 0;
+  P.periodicSector = c->cfg.periodic_sector != 0;
+  for (int g = 0; g < 3; ++g) sectorRotation(g == 0 ? 0.5 * kPi : (g == 1 ? kPi : -0.5 * kPi), P.secW[g], P.secA[g],
+                                             P.secB[g], P.secR[g]);
   const d3 dmn = mk3(c->cfg.domain_min[0], c->cfg.domain_min[1], c->cfg.domain_min[2]);
   const d3 dmx = mk3(c->cfg.domain_max[0], c->cfg.domain_max[1], c->cfg.domain_max[2]);
 
@@ -1194,12 +1226,16 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dSPRw.alloc(std::max(nP, 1)));
   HIPCHK(c->dSBoxR.alloc(std::max(nP, 1)));
   HIPCHK(c->dPairSlab.alloc(size_t(std::max(nP, 1)) * kPairSlabHost));
-  const bool periodic = P.periodicX || P.periodicY;
+  const bool periodic = P.periodicX || P.periodicY || P.periodicSector;
   for (int g = 0; g < 3; ++g) {
-    const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY);
+    const bool on = (g == 0 && P.periodicX) || (g == 1 && P.periodicY) || (g == 2 && P.periodicX && P.periodicY) ||
+                    P.periodicSector;
     const size_t n = on ? size_t(std::max(nP, 1)) : 1;
     HIPCHK(c->dGPos[g].alloc(n));
+    HIPCHK(c->dGQuat[g].alloc(P.periodicSector ? n : 1));
+    HIPCHK(c->dGVel[g].alloc(P.periodicSector ? n : 1));
   }
+  HIPCHK(c->dQuatDet.alloc(P.periodicSector ? size_t(std::max(nP, 1)) : 1));
 
   // ---- pairs and contacts ----
   long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);

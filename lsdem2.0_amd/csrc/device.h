@@ -75,7 +75,7 @@ struct Params {
                       // kick, no drift, no contact law; springs carried over as found)
   int callStep;       // index of this step within the lsdem_step call (recorded with an error flag)
   int barrierSpin;    // poll limit of a broad-phase grid barrier before it gives up (time-out error)
-  int pad1;
+  int periodicSector; // quarter-sector rotational periodicity about the z axis through gridP.mn
   GridGeom gridP, gridW;
   SearchGrid sg;
   double rMax;        // max particle bounding radius
@@ -137,9 +137,17 @@ struct Params {
   float4* wMotion;       // as motion, for walls
   float4* wQrel;         // as qrel, for walls
 
-  // ---- periodic ghost images (3 sets: +x, +y, +x+y) ----
+  // ---- periodic ghost images (3 sets: +x, +y, +x+y; with sector periodicity, turned by +90, 180
+  // and -90 degrees about the z axis through gridP.mn) ----
   d3* gpos[3];       // image positions, written only for images that exist (key >= 0 in the step
                      // record); the keys live in the step record
+  q4* gquat[3];      // sector images only: orientation and velocity of each existing image
+  d3* gvel[3];
+  q4* quatDet;       // sector periodicity: orientation used for particle-particle detection (before
+                     // this step's turn of a body that left the sector)
+  d3 secW[3];        // rotation vectors (0, 0, angle) of the three sector image sets
+  double secA[3], secB[3];  // their Rodrigues coefficients, computed on the host
+  q4 secR[3];        // their unit quaternions, computed on the host
 
   // ---- search grid of particles ----
   int* cellCount;   // ncell (scratch)

@@ -73,6 +73,24 @@ LSD_HD d3 rotInv(d3 v, q4 q) {
   d3 t = cross(qv, v) * 2.0;
   return v + (t * q.q0) + cross(qv, t);
 }
+// v turned about the axis of w by the rotation whose coefficients A = sin|w| / |w| and
+// B = (1 - cos|w|) / |w|^2 the host computed: v + A (w x v) + B (w x (w x v)), the expression of
+// rotateVector (myVec.h:143-184).
+LSD_HD d3 rodrigues(d3 v, d3 w, double A, double B) {
+  const d3 a = cross(w, v);
+  const d3 b = cross(w, a);
+  return v + A * a + B * b;
+}
+// r (x) q, normalised: the Hamilton product of rotateQuaternion (myQua.h:131-179), r the rotation's
+// unit quaternion computed on the host.
+LSD_HD q4 qrotate(q4 r, q4 q) {
+  q4 o;
+  o.q0 = r.q0 * q.q0 - r.q1 * q.q1 - r.q2 * q.q2 - r.q3 * q.q3;
+  o.q1 = r.q0 * q.q1 + r.q1 * q.q0 + r.q2 * q.q3 - r.q3 * q.q2;
+  o.q2 = r.q0 * q.q2 - r.q1 * q.q3 + r.q2 * q.q0 + r.q3 * q.q1;
+  o.q3 = r.q0 * q.q3 + r.q1 * q.q2 - r.q2 * q.q1 + r.q3 * q.q0;
+  return qnormalize(o);
+}
 LSD_HD d3 mul(s6 m, d3 v) {
   return mk3(m.xx * v.x + m.xy * v.y + m.xz * v.z, m.xy * v.x + m.yy * v.y + m.yz * v.z,
              m.xz * v.x + m.yz * v.y + m.zz * v.z);

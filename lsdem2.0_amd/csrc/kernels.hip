@@ -264,7 +264,33 @@ __device__ __forceinline__ float relChordF(float4 b, float4 q) {
 // +x image (first reference-grid column), bit 1 the +y image, bit 2 the corner image.
 __device__ __forceinline__ int imgShiftMask(const Params& P, d3 p) {
   const i3 c = refCell(P.gridP, p);
+  if (P.periodicSector)  // turned images: +90 (first y cell), 180 (first x and y cells), -90 (first x cell)
+    return (c.y == 0 ? 1 : 0) | (c.x == 0 && c.y == 0 ? 2 : 0) | (c.x == 0 ? 4 : 0);
   return (c.x == 0 ? 1 : 0) | (c.y == 0 ? 2 : 0) | (c.x == 0 && c.y == 0 ? 4 : 0);
+}
+// Column predicate (potInColumn / potOutColumn set) of image set s: with sector periodicity the sets
+// turned by +90, 180 and -90 degrees take the sources of the first y cell, of the corner and of the
+// first x cell.
+__device__ __forceinline__ int imageColumn(const Params& P, int s) { return P.periodicSector ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : s; }
+// A body that left the quarter sector across x = min (y = min) turns by -90 (+90) degrees about the
+// z axis through the corner: position, velocity and orientation (updateVelocityPositionOrientation-
+// OutOfSectorKernel, mySpatialGrid.cu:58-85).  Returns whether it turned.
+__device__ __forceinline__ bool sectorWrap(const Params& P, d3& p, d3& v, q4& q) {
+  const d3 mn = P.gridP.mn;
+  bool turned = false;
+  if (p.x < mn.x && p.y >= mn.y) {
+    v = rodrigues(v, P.secW[2], P.secA[2], P.secB[2]);
+    p = mn + rodrigues(p - mn, P.secW[2], P.secA[2], P.secB[2]);
+    q = qrotate(P.secR[2], q);
+    turned = true;
+  }
+  if (p.y < mn.y && p.x >= mn.x) {
+    v = rodrigues(v, P.secW[0], P.secA[0], P.secB[0]);
+    p = mn + rodrigues(p - mn, P.secW[0], P.secA[0], P.secB[0]);
+    q = qrotate(P.secR[0], q);
+    turned = true;
+  }
+  return turned;
 }
 
 __device__ __forceinline__ void halfKick(const Params& P, int i, d3 F, d3 T) {
@@ -318,7 +344,7 @@ __device__ __forceinline__ BodyIn loadBody(const Params& P, int i) {
   b.xBuild = P.xBuild[i];
   b.xBuildW = P.xBuildW[i];
   b.qBuild = P.qBuild[i];
-  b.imgBuild = (P.periodicX || P.periodicY) ? P.imgBuild[i] : 0;
+  b.imgBuild = (P.periodicX || P.periodicY || P.periodicSector) ? P.imgBuild[i] : 0;
   return b;
 }
 __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& in, d3 F, d3 T, int kicks, int buf) {
@@ -345,16 +371,21 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
     qn = qintegrate(q0, w, P.dt);
     P.quat[i] = qn;
   }
-  const bool periodic = P.periodicX || P.periodicY;
+  const bool sector = P.periodicSector != 0;
+  const bool periodic = P.periodicX || P.periodicY || sector;
   const bool wrapAfter = (P.quirks & 4) != 0;
   const GridGeom& G = P.gridP;
   const d3 ox = mk3(G.mx.x - G.mn.x, 0., 0.), oy = mk3(0., G.mx.y - G.mn.y, 0.);
+  bool turned = false;  // (sector periodicity: the body left the sector and turned this step)
   if (periodic && !wrapAfter) {
     if (P.periodicX) { if (p.x < G.mn.x) p += ox; else if (p.x >= G.mx.x) p -= ox; }
     if (P.periodicY) { if (p.y < G.mn.y) p += oy; else if (p.y >= G.mx.y) p -= oy; }
+    if (sector) turned = sectorWrap(P, p, vel, qn);
   }
   P.posDet[i] = p;
   const d3 pDet = p;
+  const q4 qDet = qn;  // orientation of particle-particle detection (before a turn after the build)
+  if (sector) P.quatDet[i] = qn;
   const int keyRef = refKey(G, p);
   int gk[3] = {-1, -1, -1};  // image keys; an image's position is stored only when it exists
   int unshifted = 0;  // valid images at the particle's own position (particle outside the domain)
@@ -387,27 +418,51 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
       gk[2] = k;
       unshifted += (k >= 0 && !(c.x == 0 && c.y == 0));
     }
+    if (sector) {
+      // (mySpatialGrid.cu:154-212) images turned by +90 degrees for bodies in the first y cell, 180 for
+      // those in the first x and y cells, -90 for those in the first x cell; other copies keep the pose
+      if (wrapAfter) turned = sectorWrap(P, p, vel, qn) || turned;
+      const i3 c = refCell(G, p);
+      for (int s = 0; s < 3; ++s) {
+        const bool on = s == 0 ? c.y == 0 : (s == 1 ? (c.x == 0 && c.y == 0) : c.x == 0);
+        const d3 im = on ? G.mn + rodrigues(p - G.mn, P.secW[s], P.secA[s], P.secB[s]) : p;
+        const int k = ghostKey(P, im);
+        if (k >= 0) {
+          P.gpos[s][i] = im;
+          P.gquat[s][i] = on ? qrotate(P.secR[s], qn) : qn;
+          P.gvel[s][i] = on ? rodrigues(vel, P.secW[s], P.secA[s], P.secB[s]) : vel;
+        }
+        gk[s] = k;
+        unshifted += (k >= 0 && !on);
+      }
+    }
+  }
+  if (turned) {
+    P.vel[i] = vel;
+    P.quat[i] = qn;
   }
   P.pos[i] = p;
   // sources that may hold an unshifted image before the next rebuild
   if (periodic && P.halfSkin > 0.0) {
     unshifted = 0;
     if (potOutOfDomain(P, p, P.halfSkin))
-      for (int s = 0; s < 3; ++s) unshifted += potOutColumn(G, p, s, P.halfSkin);
+      for (int s = 0; s < 3; ++s) unshifted += potOutColumn(G, p, imageColumn(P, s), P.halfSkin);
   }
   if (unshifted) atomicAdd(&P.devInt[ghostCountIdx(buf)], unshifted);
   // rebuild the candidate pairs when a body has moved half the skin since the last build (a
   // periodic wrap always does)
   const double hs2 = P.halfSkin * P.halfSkin;
   const d3 dd = pDet - in.xBuild, dw = p - in.xBuildW;
-  const bool moved = len2(dd) > hs2 || len2(dw) > hs2;
+  // (with sector periodicity the pairs are rebuilt every step: a turned body's images and
+  // detection pose change by a quarter turn)
+  const bool moved = sector || len2(dd) > hs2 || len2(dw) > hs2;
   if (__ballot(moved) && (threadIdx.x & 63) == __builtin_ctzll(__ballot(moved))) atomicOr(&P.devInt[rebuildFlagIdx(buf)], 1);
   // motion since the build for the separation certificates (a step that wraps rebuilds, so posDet
   // and pos move alike on every step that reads this)
   BodyStep st;
   const q4 qb = in.qBuild;
-  st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qn)));
-  P.qrel[i] = relQuat(qb, qn);
+  st.motion = make_float4(float(dd.x), float(dd.y), float(dd.z), float(relChord(qb, qDet)));
+  P.qrel[i] = relQuat(qb, qDet);
   st.key = make_int4(keyRef, -1, -1, -1);
   if (periodic) {
     const int flip = imgShiftMask(P, p) ^ in.imgBuild;
@@ -466,7 +521,7 @@ __device__ void markBuildOne(const Params& P, int i) {
   if (i < P.nP) {
     P.xBuild[i] = P.posDet[i];
     P.xBuildW[i] = P.pos[i];
-    P.qBuild[i] = P.quat[i];
+    P.qBuild[i] = P.periodicSector ? P.quatDet[i] : P.quat[i];
     BodyStep st = P.step[i];
     st.motion = make_float4(0.f, 0.f, 0.f, 0.f);
     P.qrel[i] = make_float4(1.f, 0.f, 0.f, 0.f);
@@ -474,7 +529,7 @@ __device__ void markBuildOne(const Params& P, int i) {
     st.key.z = st.key.z >= 0 ? (st.key.z & ~(1 << kFlipBit)) : -1;
     st.key.w = st.key.w >= 0 ? (st.key.w & ~(1 << kFlipBit)) : -1;
     P.step[i] = st;
-    if (P.periodicX || P.periodicY) P.imgBuild[i] = imgShiftMask(P, P.pos[i]);
+    if (P.periodicX || P.periodicY || P.periodicSector) P.imgBuild[i] = imgShiftMask(P, P.pos[i]);
   }
   if (i < P.nW) {
     P.wBuildPos[i] = P.wpos[i];
@@ -734,10 +789,15 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
     const d3 xA = P.posDet[A];
     for (int w = 0; w < P.nW; ++w) {  // a wall's key is checked each step when the pair is used
       const ShapeDev S = P.shapes[P.wshape[w]];
-      const d3 loc = rotInv(xA - P.wpos[w], P.wquat[w]);
-      const d3 o = mk3(S.ox, S.oy, S.oz);
       const double rs = rA + 2.0 * P.halfSkin;  // the wall and the particle each move < half the skin
       const double rr = rs * (1.0 + 1e-9) + 1e-12;
+      // every point of the grid box lies within boxR of the wall's origin: walls far away (the other
+      // drums of an ensemble) are dropped without turning the centre into the wall's frame
+      const d3 dw = xA - P.wpos[w];
+      const double rb = S.boxR + rr;
+      if (len2(dw) > (rb * rb) * (1.0 + 1e-8)) continue;
+      const d3 loc = rotInv(dw, P.wquat[w]);
+      const d3 o = mk3(S.ox, S.oy, S.oz);
       if (boxDist2(loc, o, S.h, S) > rr * rr) continue;
       bool inside;
       double cmax;
@@ -750,7 +810,7 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
   // the image's grid-box radius.  Images are found through their sources in the particle grid:
   // an image shifted by the set's offset lies near x_A - offset; images left at the source's own
   // position (a source outside the domain) are searched around x_A only when such images exist.
-  if (P.periodicX || P.periodicY) {
+  if (P.periodicX || P.periodicY || P.periodicSector) {
     const float4 g = P.sPRw[t];  // post-wrap centre (images are built from it)
     const float mg = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
     const double reach = (double(g.w) + P.boxRMax + 2.0 * P.halfSkin) * (1.0 + 2e-6) + 1e-6 * double(mg) + 1e-12;
@@ -758,30 +818,41 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
     const double hs = P.halfSkin;
     const float lx = float(G.mx.x - G.mn.x), ly = float(G.mx.y - G.mn.y);
     const bool anyUnshifted = P.devInt[ghostCountIdx(P.curBuf)] > 0;
+    const bool sector = P.periodicSector != 0;
     for (int s = 0; s < 3; ++s) {
-      const bool on = (s == 0 && P.periodicX) || (s == 1 && P.periodicY) || (s == 2 && P.periodicX && P.periodicY);
+      const bool on = sector || (s == 0 && P.periodicX) || (s == 1 && P.periodicY) ||
+                      (s == 2 && P.periodicX && P.periodicY);
       if (!on) continue;
       const float offx = s != 1 ? lx : 0.f, offy = s != 0 ? ly : 0.f;
+      const int col = imageColumn(P, s);
+      // the image of a source at b (shifted by the period, or turned about the corner's z axis)
+      auto image = [&](float4 b) {
+        if (!sector) return make_float4(b.x + offx, b.y + offy, b.z, 0.f);
+        const d3 r = G.mn + rodrigues(mk3(b.x, b.y, b.z) - G.mn, P.secW[s], P.secA[s], P.secB[s]);
+        return make_float4(float(r.x), float(r.y), float(r.z), 0.f);
+      };
       for (int pass = 0; pass < 2; ++pass) {
         const bool shifted = pass == 0;
         if (!shifted && !anyUnshifted) break;
-        const d3 c = shifted ? mk3(double(g.x) - double(offx), double(g.y) - double(offy), double(g.z))
-                             : mk3(g.x, g.y, g.z);
+        // sources whose image lies near x_A lie near the inverse image of x_A (a turn the other way)
+        const int inv = 2 - s;
+        const d3 c = !shifted ? mk3(g.x, g.y, g.z)
+                     : (sector ? G.mn + rodrigues(mk3(g.x, g.y, g.z) - G.mn, P.secW[inv], P.secA[inv], P.secB[inv])
+                               : mk3(double(g.x) - double(offx), double(g.y) - double(offy), double(g.z)));
         if (searchBoxDist2(P.sg, c) > reach * reach) continue;
         visitNear(P.sg, P.cellStart, c, reach, [&](int k) {
           const float4 b = P.sPRw[k];
           const float rb = P.sBoxR[k] + skin;
-          const float4 im = shifted ? make_float4(b.x + offx, b.y + offy, b.z, 0.f) : b;
+          const float4 im = shifted ? image(b) : b;
           if (!spheresMayOverlap(g, im, rb)) return;
           const int j = P.sorted[k];
           const d3 pj = P.pos[j];
           if (shifted) {
-            if (!potInColumn(G, pj, s, hs)) return;
+            if (!potInColumn(G, pj, col, hs)) return;
           } else {
-            if (!potOutOfDomain(P, pj, hs) || !potOutColumn(G, pj, s, hs)) return;
+            if (!potOutOfDomain(P, pj, hs) || !potOutColumn(G, pj, col, hs)) return;
             // one pair per (A, image): the shifted pass already took it
-            const float4 sh = make_float4(b.x + offx, b.y + offy, b.z, 0.f);
-            if (potInColumn(G, pj, s, hs) && spheresMayOverlap(g, sh, rb)) return;
+            if (potInColumn(G, pj, col, hs) && spheresMayOverlap(g, image(b), rb)) return;
           }
           emit(PT_GX + s, j);
         });
@@ -1005,7 +1076,7 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t in
   v.A = A;
   const bool ghost = v.type >= PT_GX;
   v.xA = ghost ? P.pos[v.A] : P.posDet[v.A];
-  v.qA = P.quat[v.A];
+  v.qA = (P.periodicSector && !ghost) ? P.quatDet[v.A] : P.quat[v.A];
   v.sA = P.scale[v.A];
   const ShapeDev SA = P.shapes[P.shape[v.A]];
   v.clusterOff = SA.clusterOff;
@@ -1022,12 +1093,15 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t in
     // keys from the partner's step record (an image key carries its shift-flip bit there)
     const int* keys = reinterpret_cast<const int*>(P.step + j) + 4;  // BodyStep::key as 4 ints
     const int k = keys[v.type == PT_PP ? 0 : 1 + int(v.type - PT_GX)];
-    if (v.type == PT_PP) { v.xB = P.posDet[j]; v.keyB = k; }
-    else {
+    if (v.type == PT_PP) {
+      v.xB = P.posDet[j]; v.keyB = k;
+      v.qB = P.periodicSector ? P.quatDet[j] : P.quat[j];
+    } else {
       v.xB = P.gpos[int(v.type - PT_GX)][j];
       v.keyB = k >= 0 ? (k & ~(1 << kFlipBit)) : -1;
+      v.qB = P.periodicSector ? P.gquat[int(v.type - PT_GX)][j] : P.quat[j];
     }
-    v.qB = P.quat[j]; v.rB = P.radius[j]; v.shapeB = P.shape[j]; v.scB = P.scale[j];
+    v.rB = P.radius[j]; v.shapeB = P.shape[j]; v.scB = P.scale[j];
     v.invSpB = P.invSp[j]; v.goB = P.gorig[j];
   }
   v.valid = 1;
@@ -1241,8 +1315,11 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       const int p = pv[r];
       const bool live = key[r] >= 0;
       bool act = live;
-      // an audit step of the surrogate (full) trusts only certificates of the exact culls
-      if (live && !fresh && (cv[r].x > 0.f || cv[r].w > 0.f) && !(full && signbit(cv[r].z))) {
+      // an audit step of the surrogate (full) trusts only certificates of the exact culls; the
+      // turned images of sector periodicity are evaluated every step (their motion bound would need
+      // the turn)
+      if (live && !fresh && (cv[r].x > 0.f || cv[r].w > 0.f) && !(full && signbit(cv[r].z)) &&
+          !(P.periodicSector && tv[r] >= PT_GX)) {
         float4 c = cv[r];
         c.z = fabsf(c.z);
         const bool flip = tv[r] >= PT_GX && ((key[r] >> kFlipBit) & 1);
@@ -1558,7 +1635,10 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
     res = fmin(P.res[A], P.wres[j]);
   } else {
     xf = (type == PT_PP) ? P.pos[j] : P.gpos[type - PT_GX][j];
-    vB = P.vel[j]; wB = P.angvel[j];
+    // an image's velocity (a turned image's is turned), the real body's angular velocity
+    // (contactKernel.cu:236-238)
+    vB = (type != PT_PP && P.periodicSector) ? P.gvel[type - PT_GX][j] : P.vel[j];
+    wB = P.angvel[j];
     const double knB = P.kn[j], ksB = P.ks[j], invMB = P.invM[j];
     if (knA > 0. && knB > 0.) kn = knA * knB / (knA + knB);
     if (ksA > 0. && ksB > 0.) ks = ksA * ksB / (ksA + ksB);

@@ -51,6 +51,7 @@ class Config(C.Structure):
         ("no_recovery", C.c_int32),
         ("barrier_spin", C.c_int32),
         ("broad_phase_coop", C.c_int32),
+        ("periodic_sector", C.c_int32),
     ]
 
 
@@ -187,6 +188,7 @@ class Sim:
         c.no_recovery = int(cfg.get("no_recovery", 0))
         c.barrier_spin = int(cfg.get("barrier_spin", 0))
         c.broad_phase_coop = int(cfg.get("broad_phase_coop", 0))
+        c.periodic_sector = int(cfg.get("periodic_sector", 0))
         c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()

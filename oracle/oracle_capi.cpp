@@ -39,6 +39,11 @@ int lsdem_oracle_create(const lsdem_config* cfg, void** out) {
   c->s.dt = cfg->dt;
   c->s.perX = cfg->periodic_x != 0;
   c->s.perY = cfg->periodic_y != 0;
+  c->s.perSector = cfg->periodic_sector != 0;
+  if (c->s.perSector && (c->s.perX || c->s.perY)) {
+    delete c;
+    return fail(LSDEM_ERR_ARG, "sector periodicity with x/y periodicity");
+  }
   c->s.quirks = cfg->quirks;
   *out = c;
   return LSDEM_OK;
