@@ -289,6 +289,87 @@ def drums(args):
                       "broad_phase": bp, **tuned, "unit": "particle-steps/s", "dtype": "f64", "data": "synthetic"}))
 
 
+def ensemble(args):
+    """K independent rotating drums in ONE simulation (drums far apart in one domain, one wall body
+    each turning at its own speed; per-drum friction and restitution on its particles and wall), so
+    one step of 4 launches, replayed from a captured HIP graph, advances every drum.  Drum k's count
+    is lognormal (median 4,000, within 500-20,000), scaled so that 20,000 fills half of the drum's
+    lattice sites (a 0.1 m x 0.05 m drum holds a few hundred 3-5 mm particles); superellipsoids with
+    exponent U[2,6] and semi-axes 3-5 mm, SDF 32^3, 300 nodes; 1-30 rpm; mu U[0.1,0.8], e U[0.3,0.9];
+    dt 1e-5.  Against it: the same drums as separate simulations stepped one after another."""
+    import __graft_entry__ as g
+    g.load_package()
+    import lsdem_amd.lsdem as L
+    import lsdem_amd.scenes as S
+    K = args.ensemble
+    rng = np.random.default_rng(args.seed)
+    counts = np.clip(rng.lognormal(np.log(4000.0), 0.8, K), 500, 20000)
+    sc = S.scene_drum_ensemble(counts, rng.uniform(1, 30, K), rng.uniform(0.1, 0.8, K), rng.uniform(0.3, 0.9, K),
+                               seed=args.seed, max_count=20000)
+    sizes = np.bincount(sc.meta["drum_of"], minlength=K)
+    n = sc.n_particles
+    log(f"[bench] ensemble: {K} drums, {n} particles (per drum {sizes.min()}-{sizes.max()}, median "
+        f"{int(np.median(sizes))})")
+    cfg = dict(sc.cfg)
+    tuned = {}
+    if args.autotune:
+        import lsdem_amd.autotune as A
+        t = time.perf_counter()
+        best, times = A.tune(L, sc)
+        tuned = {"tuned": best, "tuning_s": time.perf_counter() - t}
+        cfg = A.apply(sc, {}, best)
+        log(f"[bench] autotune: {tuned['tuning_s']:.1f} s -> {best}")
+    chunk = args.ensemble_chunk
+    steps = (args.drum_steps // chunk) * chunk
+    sim = sc.load(L.Sim(cfg))
+    sim.step(10)  # first steps (builds) outside the timing
+    sim.step_graph(chunk)  # capture and first replay outside the timing
+    t = time.perf_counter()
+    for _ in range(steps // chunk):
+        sim.step_graph(chunk)
+    graph_s = time.perf_counter() - t
+    log(f"[bench] ensemble graph replays: {1e3 * graph_s / steps:.4f} ms/step")
+    t = time.perf_counter()
+    sim.step(steps)
+    eager_s = time.perf_counter() - t
+    # work and phase split of a further stretch of eager steps (events on every 10th step)
+    m = max(10, steps // 4)
+    sim.profile_begin((m + 9) // 10, every=10)
+    sim.step(m)
+    prof = sim.profile_end()
+    work = {k: prof[k] / m for k in ("pairs", "evaluated_pairs", "master_nodes", "cluster_tests", "node_tests",
+                                     "sdf_queries", "contacts", "rebuilds")}
+    phases = {k: v / max(1, prof["sampled_steps"]) for k, v in prof["ms"].items()}
+    lists = {name: len(sim.contacts(l)["master"]) for l, name in ((0, "pp"), (1, "pw"))}
+    log(f"[bench] ensemble work per step: {work}; node contacts {lists}")
+    del sim
+    seq = None
+    if args.ensemble_seq_steps > 0:
+        # the same drums one after another, each a simulation of its own on the plain path
+        m = args.ensemble_seq_steps
+        t_all = 0.0
+        for k in range(K):
+            one = S.scene_drum_single(sc, k)
+            s1 = one.load(L.Sim(one.cfg))
+            s1.step(10)
+            t = time.perf_counter()
+            s1.step(m)
+            t_all += time.perf_counter() - t
+            del s1
+        seq = n * m / t_all
+        log(f"[bench] sequential: {K} drums x {m} steps in {t_all:.2f} s")
+    value = n * steps / graph_s
+    print(json.dumps({"metric": "aggregate particle-steps/s of independent rotating drums in one simulation, one GPU",
+                      "value": value, "unit": "particle-steps/s", "drums": K, "particles": int(n),
+                      "per_drum": {"min": int(sizes.min()), "median": float(np.median(sizes)), "max": int(sizes.max())},
+                      "steps": steps, "ms_per_step": 1e3 * graph_s / steps, "graph_chunk": chunk,
+                      "eager": n * steps / eager_s, "sequential": seq,
+                      "sequential_sample_steps": args.ensemble_seq_steps,
+                      "speedup_vs_sequential": value / seq if seq else None, **tuned,
+                      "work_per_step": work, "phases_ms_per_step": phases, "node_contacts": lists,
+                      "dtype": "f64", "data": "synthetic"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +400,12 @@ def main():
                          "captured HIP graphs, one host thread per drum, not run by default: a host-side fault was "
                          "once seen in that mode under a profiler's kernel trace and its cause is not established; "
                          "sequential: one drum after another)")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="K",
+                    help="instead of the main line: K independent rotating drums in one simulation, stepped by "
+                         "graph replays (--drum-steps steps), against the same drums one after another")
+    ap.add_argument("--ensemble-chunk", type=int, default=100, help="with --ensemble: steps per captured graph")
+    ap.add_argument("--ensemble-seq-steps", type=int, default=500,
+                    help="with --ensemble: steps per drum of the sequential comparison (0: skip it)")
     ap.add_argument("--autotune", action="store_true",
                     help="with --drums: tune block size, narrow batch and skin per size bucket, then run again")
     ap.add_argument("--tune-edges", type=int, nargs="*", default=[450],
@@ -333,13 +420,16 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {args.calibrate_copy} MiB x 10")
         return
-    if args.drums:
-        drums(args)
-        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.ensemble:
+        ensemble(args)
+        return
+    if args.drums:
+        drums(args)
+        return
 
     import __graft_entry__ as g
     g.load_package()

@@ -328,6 +328,100 @@ def scene_drum(n: int = 4000, seed: int = 20261206, radius: float = 0.1, depth: 
                  meta={"name": "drum", "seed": seed, "rpm": rpm, "mu": mu, "e": restitution})
 
 
+def scene_drum_ensemble(counts, rpms, mus, restitutions, seed: int = 20261208, radius: float = 0.1,
+                        depth: float = 0.05, n_types: int = 8, n_grid: int = 32, n_nodes: int = 300,
+                        semi_axes=(3e-3, 5e-3), exponents=(2.0, 6.0), wall_spacing: float = 1e-3, dt: float = 1e-5,
+                        kn: float = 1e5, density: float = 2500.0, gravity: float = 9.81, fill: float = 0.5,
+                        gap: float = 0.06, max_count=None) -> Scene:
+    """Independent rotating drums in one simulation: drum k (speed rpms[k], friction mus[k],
+    restitution restitutions[k]) holds up to counts[k] superellipsoids on a jittered lattice in its
+    lower part, as scene_drum does (capped by what `fill` of its lattice sites holds; with max_count,
+    the counts are first scaled so that max_count maps to that cap, keeping their spread).  The drums share
+    a pool of n_types shapes and one drum-wall shape, and sit on a square array in the x-z plane, their
+    grid boxes `gap` apart, so that no particle of one drum ever lies in another drum's grid box: every
+    drum evolves exactly as it would alone, and one step of the simulation steps all of them.
+    meta["drum_of"] gives each particle's drum, meta["centers"] the drum centres."""
+    rng = np.random.default_rng(seed)
+    shapes: list = []
+    types = []
+    for _ in range(n_types):
+        ax = rng.uniform(semi_axes[0], semi_axes[1], 3)
+        p1, p2 = rng.uniform(exponents[0], exponents[1], 2)
+        ee, en = 2.0 / p1, 2.0 / p2
+        h = 2.0 * ax.max() / (n_grid - 5)
+        sdf, org = superellipsoid_grid(*ax, ee, en, n_grid, h)
+        sid = _add_shape(shapes, sdf, org, h, superellipsoid_nodes(*ax, ee, en, n_nodes), density)
+        types.append((sid, float(np.linalg.norm(shapes[sid]["nodes"], axis=1).max())))
+    rb = max(t[1] for t in types)
+    pitch = 2.0 * rb * 1.01
+    g = np.arange(-radius + 1.01 * rb, radius - rb, pitch)
+    gy = np.arange(-0.5 * depth + 1.01 * rb, 0.5 * depth - rb, pitch)
+    X, Y, Z = np.meshgrid(g, gy, g, indexing="ij")
+    site = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    site = site[np.sqrt(site[:, 0] ** 2 + site[:, 2] ** 2) <= radius - 1.01 * rb]
+    site = site[np.argsort(site[:, 2], kind="stable")]  # lowest first
+    wsdf, wo, wh = drum_wall_shape(radius, depth, wall_spacing)
+    wid = _add_shape(shapes, wsdf, wo, wh, np.zeros((0, 3)))
+    half_box = 0.5 * (wsdf.shape[2] - 1) * wh  # the wall grid's half-width in x and z
+    K = len(counts)
+    side = int(np.ceil(np.sqrt(K)))
+    step = 2.0 * half_box + gap
+    centers = np.array([((k % side) * step, 0.0, (k // side) * step) for k in range(K)])
+    parts, walls, drum_of = [], _walls_dict(K), []
+    cap = int(fill * len(site))
+    scale = 1.0 if max_count is None else cap / float(max_count)
+    for k in range(K):
+        n = int(min(max(1, round(counts[k] * scale)), cap))
+        P = _particles(n)
+        kind = rng.integers(0, n_types, n)
+        P["pos"] = centers[k] + site[:n] + rng.uniform(-0.002 * pitch, 0.002 * pitch, (n, 3))
+        P["quat"] = random_quaternions(rng, n)
+        for t, (sid, _) in enumerate(types):
+            m = kind == t
+            S = shapes[sid]
+            P["shape"][m] = sid
+            P["inv_mass"][m] = 1.0 / S["mass"]
+            P["inv_inertia"][m] = S["inv_inertia"]
+        P["kn"][:] = kn
+        P["ks"][:] = 0.4 * kn
+        P["mu"][:] = mus[k]
+        P["e"][:] = restitutions[k]
+        parts.append(P)
+        drum_of.append(np.full(n, k, np.int32))
+        walls["shape"][k] = wid
+        walls["radius"][k] = np.sqrt(radius ** 2 + (0.5 * depth) ** 2)
+        walls["pos"][k] = centers[k]
+        walls["quat"][k] = (1.0, 0.0, 0.0, 0.0)
+        walls["angvel"][k] = (0.0, rpms[k] * 2.0 * np.pi / 60.0, 0.0)
+        walls["mu"][k] = mus[k]
+        walls["e"][k] = restitutions[k]
+    P = {key: np.concatenate([q[key] for q in parts]) for key in parts[0]}
+    m = half_box + wall_spacing
+    lo = centers.min(axis=0) - m
+    hi = centers.max(axis=0) + m
+    lo[1], hi[1] = -0.5 * depth - 4 * wall_spacing, 0.5 * depth + 4 * wall_spacing
+    cfg = {"domain_min": tuple(lo), "domain_max": tuple(hi), "gravity": (0.0, 0.0, -gravity), "dt": dt}
+    return Scene(cfg=cfg, shapes=shapes, particles=P, walls=walls,
+                 meta={"name": "drum ensemble", "seed": seed, "drum_of": np.concatenate(drum_of),
+                       "centers": centers, "rpm": list(rpms), "mu": list(mus), "e": list(restitutions)})
+
+
+def scene_drum_single(ens: Scene, k: int) -> Scene:
+    """Drum k of an ensemble as a simulation of its own (same shapes, particles, wall and time step;
+    the domain shrunk around it)."""
+    sel = ens.meta["drum_of"] == k
+    P = {key: v[sel].copy() for key, v in ens.particles.items()}
+    W = {key: v[k:k + 1].copy() for key, v in ens.walls.items()}
+    c = ens.meta["centers"][k]
+    lo, hi = np.array(ens.cfg["domain_min"]), np.array(ens.cfg["domain_max"])
+    half = 0.5 * (hi[0] - lo[0] - (ens.meta["centers"][:, 0].max() - ens.meta["centers"][:, 0].min()))
+    dlo = np.array([c[0] - half, lo[1], c[2] - half])
+    dhi = np.array([c[0] + half, hi[1], c[2] + half])
+    cfg = dict(ens.cfg)
+    cfg["domain_min"], cfg["domain_max"] = tuple(dlo), tuple(dhi)
+    return Scene(cfg=cfg, shapes=ens.shapes, particles=P, walls=W, meta={"name": f"drum {k}"})
+
+
 # ----------------------------------------------------------------------------------------------
 # Config 2: 200,000 superquadrics of 10 types, periodic x/y (0.3 m cell), floor wall
 # ----------------------------------------------------------------------------------------------

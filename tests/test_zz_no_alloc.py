@@ -1,8 +1,9 @@
 """The allocation-free surrogate gradient (lsdem_surrogate_gradient_noalloc) against the
 allocating entry point lsdem_surrogate_gradient: identical gradients and loss, bit for bit, on two
 calls in a row (the second with fewer samples, so it runs on what the first left in the shared
-scratch), plus its capacity and shape-id errors.  Inputs and outputs are device buffers (torch
-tensors on the context's device); the calls go through the C ABI."""
+scratch), plus its capacity and shape-id errors.  Inputs and outputs are device buffers allocated
+through the HIP runtime the library itself runs on (no second runtime in the process); the calls go
+through the C ABI."""
 import ctypes as C
 import sys
 from pathlib import Path
@@ -37,29 +38,53 @@ def _samples(rng, n, half):
     return p, shape, t
 
 
-def _noalloc(sim, torch, p, shape, t, n_param):
-    dev = torch.device("cuda", 0)
-    dp = torch.from_numpy(p).to(dev)
-    ds = torch.from_numpy(shape).to(dev)
-    dt = torch.from_numpy(t).to(dev)
-    dg = torch.full((n_param,), float("nan"), dtype=torch.float32, device=dev)
-    dl = torch.full((1,), float("nan"), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    rc = sim.lib.lsdem_surrogate_gradient_noalloc(sim.ctx, C.c_int64(len(shape)), C.c_void_p(dp.data_ptr()),
-                                                  C.c_void_p(ds.data_ptr()), C.c_void_p(dt.data_ptr()),
-                                                  C.c_void_p(dg.data_ptr()), C.c_void_p(dl.data_ptr()))
-    return rc, dg.cpu().numpy(), float(dl.cpu().numpy()[0])
+class _Hip:
+    """hipMalloc / hipMemcpy / hipFree of the HIP runtime already loaded with the library."""
+    H2D, D2H = 1, 2
+
+    def __init__(self):
+        self.rt = C.CDLL("libamdhip64.so.7")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+
+    def put(self, a: np.ndarray) -> C.c_void_p:
+        a = np.ascontiguousarray(a)
+        ptr = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(ptr), max(a.nbytes, 1)) == 0
+        assert self.rt.hipMemcpy(ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, self.H2D) == 0
+        return ptr
+
+    def get(self, ptr: C.c_void_p, like: np.ndarray) -> np.ndarray:
+        out = np.empty_like(like)
+        assert self.rt.hipMemcpy(out.ctypes.data_as(C.c_void_p), ptr, out.nbytes, self.D2H) == 0
+        return out
+
+    def free(self, *ptrs):
+        for q in ptrs:
+            assert self.rt.hipFree(q) == 0
+
+
+def _noalloc(sim, hip, p, shape, t, n_param):
+    g0 = np.full(n_param, np.nan, np.float32)
+    l0 = np.full(1, np.nan, np.float64)
+    dp, ds, dt, dg, dl = (hip.put(x) for x in (p, shape, t, g0, l0))
+    try:
+        rc = sim.lib.lsdem_surrogate_gradient_noalloc(sim.ctx, C.c_int64(len(shape)), dp, ds, dt, dg, dl)
+        return rc, hip.get(dg, g0), float(hip.get(dl, l0)[0])
+    finally:
+        hip.free(dp, ds, dt, dg, dl)
 
 
 def test_noalloc_gradient_equals_allocating_entry_point(L, S):
-    import torch
     sim, half = _setup(L, S)
+    hip = _Hip()
     n_param = L.SURROGATE_N_PARAM + 16 * 3
     rng = np.random.default_rng(5)
     for n in (4096, 1280):  # the second call is smaller than the first
         p, shape, t = _samples(rng, n, half)
         loss_ref, g_ref = sim.surrogate_gradient(p, shape, t, 3)
-        rc, g, loss = _noalloc(sim, torch, p, shape, t, n_param)
+        rc, g, loss = _noalloc(sim, hip, p, shape, t, n_param)
         print(f"n={n}: rc={rc} loss {loss!r} vs {loss_ref!r}", flush=True)
         assert rc == 0, sim._err()
         assert np.array_equal(g.view(np.uint32), g_ref.view(np.uint32)), f"n={n}: gradient differs"
@@ -67,18 +92,18 @@ def test_noalloc_gradient_equals_allocating_entry_point(L, S):
 
 
 def test_noalloc_errors(L, S):
-    import torch
     sim, half = _setup(L, S)
+    hip = _Hip()
     n_param = L.SURROGATE_N_PARAM + 16 * 3
     p, shape, t = _samples(np.random.default_rng(9), 64 * 400, half)  # beyond the 16 MiB scratch
-    rc, _, _ = _noalloc(sim, torch, p, shape, t, n_param)
+    rc, _, _ = _noalloc(sim, hip, p, shape, t, n_param)
     assert rc == L.LSDEM_ERR_CAPACITY and "scratch" in sim._err()
     p, shape, t = _samples(np.random.default_rng(9), 128, half)
     shape[17] = 7  # no such shape in the family
-    rc, _, _ = _noalloc(sim, torch, p, shape, t, n_param)
+    rc, _, _ = _noalloc(sim, hip, p, shape, t, n_param)
     assert rc == L.LSDEM_ERR_ARG and "family" in sim._err()
     # a valid call afterwards still equals the allocating entry point
     shape[17] = 1
     loss_ref, g_ref = sim.surrogate_gradient(p, shape, t, 3)
-    rc, g, loss = _noalloc(sim, torch, p, shape, t, n_param)
+    rc, g, loss = _noalloc(sim, hip, p, shape, t, n_param)
     assert rc == 0 and np.array_equal(g.view(np.uint32), g_ref.view(np.uint32)) and loss == loss_ref
