@@ -311,24 +311,30 @@ def ensemble(args):
     log(f"[bench] ensemble: {K} drums, {n} particles (per drum {sizes.min()}-{sizes.max()}, median "
         f"{int(np.median(sizes))})")
     cfg = dict(sc.cfg)
+    for kv in args.ensemble_cfg:  # lsdem_config overrides, e.g. broad_phase_launches=1
+        k, v = kv.split("=", 1)
+        cfg[k] = float(v) if "." in v or "e" in v else int(v)
     tuned = {}
     if args.autotune:
         import lsdem_amd.autotune as A
         t = time.perf_counter()
-        best, times = A.tune(L, sc)
+        base = {k: v for k, v in cfg.items() if k not in sc.cfg}
+        best, times = A.tune(L, sc, base=base)
         tuned = {"tuned": best, "tuning_s": time.perf_counter() - t}
-        cfg = A.apply(sc, {}, best)
+        cfg = A.apply(sc, base, best)
         log(f"[bench] autotune: {tuned['tuning_s']:.1f} s -> {best}")
-    chunk = args.ensemble_chunk
-    steps = (args.drum_steps // chunk) * chunk
+    chunk = args.ensemble_chunk  # 0: eager steps only
+    steps = (args.drum_steps // chunk) * chunk if chunk else args.drum_steps
     sim = sc.load(L.Sim(cfg))
     sim.step(10)  # first steps (builds) outside the timing
-    sim.step_graph(chunk)  # capture and first replay outside the timing
-    t = time.perf_counter()
-    for _ in range(steps // chunk):
-        sim.step_graph(chunk)
-    graph_s = time.perf_counter() - t
-    log(f"[bench] ensemble graph replays: {1e3 * graph_s / steps:.4f} ms/step")
+    graph_s = None
+    if chunk:
+        sim.step_graph(chunk)  # capture and first replay outside the timing
+        t = time.perf_counter()
+        for _ in range(steps // chunk):
+            sim.step_graph(chunk)
+        graph_s = time.perf_counter() - t
+        log(f"[bench] ensemble graph replays: {1e3 * graph_s / steps:.4f} ms/step")
     t = time.perf_counter()
     sim.step(steps)
     eager_s = time.perf_counter() - t
@@ -358,11 +364,12 @@ def ensemble(args):
             del s1
         seq = n * m / t_all
         log(f"[bench] sequential: {K} drums x {m} steps in {t_all:.2f} s")
-    value = n * steps / graph_s
+    timed_s = graph_s if graph_s is not None else eager_s
+    value = n * steps / timed_s
     print(json.dumps({"metric": "aggregate particle-steps/s of independent rotating drums in one simulation, one GPU",
                       "value": value, "unit": "particle-steps/s", "drums": K, "particles": int(n),
                       "per_drum": {"min": int(sizes.min()), "median": float(np.median(sizes)), "max": int(sizes.max())},
-                      "steps": steps, "ms_per_step": 1e3 * graph_s / steps, "graph_chunk": chunk,
+                      "steps": steps, "ms_per_step": 1e3 * timed_s / steps, "graph_chunk": chunk,
                       "eager": n * steps / eager_s, "sequential": seq,
                       "sequential_sample_steps": args.ensemble_seq_steps,
                       "speedup_vs_sequential": value / seq if seq else None, **tuned,
@@ -403,7 +410,10 @@ def main():
     ap.add_argument("--ensemble", type=int, default=0, metavar="K",
                     help="instead of the main line: K independent rotating drums in one simulation, stepped by "
                          "graph replays (--drum-steps steps), against the same drums one after another")
-    ap.add_argument("--ensemble-chunk", type=int, default=100, help="with --ensemble: steps per captured graph")
+    ap.add_argument("--ensemble-chunk", type=int, default=100,
+                    help="with --ensemble: steps per captured graph (0: eager steps only)")
+    ap.add_argument("--ensemble-cfg", nargs="*", default=[], metavar="KEY=VALUE",
+                    help="with --ensemble: lsdem_config entries of the ensemble's simulation")
     ap.add_argument("--ensemble-seq-steps", type=int, default=500,
                     help="with --ensemble: steps per drum of the sequential comparison (0: skip it)")
     ap.add_argument("--autotune", action="store_true",

@@ -41,6 +41,7 @@ class LSParticleDeviceView {
   double3* velocity() const { return reinterpret_cast<double3*>(v_.velocity); }
   double3* angularVelocity() const { return reinterpret_cast<double3*>(v_.angular_velocity); }
   const double3* position() const { return reinterpret_cast<const double3*>(v_.position); }
+  const quaternion* orientation() const { return reinterpret_cast<const quaternion*>(v_.quaternion); }
   size_t num_device() const { return size_t(v_.num); }
   size_t gridDim() const { return size_t(v_.grid_dim); }
   size_t blockDim() const { return size_t(v_.block_dim); }
@@ -49,6 +50,24 @@ class LSParticleDeviceView {
 
  private:
   lsdem_device_view v_{};
+};
+
+// The walls' device arrays, as getWall() returns them (LSSolver.h:512; the LSParticle accessors of
+// LSParticle.h:660-671).  Walls are kinematic here: no force or torque arrays.
+class LSWallDeviceView {
+ public:
+  const double3* position() const { return reinterpret_cast<const double3*>(v_.position); }
+  const quaternion* orientation() const { return reinterpret_cast<const quaternion*>(v_.quaternion); }
+  double3* velocity() const { return reinterpret_cast<double3*>(v_.velocity); }
+  double3* angularVelocity() const { return reinterpret_cast<double3*>(v_.angular_velocity); }
+  size_t num_device() const { return size_t(v_.num); }
+  size_t gridDim() const { return size_t(v_.grid_dim); }
+  size_t blockDim() const { return size_t(v_.block_dim); }
+  hipStream_t stream() const { return static_cast<hipStream_t>(v_.stream); }
+  void set(const lsdem_wall_device_view& v) { v_ = v; }
+
+ private:
+  lsdem_wall_device_view v_{};
 };
 
 // Host copies of one contact list of the last step, with the names of the reference's
@@ -355,6 +374,12 @@ class LSSolver {
   virtual void addExternalForceTorque(const double time) { (void)time; }
 
   LSParticleDeviceView& getLSParticle() { return view_; }
+  // (valid once solve() has set the simulation up; empty before)
+  LSWallDeviceView& getWall() {
+    lsdem_wall_device_view v{};
+    if (ctx_ && lsdem_get_wall_device_view(ctx_, &v) == LSDEM_OK) wallView_.set(v);
+    return wallView_;
+  }
   // Contact lists and bonds of the last step, as host copies (the reference returns its
   // SolidInteraction / VBondedInteraction objects, LSSolver.h:510-518).
   LSContactListView& getLSParticleInteraction() {
@@ -420,6 +445,7 @@ class LSSolver {
   double cfgDt_ = 0;
   bool cfgPX_ = false, cfgPY_ = false, cfgPS_ = false;
   LSParticleDeviceView view_;
+  LSWallDeviceView wallView_;
   LSContactListView ppView_, pwView_;
   LSBondListView bondView_;
   std::map<std::string, std::vector<std::pair<double, std::string>>> pvd_;

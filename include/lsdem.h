@@ -67,7 +67,8 @@ typedef struct lsdem_config {
                               device instead of one launch with grid barriers; results do not
                               depend on it */
   int32_t broad_phase_blocks; /* grid of the single-launch broad phase, blocks of 1024 threads (0:
-                              one per 512 particles, at least 8); capped at the CU count */
+                              one per 512 particles or per 16,384 search cells, at least 8); capped
+                              at the CU count */
   int32_t narrow_batch;   /* candidate pairs per wavefront batch of the narrow phase: 1, 2, 4 or 8
                              (0: chosen each step from the number of pairs it evaluates) */
   int32_t no_fused_steps; /* 1: launch each step's kick/drift and its closing sums separately
@@ -168,6 +169,48 @@ int lsdem_begin_phase(lsdem_ctx* ctx);
  * counterpart in the reference. */
 int lsdem_step_graph(lsdem_ctx* ctx, int32_t n_steps);
 
+/* One step split into its phases, for tests and tools that inspect what a phase produced.  Called
+ * in this order, the six make one step with the same results as lsdem_step(ctx, 1); a call out of
+ * order fails with LSDEM_ERR_STATE and so do lsdem_step, lsdem_step_graph and lsdem_begin_phase
+ * while a step driven this way is unfinished.  Each call waits for its launches and reports the
+ * device's error flags (a capacity overflow abandons the step: LSDEM_ERR_CAPACITY, state undefined,
+ * buffers not grown).  A step driven this way rebuilds its cell table and candidate pairs (a step
+ * of lsdem_step reuses them within the skin; contacts and forces are the same either way).
+ *   integrate: first half-kick, drift, periodic wrap, ghost images, cell keys
+ *              (particleIntegrationKernel.cu:42; mySpatialGrid.cu:58-240; the pre-loop build first
+ *              when one is pending, LSSolver.h:1061-1066);
+ *   hash:      search cell of every body, counting sort by cell, cell start table
+ *              (launchUpdateSpatialGridHashStartEnd, buildHashStartEnd.cu:69-139);
+ *   pairs:     candidate pairs (bodies, walls, images) and per-body reaction lists
+ *              (LSParticleContactDetectionKernel.cuh:79-215, count pass);
+ *   narrow:    pair check, node-vs-SDF tests, contact law with tangential history
+ *              (LSParticleContactDetectionKernel.cu:58-115; contactKernel.cuh:4-77);
+ *   forces:    parallel-bond forces (contactKernel.cuh:138-199, contactKernel.cu:343-350);
+ *   reduce:    per-body sums, second half-kick, external-force hook (contactKernel.cu:10-44;
+ *              particleIntegrationKernel.cu:72); ends the step. */
+int lsdem_phase_integrate(lsdem_ctx* ctx);
+int lsdem_phase_hash(lsdem_ctx* ctx);
+int lsdem_phase_pairs(lsdem_ctx* ctx);
+int lsdem_phase_narrow(lsdem_ctx* ctx);
+int lsdem_phase_forces(lsdem_ctx* ctx);
+int lsdem_phase_reduce(lsdem_ctx* ctx);
+
+/* The search grid and cell table of the last broad phase (the reference's hash / start / end
+ * arrays, mySpatialGrid.h:33, buildHashStartEnd.cu:69-139, laid out as this library's grid):
+ * origin and edge of the cubic cells, cells per axis; cell_of[i] the cell of body i's wrapped centre,
+ * clamped to the grid; sorted the bodies by (cell, index); cell_start[c] .. cell_start[c + 1] the
+ * range of cell c in sorted (ncell + 1 entries).  NULL arrays are skipped; cell_start needs
+ * start_capacity >= ncell + 1 (else LSDEM_ERR_CAPACITY). */
+int lsdem_get_cell_table(lsdem_ctx* ctx, double origin[3], double* cell_edge, int32_t dims[3],
+                         int32_t* cell_of, int32_t* sorted, int64_t start_capacity, int32_t* cell_start);
+/* Candidate pairs of the last broad phase, grouped by master body in ascending order: master,
+ * partner (a body, a wall, or the source body of a periodic image) and type (0 body, 1 wall,
+ * 2/3/4 the +x/+y/+xy images or, with sector periodicity, the +90/180/-90 degree images).  Writes at
+ * most capacity records; *count receives the total.  Counterpart of the reference's per-node
+ * candidate counts (LSParticleContactDetectionKernel.cuh:79-140), kept per body pair here. */
+int lsdem_get_pairs(lsdem_ctx* ctx, int64_t capacity, int64_t* count, int32_t* master, int32_t* partner,
+                    int32_t* type);
+
 /* Copy particle state to / from the host; NULL pointers are skipped.  force/torque are the
  * totals of the last step.  Replaces LSParticle::finalize / copyHostToDevice
  * (LSParticle.h:699-743). */
@@ -241,6 +284,23 @@ typedef struct lsdem_device_view {
   int64_t grid_dim, block_dim;   /* one thread per particle at the configured block size */
   void* stream;                  /* hipStream_t the step runs on */
 } lsdem_device_view;
+
+/* Device arrays of the walls, as apps reach them through getWall() (LSSolver.h:512) and its
+ * LSParticle accessors position() / orientation() / velocity() / angularVelocity() and
+ * num_device(), gridDim(), blockDim() (LSParticle.h:660-671): W x 3 doubles (W x 4, scalar first,
+ * for the orientation), on the context's device, in the caller's wall order.  Walls are kinematic:
+ * they carry no force or torque.  A velocity or angular velocity written here (on the stream, from a
+ * hook) moves the wall from the next drift on. */
+typedef struct lsdem_wall_device_view {
+  const double* position;
+  const double* quaternion;
+  double* velocity;
+  double* angular_velocity;
+  int64_t num;
+  int64_t grid_dim, block_dim;
+  void* stream;
+} lsdem_wall_device_view;
+int lsdem_get_wall_device_view(lsdem_ctx* ctx, lsdem_wall_device_view* out);
 
 /* External-force hook: replaces the virtual LSSolver::addExternalForceTorque(double time)
  * (LSSolver.h:508; called at :1009,1042,1079,1117 after all contact and bond forces and before

@@ -19,9 +19,9 @@ __global__ void k_fill_int(int* a, long long n, int v, const int* gate);
 __global__ void k_fill_u64(unsigned long long* a, long long n, unsigned long long v);
 __global__ void k_scan_reduce(const int* in, long long n, int* blockSum, const int* gate);
 __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, int* out, const int* gate);
-__global__ void k_kick_drift(Params P, int forceRebuild);
+template <bool SEC> __global__ void k_kick_drift(Params P, int forceRebuild);
 __global__ void k_search_cells(Params P);
-__global__ void k_reduce_drift(Params P, StepHead H);
+template <bool SEC> __global__ void k_reduce_drift(Params P, StepHead H);
 __global__ void k_broad_phase(Params P);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
@@ -38,8 +38,8 @@ __global__ void k_inherit_certs(Params P);
 __global__ void k_rc_count(Params P);
 __global__ void k_rc_scatter(Params P);
 __global__ void k_gather_particles(Params P);
-__global__ void k_pair_check(Params P, int full);
-template <bool SUR> __global__ void k_candidates(Params P);
+template <bool SEC> __global__ void k_pair_check(Params P, int full);
+template <bool SUR, bool SEC> __global__ void k_candidates(Params P);
 __global__ void k_sur_audit(Params P);
 __global__ void k_sur_masks(Params P);
 size_t candidatesLdsBytes(int maxClusters);
@@ -244,13 +244,14 @@ struct lsdem_ctx {
   DBuf<q4> dQBuild;
   DBuf<int> dImgBuild, dPairRec;
   DBuf<BodyStep> dStep;
-  DBuf<float4> dQrel, dWMotion, dWQrel, dCert, dCertE, dCertQA, dCertQB;
+  DBuf<float4> dQrel, dWMotion, dWSph, dWQrel, dCert, dCertE, dCertQA, dCertQB;
   DBuf<float> dSurSlack;
   DBuf<unsigned char> dBodyHit, dForceSet;
   DBuf<q4> dWBuildQ;
   bool forceRebuild = true;
   bool primePending = false;  // the pre-loop build runs before the next step
   bool haveContacts = false;  // a detection has filled the contact buffers
+  int phaseNext = 0;          // next lsdem_phase_* call of a step driven phase by phase (0: none begun)
   // the state at the start of the running call (see SnapDev), with the host's own step bookkeeping
   struct Snap {
     DBuf<d3> pos, vel, ang, force, torque, wpos, hSpring;
@@ -398,7 +399,7 @@ void lsdem_ctx::releaseAll() {
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
   dQrel.release(); dWQrel.release(); dCertE.release(); dCertQA.release(); dCertQB.release();
-  dWMotion.release(); dCert.release(); dSurSlack.release(); dBodyHit.release(); dForceSet.release();
+  dWMotion.release(); dWSph.release(); dCert.release(); dSurSlack.release(); dBodyHit.release(); dForceSet.release();
   sur.release();
   snap.release();
 }
@@ -492,16 +493,23 @@ void deviceView(lsdem_ctx* c, lsdem_device_view* v) {
   v->stream = c->stream;
 }
 
-// The broad phase as separate launches, each gated on the device rebuild flag.
-template <class Phase>
-int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
+// The broad phase as separate launches, each gated on the device rebuild flag: search cells,
+// counting sort and cell table, then the candidate pairs and reaction lists.
+int hashLaunches(lsdem_ctx* c, const int* gate) {
   Params& P = c->P;
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(k_search_cells, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
   int rc = bucketSort(c, P.scell, P.nP, P.sg.ncell, P.cellCount, P.cellStart, P.rank, P.sorted, gate);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gather_particles, dim3(blocksFor(P.nP, 256)), dim3(256), 0, s, P);
-  phase();
+  return LSDEM_OK;
+}
+
+template <class Phase>
+int pairLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
+  Params& P = c->P;
+  hipStream_t s = c->stream;
+  int rc;
   hipLaunchKernelGGL(k_pairs_find, dim3(blocksFor(P.nP, 128)), dim3(128), 0, s, P);
   hipLaunchKernelGGL(k_backup_pairs, dim3(unsigned(c->numCU * 4)), dim3(256), 0, s, P);
   rc = scan(c, P.pairCount, P.nP, P.pairStart, gate);
@@ -524,6 +532,89 @@ int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
                      (const int*)P.rcTmp, P.rcPair, (const int*)c->dSortBig.p, (const int*)(P.devInt + 2), gate,
                      rcItems);
   return LSDEM_OK;
+}
+
+template <class Phase>
+int broadPhaseLaunches(lsdem_ctx* c, const int* gate, Phase& phase) {
+  const int rc = hashLaunches(c, gate);
+  if (rc) return rc;
+  phase();
+  return pairLaunches(c, gate, phase);
+}
+
+// The narrow phase: pair check, then the exact node tests and the contact law (behind the
+// surrogate's masks when it is on, except on audit steps, which run the exact path on every live
+// pair that holds no certificate of the exact culls and then check the surrogate's decisions).
+// Launches of the kernels instantiated for sector periodicity or not (a compile-time switch in the
+// kernels, so that simulations without it run the code they ran before it existed).
+void launchKickDrift(lsdem_ctx* c, int force) {
+  const Params& P = c->P;
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  if (P.periodicSector)
+    hipLaunchKernelGGL(k_kick_drift<true>, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, force);
+  else
+    hipLaunchKernelGGL(k_kick_drift<false>, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, force);
+}
+void launchReduceDrift(lsdem_ctx* c, const StepHead& H) {
+  const Params& P = c->P;
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  if (P.periodicSector)
+    hipLaunchKernelGGL(k_reduce_drift<true>, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, H);
+  else
+    hipLaunchKernelGGL(k_reduce_drift<false>, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, H);
+}
+template <bool SUR>
+void launchCandidatesKernel(lsdem_ctx* c) {
+  const Params& P = c->P;
+  const unsigned lds = unsigned(candidatesLdsBytes(P.maxClusters));
+  if (P.periodicSector)
+    hipLaunchKernelGGL((k_candidates<SUR, true>), dim3(unsigned(c->narrowBlocks)), dim3(256), lds, c->stream, P);
+  else
+    hipLaunchKernelGGL((k_candidates<SUR, false>), dim3(unsigned(c->narrowBlocks)), dim3(256), lds, c->stream, P);
+}
+
+bool auditStep(lsdem_ctx* c) {
+  const bool useSur = c->sur.enabled && c->sur.trained;
+  return useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
+}
+void launchPairCheck(lsdem_ctx* c, bool audit) {
+  // (a multiple of 8 workgroups: the kernel gives each XCD an eighth of the pairs)
+  const dim3 grid(unsigned((c->numCU * 4 + 7) / 8 * 8));
+  if (c->P.periodicSector)
+    hipLaunchKernelGGL(k_pair_check<true>, grid, dim3(256), 0, c->stream, c->P, audit ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_pair_check<false>, grid, dim3(256), 0, c->stream, c->P, audit ? 1 : 0);
+}
+void launchCandidates(lsdem_ctx* c, bool audit) {
+  const Params& P = c->P;
+  hipStream_t s = c->stream;
+  const bool useSur = c->sur.enabled && c->sur.trained;
+  if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
+  if (useSur && !audit)
+    launchCandidatesKernel<true>(c);
+  else
+    launchCandidatesKernel<false>(c);
+  if (audit) {
+    hipLaunchKernelGGL(k_sur_audit, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P);
+    c->sur.audits++;
+  }
+}
+
+// The end of a step that is not fused with the next one: per-body sums and the second half-kick
+// (split around the external-force hook when one is set).
+void launchReduceKick(lsdem_ctx* c) {
+  Params& P = c->P;
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  hipStream_t s = c->stream;
+  if (c->hookFn) {
+    hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 0);
+    lsdem_device_view v;
+    deviceView(c, &v);
+    c->hookFn(c->hookUser, c->time, &v);
+    hipLaunchKernelGGL(k_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
+  } else {
+    hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 1);
+  }
 }
 
 int launchBroadPhase(lsdem_ctx* c);
@@ -563,7 +654,7 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext, int callStep) {
   if (!headDone) {
     const int force = c->forceRebuild ? 1 : 0;
     c->forceRebuild = false;
-    hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, force);
+    launchKickDrift(c, force);
   }
   phase(!headDone);
   if (c->coopBlocks > 0) {
@@ -577,41 +668,21 @@ int runStep(lsdem_ctx* c, bool headDone, bool fuseNext, int callStep) {
     if (rc) return rc;
   }
   phase(c->coopBlocks == 0);  // reaction lists
-  // narrow phase + contact law (an audit step runs the exact path on every live pair that holds no
-  // certificate of the exact culls)
-  const bool useSur = c->sur.enabled && c->sur.trained;
-  const bool audit = useSur && c->sur.auditEvery > 0 && (c->stepCount % c->sur.auditEvery == 0);
-  // (a multiple of 8 workgroups: the kernel gives each XCD an eighth of the pairs)
-  hipLaunchKernelGGL(k_pair_check, dim3(unsigned((c->numCU * 4 + 7) / 8 * 8)), dim3(256), 0, s, P, audit ? 1 : 0);
+  // narrow phase + contact law
+  const bool audit = auditStep(c);
+  launchPairCheck(c, audit);
   phase();
-  if (useSur && !audit) hipLaunchKernelGGL(k_sur_masks, dim3(unsigned(c->narrowBlocks)), dim3(256), 0, s, P);
-  if (useSur && !audit)
-    hipLaunchKernelGGL(k_candidates<true>, dim3(unsigned(c->narrowBlocks)), dim3(256),
-                       unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
-  else
-    hipLaunchKernelGGL(k_candidates<false>, dim3(unsigned(c->narrowBlocks)), dim3(256),
-                       unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
-  if (audit) {
-    // the step ran the exact path; check every contact against the surrogate's decision
-    hipLaunchKernelGGL(k_sur_audit, dim3(unsigned(c->numCU * 2)), dim3(256), 0, s, P);
-    c->sur.audits++;
-  }
+  launchCandidates(c, audit);
   phase();  // (this step's contacts went into this step's history table inside k_candidates)
   if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, s, P);
   phase(P.nBonds > 0);
-  if (c->hookFn) {
-    hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 0);
-    lsdem_device_view v;
-    deviceView(c, &v);
-    c->hookFn(c->hookUser, c->time, &v);
-    hipLaunchKernelGGL(k_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
-  } else if (fuseNext) {
+  if (fuseNext && !c->hookFn) {
     const int nb = 1 - c->cur;  // the next step's buffers
     const StepHead H{c->dHashKey[nb].p, c->dHashSlot[nb].p, nb, c->forceRebuild ? 1 : 0};
     c->forceRebuild = false;
-    hipLaunchKernelGGL(k_reduce_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, H);
+    launchReduceDrift(c, H);
   } else {
-    hipLaunchKernelGGL(k_reduce_kick, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P, 1);
+    launchReduceKick(c);
   }
   phase();
   if (prof) c->profSteps++;
@@ -649,7 +720,7 @@ int runPrime(lsdem_ctx* c) {
   P.prime = 1;
   P.callStep = 0;
   c->forceRebuild = false;
-  hipLaunchKernelGGL(k_kick_drift, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, s, P, 1);
+  launchKickDrift(c, 1);
   int rc = LSDEM_OK;
   if (c->coopBlocks > 0) {
     rc = launchBroadPhase(c);
@@ -658,9 +729,8 @@ int runPrime(lsdem_ctx* c) {
     rc = broadPhaseLaunches(c, P.devInt + rebuildFlagIdx(P.curBuf), none);
   }
   if (rc == LSDEM_OK) {
-    hipLaunchKernelGGL(k_pair_check, dim3(unsigned((c->numCU * 4 + 7) / 8 * 8)), dim3(256), 0, s, P, 0);
-    hipLaunchKernelGGL(k_candidates<false>, dim3(unsigned(c->narrowBlocks)), dim3(256),
-                       unsigned(candidatesLdsBytes(P.maxClusters)), s, P);
+    launchPairCheck(c, false);
+    launchCandidatesKernel<false>(c);
     hipLaunchKernelGGL(k_prime_end, dim3(blocksFor(P.nP, bs)), dim3(bs), 0, s, P);
   }
   P.prime = 0;
@@ -833,7 +903,7 @@ void bindPointers(lsdem_ctx* c) {
   P.nodeStart = c->dNodeStart.p; P.scell = c->dScell.p;
   P.xBuild = c->dXBuild.p; P.xBuildW = c->dXBuildW.p; P.wBuildPos = c->dWBuildPos.p; P.wBuildQ = c->dWBuildQ.p;
   P.qBuild = c->dQBuild.p; P.imgBuild = c->dImgBuild.p; P.step = c->dStep.p;
-  P.wMotion = c->dWMotion.p; P.cert = c->dCert.p; P.oldInfo = c->dOldInfo.p; P.oldCert = c->dOldCert.p; P.oldStart = c->dOldStart.p; P.pairRec = c->dPairRec.p;
+  P.wMotion = c->dWMotion.p; P.wSph = c->dWSph.p; P.cert = c->dCert.p; P.oldInfo = c->dOldInfo.p; P.oldCert = c->dOldCert.p; P.oldStart = c->dOldStart.p; P.pairRec = c->dPairRec.p;
   P.qrel = c->dQrel.p; P.wQrel = c->dWQrel.p; P.certE = c->dCertE.p; P.certQA = c->dCertQA.p; P.certQB = c->dCertQB.p;
   P.surSlack = c->dSurSlack.p;
   P.bodyHit = c->dBodyHit.p;
@@ -1186,6 +1256,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dWBuildPos.alloc(std::max(nW, 1)));
   HIPCHK(c->dWBuildQ.alloc(std::max(nW, 1)));
   HIPCHK(c->dWMotion.alloc(std::max(nW, 1)));
+  HIPCHK(c->dWSph.alloc(std::max(nW, 1)));
   HIPCHK(c->dWQrel.alloc(std::max(nW, 1)));
 
   // ---- grids ----
@@ -1248,11 +1319,13 @@ int lsdem_initialize(lsdem_ctx* c) {
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k_broad_phase),
                                                         kBroadThreadsHost, 0));
     if (perCU < 1) return fail(LSDEM_ERR_HIP, "the single-launch broad phase does not fit a CU");
-    // one block of 1024 threads per 512 particles, at least 8 and at most one per CU: a small
-    // simulation leaves the rest of the GPU to others stepped concurrently, and its rebuilds meet
-    // at barriers of few blocks
-    c->coopBlocks = int(std::min<long long>(c->numCU, std::max<long long>(8, (nP + 511) / 512)));
-    if (c->cfg.broad_phase_blocks > 0) c->coopBlocks = std::min(c->coopBlocks, int(c->cfg.broad_phase_blocks));
+    // one block of 1024 threads per 512 particles or per 16,384 search cells (the cell loops of a
+    // sparse domain, such as an ensemble of drums far apart), at least 8 and at most one per CU: a
+    // small simulation leaves the rest of the GPU to others stepped concurrently, and its rebuilds
+    // meet at barriers of few blocks
+    const long long byCells = ((long long)P.sg.ncell + 16383) / 16384;
+    c->coopBlocks = int(std::min<long long>(c->numCU, std::max<long long>({8, (nP + 511) / 512, byCells})));
+    if (c->cfg.broad_phase_blocks > 0) c->coopBlocks = std::min(c->numCU, int(c->cfg.broad_phase_blocks));
     c->coopBlocks = std::min(c->coopBlocks, perCU * c->numCU);  // all resident together on an idle GPU
   }
   // the narrow phase's per-wavefront LDS grows with the largest node count of a shape
@@ -1261,10 +1334,14 @@ int lsdem_initialize(lsdem_ctx* c) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, c->cfg.device));
     if (lds > prop.sharedMemPerBlock) {
-      if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_candidates<false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess ||
-          hipFuncSetAttribute(reinterpret_cast<const void*>(k_candidates<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
+      const void* const kernels[4] = {reinterpret_cast<const void*>(k_candidates<false, false>),
+                                      reinterpret_cast<const void*>(k_candidates<true, false>),
+                                      reinterpret_cast<const void*>(k_candidates<false, true>),
+                                      reinterpret_cast<const void*>(k_candidates<true, true>)};
+      bool ok = true;
+      for (const void* k : kernels)
+        ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) == hipSuccess;
+      if (!ok)
         return fail(LSDEM_ERR_ARG, "shapes with " + std::to_string(64 * maxClusters) +
                                        " boundary nodes need " + std::to_string(lds) +
                                        " bytes of LDS per narrow-phase block, more than the device's " +
@@ -1445,6 +1522,7 @@ int runCall(lsdem_ctx* c, Work&& work) {
 int lsdem_step(lsdem_ctx* c, int32_t n) {
   if (!c) return fail(LSDEM_ERR_ARG, "null context");
   if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (c->phaseNext) return fail(LSDEM_ERR_STATE, "a step driven by lsdem_phase_* calls is unfinished");
   if (n < 0) return fail(LSDEM_ERR_ARG, "negative step count");
   HIPCHK(hipSetDevice(c->cfg.device));
   if (c->P.nP == 0) { c->stepCount += n; return LSDEM_OK; }
@@ -1455,6 +1533,7 @@ int lsdem_step(lsdem_ctx* c, int32_t n) {
 int lsdem_begin_phase(lsdem_ctx* c) {
   if (!c) return fail(LSDEM_ERR_ARG, "null context");
   if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (c->phaseNext) return fail(LSDEM_ERR_STATE, "a step driven by lsdem_phase_* calls is unfinished");
   HIPCHK(hipSetDevice(c->cfg.device));
   if (c->P.nP == 0) { c->primePending = false; return LSDEM_OK; }
   return runCall(c, [&]() { return runPrime(c); });
@@ -1463,6 +1542,7 @@ int lsdem_begin_phase(lsdem_ctx* c) {
 int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
   if (!c) return fail(LSDEM_ERR_ARG, "null context");
   if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (c->phaseNext) return fail(LSDEM_ERR_STATE, "a step driven by lsdem_phase_* calls is unfinished");
   if (n < 2 || n % 2) return fail(LSDEM_ERR_ARG, "graph step count must be even and at least 2");
   if (c->hookFn || c->profiling || c->sur.enabled)
     return fail(LSDEM_ERR_STATE, "graph steps run without an external-force hook, profiling or the surrogate");
@@ -1517,6 +1597,160 @@ int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
   HIPCHK(hipStreamSynchronize(c->stream));  // (the restore still reads the old snapshot)
   if ((rc = snapAlloc(c))) return rc;
   return lsdem_step(c, n);
+}
+
+}  // extern "C"
+
+namespace {
+
+const char* const kPhaseNames[6] = {"integrate", "hash", "pairs", "narrow", "forces", "reduce"};
+
+// Checks that phase k of a step driven phase by phase is the one due.
+int phaseCheck(lsdem_ctx* c, int k) {
+  if (!c) return fail(LSDEM_ERR_ARG, "null context");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  if (c->phaseNext != k)
+    return fail(LSDEM_ERR_STATE, std::string("lsdem_phase_") + kPhaseNames[k] + " called out of order (next: " +
+                                     kPhaseNames[c->phaseNext] + ")");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  return LSDEM_OK;
+}
+
+// Waits for the phase's launches and reports the device's error flags; an error abandons the step.
+int phaseDone(lsdem_ctx* c, int next) {
+  int flags = 0, step = -1, rc;
+  if ((rc = readFlags(c, flags, step))) {
+    c->phaseNext = 0;
+    return rc;
+  }
+  if (flags) {
+    const int err = reportFlags(c, flags, step);
+    HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
+    HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
+    c->forceRebuild = true;
+    c->phaseNext = 0;
+    return err;
+  }
+  c->phaseNext = next;
+  return LSDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsdem_phase_integrate(lsdem_ctx* c) {
+  int rc;
+  if ((rc = phaseCheck(c, 0))) return rc;
+  Params& P = c->P;
+  if (P.nP == 0) { c->phaseNext = 1; return LSDEM_OK; }
+  if (c->primePending) {
+    if ((rc = runPrime(c))) return rc;
+    if ((rc = phaseDone(c, 0))) return rc;
+  }
+  setCurrentBuffers(c);
+  P.callStep = 0;
+  c->forceRebuild = false;
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  // (a rebuild requested from the host: the hash and pairs phases run on every such step)
+  launchKickDrift(c, 1);
+  return phaseDone(c, 1);
+}
+
+int lsdem_phase_hash(lsdem_ctx* c) {
+  int rc;
+  if ((rc = phaseCheck(c, 1))) return rc;
+  if (c->P.nP == 0) { c->phaseNext = 2; return LSDEM_OK; }
+  if ((rc = hashLaunches(c, c->P.devInt + rebuildFlagIdx(c->P.curBuf)))) return rc;
+  return phaseDone(c, 2);
+}
+
+int lsdem_phase_pairs(lsdem_ctx* c) {
+  int rc;
+  if ((rc = phaseCheck(c, 2))) return rc;
+  if (c->P.nP == 0) { c->phaseNext = 3; return LSDEM_OK; }
+  auto none = [](bool = true) {};
+  if ((rc = pairLaunches(c, c->P.devInt + rebuildFlagIdx(c->P.curBuf), none))) return rc;
+  return phaseDone(c, 3);
+}
+
+int lsdem_phase_narrow(lsdem_ctx* c) {
+  int rc;
+  if ((rc = phaseCheck(c, 3))) return rc;
+  if (c->P.nP == 0) { c->phaseNext = 4; return LSDEM_OK; }
+  const bool audit = auditStep(c);
+  launchPairCheck(c, audit);
+  launchCandidates(c, audit);
+  return phaseDone(c, 4);
+}
+
+int lsdem_phase_forces(lsdem_ctx* c) {
+  int rc;
+  if ((rc = phaseCheck(c, 4))) return rc;
+  const Params& P = c->P;
+  if (P.nP == 0) { c->phaseNext = 5; return LSDEM_OK; }
+  const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  if (P.nBonds > 0) hipLaunchKernelGGL(k_bonds, dim3(blocksFor(P.nBonds, bs)), dim3(bs), 0, c->stream, P);
+  return phaseDone(c, 5);
+}
+
+int lsdem_phase_reduce(lsdem_ctx* c) {
+  int rc;
+  if ((rc = phaseCheck(c, 5))) return rc;
+  if (c->P.nP == 0) {
+    c->phaseNext = 0;
+    c->stepCount++;
+    return LSDEM_OK;
+  }
+  launchReduceKick(c);
+  if ((rc = phaseDone(c, 0))) return rc;
+  c->cur = 1 - c->cur;
+  c->stepCount++;
+  c->time += c->P.dt;
+  c->haveContacts = true;
+  return LSDEM_OK;
+}
+
+int lsdem_get_cell_table(lsdem_ctx* c, double origin[3], double* cell_edge, int32_t dims[3], int32_t* cell_of,
+                         int32_t* sorted, int64_t start_capacity, int32_t* cell_start) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const SearchGrid& S = c->P.sg;
+  if (origin) { origin[0] = S.mn.x; origin[1] = S.mn.y; origin[2] = S.mn.z; }
+  if (cell_edge) *cell_edge = S.cell;
+  if (dims) { dims[0] = S.size.x; dims[1] = S.size.y; dims[2] = S.size.z; }
+  if (cell_start && start_capacity < (int64_t)S.ncell + 1)
+    return fail(LSDEM_ERR_CAPACITY, "cell_start needs " + std::to_string((long long)S.ncell + 1) + " entries");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t n = size_t(c->P.nP);
+  if (n && cell_of) HIPCHK(hipMemcpy(cell_of, c->P.scell, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (n && sorted) HIPCHK(hipMemcpy(sorted, c->P.sorted, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (cell_start)
+    HIPCHK(hipMemcpy(cell_start, c->P.cellStart, (size_t(S.ncell) + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  return LSDEM_OK;
+}
+
+int lsdem_get_pairs(lsdem_ctx* c, int64_t capacity, int64_t* count, int32_t* master, int32_t* partner,
+                    int32_t* type) {
+  if (!c || !c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
+  HIPCHK(hipSetDevice(c->cfg.device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const Params& P = c->P;
+  int total = 0;
+  if (P.nP > 0) HIPCHK(hipMemcpy(&total, P.pairStart + P.nP, sizeof(int), hipMemcpyDeviceToHost));
+  if (count) *count = total;
+  const size_t m = size_t(std::max<int64_t>(0, std::min<int64_t>({capacity, (int64_t)total, (int64_t)P.pairCap})));
+  if (!m) return LSDEM_OK;
+  if (master) HIPCHK(hipMemcpy(master, P.pairA, m * sizeof(int), hipMemcpyDeviceToHost));
+  if (partner || type) {
+    std::vector<uint32_t> info(m);
+    HIPCHK(hipMemcpy(info.data(), P.pairInfo, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < m; ++k) {
+      if (partner) partner[k] = int32_t(info[k] & 0x0fffffffu);
+      if (type) type[k] = int32_t(info[k] >> 28);
+    }
+  }
+  return LSDEM_OK;
 }
 
 int lsdem_get_particles(lsdem_ctx* c, double* pos, double* quat, double* vel, double* angvel, double* force,
@@ -1801,6 +2035,21 @@ int lsdem_get_device_view(lsdem_ctx* c, lsdem_device_view* out) {
   if (!c || !out) return fail(LSDEM_ERR_ARG, "null argument");
   if (!c->initialized) return fail(LSDEM_ERR_STATE, "not initialized");
   deviceView(c, out);
+  return LSDEM_OK;
+}
+
+int lsdem_get_wall_device_view(lsdem_ctx* c, lsdem_wall_device_view* out) {
+  if (!c || !out) return fail(LSDEM_ERR_ARG, "null argument");
+  if (!c->initialized) return fail(LSDEM_ERR_STATE, "lsdem_initialize has not been called");
+  const Params& P = c->P;
+  out->position = reinterpret_cast<const double*>(P.wpos);
+  out->quaternion = reinterpret_cast<const double*>(P.wquat);
+  out->velocity = reinterpret_cast<double*>(P.wvel);
+  out->angular_velocity = reinterpret_cast<double*>(P.wangvel);
+  out->num = P.nW;
+  out->block_dim = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+  out->grid_dim = (P.nW + out->block_dim - 1) / out->block_dim;
+  out->stream = c->stream;
   return LSDEM_OK;
 }
 

@@ -135,6 +135,8 @@ struct Params {
   d3* wBuildPos;         // wall pose when the candidate pairs were built
   q4* wBuildQ;
   float4* wMotion;       // as motion, for walls
+  float4* wSph;          // FP32 bounding sphere of each wall's grid box at its current pose (centre,
+                         // boxR), written at every drift: the pair search's first wall test
   float4* wQrel;         // as qrel, for walls
 
   // ---- periodic ghost images (3 sets: +x, +y, +x+y; with sector periodicity, turned by +90, 180

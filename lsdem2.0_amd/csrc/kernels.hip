@@ -262,16 +262,45 @@ __device__ __forceinline__ float relChordF(float4 b, float4 q) {
 
 // Which ghost images of a particle at (wrapped) position p lie a period away from it: bit 0 the
 // +x image (first reference-grid column), bit 1 the +y image, bit 2 the corner image.
+// (SEC: quarter-sector periodicity, known at compile time in the per-step kernels)
+template <bool SEC>
 __device__ __forceinline__ int imgShiftMask(const Params& P, d3 p) {
   const i3 c = refCell(P.gridP, p);
-  if (P.periodicSector)  // turned images: +90 (first y cell), 180 (first x and y cells), -90 (first x cell)
+  if (SEC)  // turned images: +90 (first y cell), 180 (first x and y cells), -90 (first x cell)
     return (c.y == 0 ? 1 : 0) | (c.x == 0 && c.y == 0 ? 2 : 0) | (c.x == 0 ? 4 : 0);
   return (c.x == 0 ? 1 : 0) | (c.y == 0 ? 2 : 0) | (c.x == 0 && c.y == 0 ? 4 : 0);
 }
+// Lanes of the wavefront below this one.
+__device__ __forceinline__ unsigned long long laneLtMask() {
+  const int lane = int(__lane_id());
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Members of Params chosen by an image set known only at run time, read through selects: indexing
+// a by-value kernel argument with a run-time index makes the compiler copy the whole argument block
+// into per-lane scratch memory.
+template <class T>
+__device__ __forceinline__ T pick3(int s, T a, T b, T c) { return s == 0 ? a : (s == 1 ? b : c); }
+// (an image position array indexed directly compiles to one load of the pointer when no other
+// member is indexed at run time: the per-step kernels without sector periodicity keep that form)
+template <bool SEC>
+__device__ __forceinline__ d3* imgPos(const Params& P, int s) {
+  return SEC ? pick3(s, P.gpos[0], P.gpos[1], P.gpos[2]) : P.gpos[s];
+}
+__device__ __forceinline__ q4* imgQuat(const Params& P, int s) { return pick3(s, P.gquat[0], P.gquat[1], P.gquat[2]); }
+__device__ __forceinline__ d3* imgVel(const Params& P, int s) { return pick3(s, P.gvel[0], P.gvel[1], P.gvel[2]); }
+// v turned by the rotation of sector image set s (about the z axis through the origin)
+__device__ __forceinline__ d3 secTurn(const Params& P, int s, d3 v) {
+  return rodrigues(v, pick3(s, P.secW[0], P.secW[1], P.secW[2]), pick3(s, P.secA[0], P.secA[1], P.secA[2]),
+                   pick3(s, P.secB[0], P.secB[1], P.secB[2]));
+}
+__device__ __forceinline__ q4 secQuat(const Params& P, int s) { return pick3(s, P.secR[0], P.secR[1], P.secR[2]); }
+
 // Column predicate (potInColumn / potOutColumn set) of image set s: with sector periodicity the sets
 // turned by +90, 180 and -90 degrees take the sources of the first y cell, of the corner and of the
 // first x cell.
-__device__ __forceinline__ int imageColumn(const Params& P, int s) { return P.periodicSector ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : s; }
+template <bool SEC>
+__device__ __forceinline__ int imageColumn(int s) { return SEC ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : s; }
 // A body that left the quarter sector across x = min (y = min) turns by -90 (+90) degrees about the
 // z axis through the corner: position, velocity and orientation (updateVelocityPositionOrientation-
 // OutOfSectorKernel, mySpatialGrid.cu:58-85).  Returns whether it turned.
@@ -333,6 +362,7 @@ struct BodyIn {
   double invM;
   int imgBuild;
 };
+template <bool SEC>
 __device__ __forceinline__ BodyIn loadBody(const Params& P, int i) {
   BodyIn b;
   b.vel = P.vel[i];
@@ -344,9 +374,12 @@ __device__ __forceinline__ BodyIn loadBody(const Params& P, int i) {
   b.xBuild = P.xBuild[i];
   b.xBuildW = P.xBuildW[i];
   b.qBuild = P.qBuild[i];
-  b.imgBuild = (P.periodicX || P.periodicY || P.periodicSector) ? P.imgBuild[i] : 0;
+  b.imgBuild = (P.periodicX || P.periodicY || SEC) ? P.imgBuild[i] : 0;
   return b;
 }
+// SEC: quarter-sector periodicity (a compile-time switch, so that simulations without it run the code
+// they ran before it existed)
+template <bool SEC>
 __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& in, d3 F, d3 T, int kicks, int buf) {
   d3 vel = in.vel, w = in.w;
   const q4 q0 = in.quat;
@@ -371,7 +404,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
     qn = qintegrate(q0, w, P.dt);
     P.quat[i] = qn;
   }
-  const bool sector = P.periodicSector != 0;
+  constexpr bool sector = SEC;
   const bool periodic = P.periodicX || P.periodicY || sector;
   const bool wrapAfter = (P.quirks & 4) != 0;
   const GridGeom& G = P.gridP;
@@ -425,12 +458,12 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
       const i3 c = refCell(G, p);
       for (int s = 0; s < 3; ++s) {
         const bool on = s == 0 ? c.y == 0 : (s == 1 ? (c.x == 0 && c.y == 0) : c.x == 0);
-        const d3 im = on ? G.mn + rodrigues(p - G.mn, P.secW[s], P.secA[s], P.secB[s]) : p;
+        const d3 im = on ? G.mn + secTurn(P, s, p - G.mn) : p;
         const int k = ghostKey(P, im);
         if (k >= 0) {
-          P.gpos[s][i] = im;
-          P.gquat[s][i] = on ? qrotate(P.secR[s], qn) : qn;
-          P.gvel[s][i] = on ? rodrigues(vel, P.secW[s], P.secA[s], P.secB[s]) : vel;
+          imgPos<SEC>(P, s)[i] = im;
+          imgQuat(P, s)[i] = on ? qrotate(secQuat(P, s), qn) : qn;
+          imgVel(P, s)[i] = on ? secTurn(P, s, vel) : vel;
         }
         gk[s] = k;
         unshifted += (k >= 0 && !on);
@@ -446,7 +479,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
   if (periodic && P.halfSkin > 0.0) {
     unshifted = 0;
     if (potOutOfDomain(P, p, P.halfSkin))
-      for (int s = 0; s < 3; ++s) unshifted += potOutColumn(G, p, imageColumn(P, s), P.halfSkin);
+      for (int s = 0; s < 3; ++s) unshifted += potOutColumn(G, p, imageColumn<SEC>(s), P.halfSkin);
   }
   if (unshifted) atomicAdd(&P.devInt[ghostCountIdx(buf)], unshifted);
   // rebuild the candidate pairs when a body has moved half the skin since the last build (a
@@ -465,7 +498,7 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
   P.qrel[i] = relQuat(qb, qDet);
   st.key = make_int4(keyRef, -1, -1, -1);
   if (periodic) {
-    const int flip = imgShiftMask(P, p) ^ in.imgBuild;
+    const int flip = imgShiftMask<SEC>(P, p) ^ in.imgBuild;
     int* sk = &st.key.y;
     for (int g = 0; g < 3; ++g) sk[g] = gk[g] >= 0 ? (gk[g] | (((flip >> g) & 1) << kFlipBit)) : -1;
   }
@@ -473,13 +506,14 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
 }
 
 // One thread per particle, then one per wall.
+template <bool SEC>
 __global__ void k_kick_drift(Params P, int forceRebuild) {
   LSD_HALT(P);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   BodyIn in;
   d3 F, T;
   if (i < P.nP) {
-    in = loadBody(P, i);
+    in = loadBody<SEC>(P, i);
     F = P.force[i];
     T = P.torque[i];
   }
@@ -488,7 +522,7 @@ __global__ void k_kick_drift(Params P, int forceRebuild) {
     if (i - P.nP < P.nW) wallDrift(P, i - P.nP, P.curBuf);
     return;
   }
-  kickDrift(P, i, in, F, T, 1, P.curBuf);
+  kickDrift<SEC>(P, i, in, F, T, 1, P.curBuf);
 }
 
 __device__ void wallDrift(const Params& P, int w, int buf) {
@@ -510,6 +544,7 @@ __device__ void wallDrift(const Params& P, int w, int buf) {
   if (disp > P.halfSkin) atomicOr(&P.devInt[rebuildFlagIdx(buf)], 1);
   P.wMotion[w] = make_float4(float(dp.x), float(dp.y), float(dp.z), float(chord));
   P.wQrel[w] = relQuat(P.wBuildQ[w], q);
+  P.wSph[w] = make_float4(float(p.x), float(p.y), float(p.z), float(R));
 }
 
 // Record the poses the candidate pairs were built from (rebuild steps only).
@@ -529,7 +564,8 @@ __device__ void markBuildOne(const Params& P, int i) {
     st.key.z = st.key.z >= 0 ? (st.key.z & ~(1 << kFlipBit)) : -1;
     st.key.w = st.key.w >= 0 ? (st.key.w & ~(1 << kFlipBit)) : -1;
     P.step[i] = st;
-    if (P.periodicX || P.periodicY || P.periodicSector) P.imgBuild[i] = imgShiftMask(P, P.pos[i]);
+    if (P.periodicX || P.periodicY || P.periodicSector)
+      P.imgBuild[i] = P.periodicSector ? imgShiftMask<true>(P, P.pos[i]) : imgShiftMask<false>(P, P.pos[i]);
   }
   if (i < P.nW) {
     P.wBuildPos[i] = P.wpos[i];
@@ -678,7 +714,26 @@ __device__ __forceinline__ double searchBoxDist2(const SearchGrid& S, d3 x) {
 
 // Visit every item of a search-grid table lying in cells that can hold a point within `reach`
 // of x.  Rows (y, z) are pruned by their distance to x; the x-run of cells of a row is
-// contiguous in the sorted order, so each row costs one range lookup.
+// contiguous in the sorted order, so each row costs one range lookup.  The lookups of four rows
+// are issued before their items are visited, so four rows cost one round trip to memory (the
+// order of the visits is the same as row by row).
+// The x-run of search-grid row (cy, cz) within reach of x, given the row's remaining squared
+// reach rem >= 0: its first and one-past-last sorted positions.
+__device__ __forceinline__ int2 rowRun(const SearchGrid& S, const int* cellStart, d3 x, int cy, int cz, double rem) {
+  const double hw = sqrt(rem);
+  const int x0 = clampCell(floor((x.x - hw - S.mn.x) * S.invCell), S.size.x);
+  const int x1 = clampCell(floor((x.x + hw - S.mn.x) * S.invCell), S.size.x);
+  const int row = (cz * S.size.y + cy) * S.size.x;
+  return make_int2(cellStart[row + x0], cellStart[row + x1 + 1]);
+}
+// r2 minus the squared distance from x to row cy of a plane at squared distance dz2.
+__device__ __forceinline__ double rowRem(const SearchGrid& S, d3 x, int cy, double r2, double dz2) {
+  const double y0 = cy == 0 ? -1e300 : S.mn.y + cy * S.cell;
+  const double y1 = cy == S.size.y - 1 ? 1e300 : S.mn.y + (cy + 1) * S.cell;
+  const double dy = fmax(0.0, fmax(y0 - x.y, x.y - y1));
+  return r2 - dy * dy - dz2;
+}
+
 template <class F>
 __device__ __forceinline__ void visitNear(const SearchGrid& S, const int* cellStart, d3 x, double reach, F&& f) {
   const i3 c = searchCell(S, x);
@@ -688,19 +743,22 @@ __device__ __forceinline__ void visitNear(const SearchGrid& S, const int* cellSt
     const double z0 = cz == 0 ? -1e300 : S.mn.z + cz * S.cell;
     const double z1 = cz == S.size.z - 1 ? 1e300 : S.mn.z + (cz + 1) * S.cell;
     const double dz = fmax(0.0, fmax(z0 - x.z, x.z - z1));
-    if (dz * dz > r2) continue;
-    for (int cy = max(0, c.y - R); cy <= min(S.size.y - 1, c.y + R); ++cy) {
-      const double y0 = cy == 0 ? -1e300 : S.mn.y + cy * S.cell;
-      const double y1 = cy == S.size.y - 1 ? 1e300 : S.mn.y + (cy + 1) * S.cell;
-      const double dy = fmax(0.0, fmax(y0 - x.y, x.y - y1));
-      const double rem = r2 - dy * dy - dz * dz;
-      if (rem < 0.0) continue;
-      const double hw = sqrt(rem);
-      const int x0 = clampCell(floor((x.x - hw - S.mn.x) * S.invCell), S.size.x);
-      const int x1 = clampCell(floor((x.x + hw - S.mn.x) * S.invCell), S.size.x);
-      const int row = (cz * S.size.y + cy) * S.size.x;
-      const int s = cellStart[row + x0], e = cellStart[row + x1 + 1];
-      for (int k = s; k < e; ++k) f(k);
+    const double dz2 = dz * dz;
+    if (dz2 > r2) continue;
+    const int cyLo = max(0, c.y - R), cyHi = min(S.size.y - 1, c.y + R);
+    // rows in groups of four: a group's range lookups are issued together, then its items are
+    // visited (four named runs picked by selects: an indexed array would go to scratch memory, and
+    // unrolling the visits would copy f four times)
+    for (int g0 = cyLo; g0 <= cyHi; g0 += 4) {
+      auto load = [&](int cy) {
+        const double rem = cy <= cyHi ? rowRem(S, x, cy, r2, dz2) : -1.0;
+        return rem >= 0.0 ? rowRun(S, cellStart, x, cy, cz, rem) : make_int2(0, 0);
+      };
+      const int2 q0 = load(g0), q1 = load(g0 + 1), q2 = load(g0 + 2), q3 = load(g0 + 3);
+      for (int u = 0; u < 4; ++u) {
+        const int2 r = u == 0 ? q0 : u == 1 ? q1 : u == 2 ? q2 : q3;
+        for (int k = r.x; k < r.y; ++k) f(k);
+      }
     }
   }
 }
@@ -787,23 +845,46 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
   if (P.nW > 0) {
     const double rA = P.radius[A];
     const d3 xA = P.posDet[A];
-    for (int w = 0; w < P.nW; ++w) {  // a wall's key is checked each step when the pair is used
-      const ShapeDev S = P.shapes[P.wshape[w]];
-      const double rs = rA + 2.0 * P.halfSkin;  // the wall and the particle each move < half the skin
-      const double rr = rs * (1.0 + 1e-9) + 1e-12;
-      // every point of the grid box lies within boxR of the wall's origin: walls far away (the other
-      // drums of an ensemble) are dropped without turning the centre into the wall's frame
-      const d3 dw = xA - P.wpos[w];
-      const double rb = S.boxR + rr;
-      if (len2(dw) > (rb * rb) * (1.0 + 1e-8)) continue;
-      const d3 loc = rotInv(dw, P.wquat[w]);
-      const d3 o = mk3(S.ox, S.oy, S.oz);
-      if (boxDist2(loc, o, S.h, S) > rr * rr) continue;
-      bool inside;
-      double cmax;
-      const double phi = triAt(P, S, 1.0, loc, o, 1. / S.h, inside, cmax);
-      if (inside && phi > S.lip * rs * (1.0 + 1e-6) + 1e-9 * S.h + 1e-12 * cmax) continue;
-      emit(PT_PW, w);
+    const double rs = rA + 2.0 * P.halfSkin;  // the wall and the particle each move < half the skin
+    const double rr = rs * (1.0 + 1e-9) + 1e-12;
+    // every point of a wall's grid box lies within boxR of the wall's origin: walls far away (the
+    // other drums of an ensemble) are dropped on an FP32 sphere test (its margin covers the FP32
+    // roundings of the centres, of boxR and of the test) before the shape record is read and the
+    // centre turned into the wall's frame.  The searching lanes of the wavefront load the spheres of
+    // as many walls at once, one each, and pass them round: one memory round trip per that many
+    // walls instead of one per wall.
+    const float fx = float(xA.x), fy = float(xA.y), fz = float(xA.z);
+    const float fr = float(rr);
+    const float mA = fmaxf(fmaxf(fabsf(fx), fabsf(fy)), fabsf(fz));
+    const unsigned long long act = __ballot(1);
+    const int nAct = __popcll(act);
+    const int myRank = __popcll(act & laneLtMask());
+    for (int w0 = 0; w0 < P.nW; w0 += nAct) {
+      const int wl = w0 + myRank;
+      const float4 mine = wl < P.nW ? P.wSph[wl] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int cnt = min(nAct, P.nW - w0);
+      unsigned long long rest = act;
+      for (int u = 0; u < cnt; ++u) {  // a wall's key is checked each step when the pair is used
+        const int src = __ffsll((long long)rest) - 1;  // the lane holding wall w0 + u
+        rest &= rest - 1;
+        const float4 sw = make_float4(__shfl(mine.x, src, 64), __shfl(mine.y, src, 64), __shfl(mine.z, src, 64),
+                                      __shfl(mine.w, src, 64));
+        const int w = w0 + u;
+        const float dx = fx - sw.x, dy = fy - sw.y, dz = fz - sw.z;
+        const float ms = fmaxf(fmaxf(fabsf(sw.x), fabsf(sw.y)), fabsf(sw.z));
+        const float reach = (sw.w + fr) * (1.f + 1e-5f) + 1e-6f * (mA + ms) + 1e-30f;
+        if (dx * dx + dy * dy + dz * dz > reach * reach * (1.f + 1e-5f)) continue;
+        const d3 dw = xA - P.wpos[w];
+        const ShapeDev S = P.shapes[P.wshape[w]];
+        const d3 loc = rotInv(dw, P.wquat[w]);
+        const d3 o = mk3(S.ox, S.oy, S.oz);
+        if (boxDist2(loc, o, S.h, S) > rr * rr) continue;
+        bool inside;
+        double cmax;
+        const double phi = triAt(P, S, 1.0, loc, o, 1. / S.h, inside, cmax);
+        if (inside && phi > S.lip * rs * (1.0 + 1e-6) + 1e-9 * S.h + 1e-12 * cmax) continue;
+        emit(PT_PW, w);
+      }
     }
   }
   // periodic ghost images: any node inside the image's grid box is tested, so the reach is
@@ -824,11 +905,11 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
                       (s == 2 && P.periodicX && P.periodicY);
       if (!on) continue;
       const float offx = s != 1 ? lx : 0.f, offy = s != 0 ? ly : 0.f;
-      const int col = imageColumn(P, s);
+      const int col = sector ? imageColumn<true>(s) : imageColumn<false>(s);
       // the image of a source at b (shifted by the period, or turned about the corner's z axis)
       auto image = [&](float4 b) {
         if (!sector) return make_float4(b.x + offx, b.y + offy, b.z, 0.f);
-        const d3 r = G.mn + rodrigues(mk3(b.x, b.y, b.z) - G.mn, P.secW[s], P.secA[s], P.secB[s]);
+        const d3 r = G.mn + secTurn(P, s, mk3(b.x, b.y, b.z) - G.mn);
         return make_float4(float(r.x), float(r.y), float(r.z), 0.f);
       };
       for (int pass = 0; pass < 2; ++pass) {
@@ -837,7 +918,7 @@ __device__ __forceinline__ int pairsOf(const Params& P, int t, int A, long long 
         // sources whose image lies near x_A lie near the inverse image of x_A (a turn the other way)
         const int inv = 2 - s;
         const d3 c = !shifted ? mk3(g.x, g.y, g.z)
-                     : (sector ? G.mn + rodrigues(mk3(g.x, g.y, g.z) - G.mn, P.secW[inv], P.secA[inv], P.secB[inv])
+                     : (sector ? G.mn + secTurn(P, inv, mk3(g.x, g.y, g.z) - G.mn)
                                : mk3(double(g.x) - double(offx), double(g.y) - double(offy), double(g.z)));
         if (searchBoxDist2(P.sg, c) > reach * reach) continue;
         visitNear(P.sg, P.cellStart, c, reach, [&](int k) {
@@ -1067,6 +1148,7 @@ struct PairView {
 };
 
 // (info and A as in pairInfo[p], pairA[p], when the caller already holds them)
+template <bool SEC>
 __device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t info, int A) {
   PairView v;
   v.pair = p;
@@ -1076,7 +1158,7 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t in
   v.A = A;
   const bool ghost = v.type >= PT_GX;
   v.xA = ghost ? P.pos[v.A] : P.posDet[v.A];
-  v.qA = (P.periodicSector && !ghost) ? P.quatDet[v.A] : P.quat[v.A];
+  v.qA = (SEC && !ghost) ? P.quatDet[v.A] : P.quat[v.A];
   v.sA = P.scale[v.A];
   const ShapeDev SA = P.shapes[P.shape[v.A]];
   v.clusterOff = SA.clusterOff;
@@ -1095,20 +1177,17 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t in
     const int k = keys[v.type == PT_PP ? 0 : 1 + int(v.type - PT_GX)];
     if (v.type == PT_PP) {
       v.xB = P.posDet[j]; v.keyB = k;
-      v.qB = P.periodicSector ? P.quatDet[j] : P.quat[j];
+      v.qB = SEC ? P.quatDet[j] : P.quat[j];
     } else {
-      v.xB = P.gpos[int(v.type - PT_GX)][j];
+      v.xB = imgPos<SEC>(P, int(v.type - PT_GX))[j];
       v.keyB = k >= 0 ? (k & ~(1 << kFlipBit)) : -1;
-      v.qB = P.periodicSector ? P.gquat[int(v.type - PT_GX)][j] : P.quat[j];
+      v.qB = SEC ? imgQuat(P, int(v.type - PT_GX))[j] : P.quat[j];
     }
     v.rB = P.radius[j]; v.shapeB = P.shape[j]; v.scB = P.scale[j];
     v.invSpB = P.invSp[j]; v.goB = P.gorig[j];
   }
   v.valid = 1;
   return v;
-}
-__device__ __forceinline__ PairView loadPair(const Params& P, int p) {
-  return loadPair(P, p, P.pairInfo[p], P.pairA[p]);
 }
 
 constexpr int kContactChunk = 1024;  // candidate slots reserved per atomic
@@ -1257,6 +1336,7 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
 // map records of its evaluated pairs (FP64 poses in, one 160-byte record out, in evaluation
 // order).
 constexpr int kCheckTile = 1024;
+template <bool SEC>
 __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   __shared__ int sList[kCheckTile], sA[kCheckTile];
   __shared__ uint32_t sInfo[kCheckTile];
@@ -1319,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       // turned images of sector periodicity are evaluated every step (their motion bound would need
       // the turn)
       if (live && !fresh && (cv[r].x > 0.f || cv[r].w > 0.f) && !(full && signbit(cv[r].z)) &&
-          !(P.periodicSector && tv[r] >= PT_GX)) {
+          !(SEC && tv[r] >= PT_GX)) {
         float4 c = cv[r];
         c.z = fabsf(c.z);
         const bool flip = tv[r] >= PT_GX && ((key[r] >> kFlipBit) & 1);
@@ -1369,7 +1449,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
     __syncthreads();
     for (int k = threadIdx.x; k < sCount; k += 256) {
       const int p = sList[k], a = sBase + k;
-      const PairView v = loadPair(P, p, sInfo[k], sA[k]);
+      const PairView v = loadPair<SEC>(P, p, sInfo[k], sA[k]);
       // the motion state of this evaluation, for the pair's certificate against later motion
       // (loaded before the stores below)
       const BodyStep sa = P.step[v.A];
@@ -1602,6 +1682,7 @@ __device__ __forceinline__ bool nodeSdfF32(const PairF32& f, const float (&c)[8]
 // Exact test and contact law of one candidate slot: the reference's node-vs-SDF membership in
 // FP64, then (for contacts) tangential history and the linear spring-dashpot.
 // (p, node, pInfo, pA: the slot's pair and node, and that pair's pairInfo and pairA)
+template <bool SEC>
 __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInfo, int pA,
                           unsigned long long& statQueries, unsigned long long& statContacts) {
   if (p < 0) {
@@ -1609,7 +1690,7 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
     P.hashSlot[c] = -1;
     return;
   }
-  const PairView v = loadPair(P, p, pInfo, pA);
+  const PairView v = loadPair<SEC>(P, p, pInfo, pA);
   NodeHit h;
   unsigned long long tests = 0;
   const bool hit = exactTest(P, v, node, true, h, tests, statQueries);
@@ -1634,10 +1715,10 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
     mu = fmin(P.mu[A], P.wmu[j]);
     res = fmin(P.res[A], P.wres[j]);
   } else {
-    xf = (type == PT_PP) ? P.pos[j] : P.gpos[type - PT_GX][j];
+    xf = (type == PT_PP) ? P.pos[j] : imgPos<SEC>(P, int(type - PT_GX))[j];
     // an image's velocity (a turned image's is turned), the real body's angular velocity
     // (contactKernel.cu:236-238)
-    vB = (type != PT_PP && P.periodicSector) ? P.gvel[type - PT_GX][j] : P.vel[j];
+    vB = (type != PT_PP && SEC) ? imgVel(P, int(type - PT_GX))[j] : P.vel[j];
     wB = P.angvel[j];
     const double knB = P.kn[j], ksB = P.ks[j], invMB = P.invM[j];
     if (knA > 0. && knB > 0.) kn = knA * knB / (knA + knB);
@@ -1725,7 +1806,7 @@ __device__ __forceinline__ float groupMin(float v) {
 // out by exactly one cull (cluster, sub-cluster or node) or becomes a candidate; the smallest
 // slack of those culls, less the rounding of the map, is the pair's separation certificate
 // (none when a node is a candidate).
-template <bool SUR>
+template <bool SUR, bool SEC>
 __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   extern __shared__ __align__(16) unsigned char candLds[];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
@@ -1993,10 +2074,10 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       for (int c = first + lane; c < first + total; c += 64) {
         if (c - first < kSlotStage) {
           const int2 st = sStage[c - first];
-          exactSlot(P, c, sF[st.x].pair, st.y, sPInfo[st.x], sPA[st.x], statQueries, statContacts);
+          exactSlot<SEC>(P, c, sF[st.x].pair, st.y, sPInfo[st.x], sPA[st.x], statQueries, statContacts);
         } else {
           const int pc = P.cPair[c];
-          exactSlot(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], statQueries, statContacts);
+          exactSlot<SEC>(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], statQueries, statContacts);
         }
       }
     }
@@ -2229,13 +2310,14 @@ __global__ void k_reduce_kick(Params P, int kick) {
 // half-kick, then the next step's first half-kick and drift (one thread per particle, then one per
 // wall).  H describes the next step; P is this step's.  Nothing here reads another body's state,
 // and this step's flags and counters are closed by thread 0 before it clears the next step's.
+template <bool SEC>
 __global__ void k_reduce_drift(Params P, StepHead H) {
   LSD_HALT(P);
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   BodyIn in;
   bool set = false;
   if (b < P.nP) {
-    in = loadBody(P, b);
+    in = loadBody<SEC>(P, b);
     set = P.forceSet[b] != 0;
   }
   if (b == 0) stepEnd(P);
@@ -2246,7 +2328,7 @@ __global__ void k_reduce_drift(Params P, StepHead H) {
   }
   d3 F, T;
   const bool any = reduceBody(P, b, F, T);
-  kickDrift(P, b, in, F, T, 2, H.buf);
+  kickDrift<SEC>(P, b, in, F, T, 2, H.buf);
   // the stored sums of a body without terms are already zero unless written otherwise since
   if (any || set) {
     P.force[b] = F;
@@ -2493,7 +2575,15 @@ __global__ __launch_bounds__(kBroadThreads) void k_broad_phase(Params P) {
   bucketSortBig(P.rcStart, P.rcTmp, P.rcPair, P.sortBig, P.devInt + 2, wave, nWaves, P.nP, nRc);
 }
 
-template __global__ void k_candidates<false>(Params P);
-template __global__ void k_candidates<true>(Params P);
+template __global__ void k_candidates<false, false>(Params P);
+template __global__ void k_candidates<true, false>(Params P);
+template __global__ void k_candidates<false, true>(Params P);
+template __global__ void k_candidates<true, true>(Params P);
+template __global__ void k_kick_drift<false>(Params P, int forceRebuild);
+template __global__ void k_kick_drift<true>(Params P, int forceRebuild);
+template __global__ void k_reduce_drift<false>(Params P, StepHead H);
+template __global__ void k_reduce_drift<true>(Params P, StepHead H);
+template __global__ void k_pair_check<false>(Params P, int full);
+template __global__ void k_pair_check<true>(Params P, int full);
 
 }  // namespace lsd

@@ -1,5 +1,6 @@
 // Learned culling surrogate: a Fourier-feature MLP per shape family, evaluated with BF16 MFMA
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulation) on wave64.
+// (gfx950's v_mfma_f32_16x16x32_bf16 over pairs of 16-wide K blocks, an odd last block paired with
+// zeros; FP32 accumulation) on wave64.
 //
 //   u = p * invLen (p: unscaled local point of shape s)
 //   x[0:96]   sin/cos(w_j u_a), w_j = pi 2^(j/3), j < 16, index 32a + 2j (+1 for cos),
@@ -22,7 +23,33 @@
 namespace lsd {
 
 typedef short sur_v4s __attribute__((ext_vector_type(4)));
+typedef short sur_v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 sur_v8b __attribute__((ext_vector_type(8)));
 typedef float sur_v4f __attribute__((ext_vector_type(4)));
+
+// acc + A B over two 16-wide K blocks in one 16x16x32 instruction.  Lane l passes its 4 values of
+// block 0 followed by its 4 values of block 1, for A and B alike: the instruction's K index 8 (l / 16)
+// + t then names the same K element in both operands, so its 32-term sum is the sum over both
+// blocks (in the instruction's own order).
+__device__ __forceinline__ sur_v4f surMfma2(sur_v4s a0, sur_v4s a1, sur_v4s b0, sur_v4s b1, sur_v4f acc) {
+  const sur_v8s a = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const sur_v8s b = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(sur_v8b, a), __builtin_bit_cast(sur_v8b, b), acc,
+                                                 0, 0, 0);
+}
+// acc + sum over K blocks kb < KB of A(tile(kb)) B(kb): pairs of blocks, an odd last block with a
+// zero block beside it.  Every instruction of the chain is the same 16x16x32 MFMA: each one takes
+// the previous one's result as its accumulator.
+template <int KB, class TileA, class OpB>
+__device__ __forceinline__ sur_v4f surMfmaK(TileA tileA, OpB opB, sur_v4f acc) {
+#pragma unroll
+  for (int kb = 0; kb + 1 < KB; kb += 2) acc = surMfma2(tileA(kb), tileA(kb + 1), opB(kb), opB(kb + 1), acc);
+  if (KB & 1) {
+    const sur_v4s zero = {0, 0, 0, 0};
+    acc = surMfma2(tileA(KB - 1), zero, opB(KB - 1), zero, acc);
+  }
+  return acc;
+}
 
 constexpr int kSurFreq = 16;
 constexpr int kSurEmb = 16;
@@ -100,11 +127,8 @@ __device__ __forceinline__ void surLayer(const SurNet& N, int layer, const sur_v
   const float* b = N.param + kSurOffB[layer];
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
-    sur_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(N.wF[(kSurTileBase[layer] + mb * KB + kb) * 64 + lane], in[kb],
-                                                       acc, 0, 0, 0);
+    const sur_v4f acc = surMfmaK<KB>([&](int kb) { return N.wF[(kSurTileBase[layer] + mb * KB + kb) * 64 + lane]; },
+                                     [&](int kb) { return in[kb]; }, sur_v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int i = 0; i < 4; ++i) z[mb][i] = acc[i] + b[16 * mb + 4 * q + i];
   }
@@ -171,11 +195,9 @@ __device__ __forceinline__ float surForwardInfer(const SurNet& N, float ux, floa
     const float* b = N.param + kSurOffB[0];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      sur_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kb = 0; kb < kSurKB1; ++kb)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(N.wF[(kSurTileBase[0] + mb * kSurKB1 + kb) * 64 + lane],
-                                                         x[kb], acc, 0, 0, 0);
+      const sur_v4f acc =
+          surMfmaK<kSurKB1>([&](int kb) { return N.wF[(kSurTileBase[0] + mb * kSurKB1 + kb) * 64 + lane]; },
+                            [&](int kb) { return x[kb]; }, sur_v4f{0.f, 0.f, 0.f, 0.f});
       h[mb] = surPack(surSilu(acc[0] + b[16 * mb + 4 * q]), surSilu(acc[1] + b[16 * mb + 4 * q + 1]),
                       surSilu(acc[2] + b[16 * mb + 4 * q + 2]), surSilu(acc[3] + b[16 * mb + 4 * q + 3]));
     }
@@ -188,10 +210,8 @@ __device__ __forceinline__ float surForwardInfer(const SurNet& N, float ux, floa
     sur_v4s hn[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      sur_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(N.wF[(base + mb * 4 + kb) * 64 + lane], h[kb], acc, 0, 0, 0);
+      const sur_v4f acc = surMfmaK<4>([&](int kb) { return N.wF[(base + mb * 4 + kb) * 64 + lane]; },
+                                      [&](int kb) { return h[kb]; }, sur_v4f{0.f, 0.f, 0.f, 0.f});
       float a[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = surSilu(acc[i] + b[16 * mb + 4 * q + i]);

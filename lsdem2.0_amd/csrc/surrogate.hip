@@ -156,21 +156,16 @@ __global__ __launch_bounds__(256) void k_sur_train(SurTrainArgs A) {
       // dh_prev^T = W dz^T, then through the previous SiLU
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
-        sur_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb)
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(N.wB[(kSurTileBase[layer] + mb * 4 + kb) * 64 + lane],
-                                                           dzp[kb], acc, 0, 0, 0);
+        const sur_v4f acc =
+            surMfmaK<4>([&](int kb) { return N.wB[(kSurTileBase[layer] + mb * 4 + kb) * 64 + lane]; },
+                        [&](int kb) { return dzp[kb]; }, sur_v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
         for (int i = 0; i < 4; ++i) dz[mb][i] = acc[i] * surDSilu(zs[layer - 1][mb][i]);
       }
     } else {
       // embedding rows 96..111 of layer 1: input block 6
-      sur_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(N.wB[(kSurTileBase[0] + 6 * 4 + kb) * 64 + lane], dzp[kb],
-                                                         acc, 0, 0, 0);
+      const sur_v4f acc = surMfmaK<4>([&](int kb) { return N.wB[(kSurTileBase[0] + 6 * 4 + kb) * 64 + lane]; },
+                                      [&](int kb) { return dzp[kb]; }, sur_v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
       for (int i = 0; i < 4; ++i) sEmb[row][4 * q + i] = acc[i];
     }
@@ -183,22 +178,27 @@ __global__ __launch_bounds__(256) void k_sur_train(SurTrainArgs A) {
     const int layer = t < 28 ? 0 : (t < 44 ? 1 : (t < 60 ? 2 : 3));
     const int local = t - kSurTileBase[layer];
     const int kb = local >> 2, nb = local & 3;
-    sur_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      sur_v4s a, b;
+    // (the K dimension is the block's 64 samples, in 4 blocks of 16)
+    const int k = 16 * kb + m16, n = 16 * nb + m16;
+    auto opA = [&](int p) {
       const int s0 = 16 * p + 4 * q;
-      const int k = 16 * kb + m16, n = 16 * nb + m16;
+      sur_v4s a;
       if (layer == 0) {
         a.x = short(sX[s0][k]); a.y = short(sX[s0 + 1][k]); a.z = short(sX[s0 + 2][k]); a.w = short(sX[s0 + 3][k]);
       } else {
         a.x = short(sH[layer - 1][s0][k]); a.y = short(sH[layer - 1][s0 + 1][k]);
         a.z = short(sH[layer - 1][s0 + 2][k]); a.w = short(sH[layer - 1][s0 + 3][k]);
       }
+      return a;
+    };
+    auto opB = [&](int p) {
+      const int s0 = 16 * p + 4 * q;
+      sur_v4s b;
       b.x = short(sDZ[layer][s0][n]); b.y = short(sDZ[layer][s0 + 1][n]);
       b.z = short(sDZ[layer][s0 + 2][n]); b.w = short(sDZ[layer][s0 + 3][n]);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
+      return b;
+    };
+    const sur_v4f acc = surMfmaK<4>(opA, opB, sur_v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int i = 0; i < 4; ++i) G[kSurOffW[layer] + (16 * kb + 4 * q + i) * kSurHid + 16 * nb + m16] = acc[i];
   }

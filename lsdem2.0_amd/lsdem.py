@@ -127,6 +127,20 @@ class DeviceView(C.Structure):
     ]
 
 
+class WallDeviceView(C.Structure):
+    """lsdem_wall_device_view: W x 3 (W x 4 for the orientation) device arrays of the walls."""
+    _fields_ = [
+        ("position", C.c_void_p),
+        ("quaternion", C.c_void_p),
+        ("velocity", C.c_void_p),
+        ("angular_velocity", C.c_void_p),
+        ("num", C.c_int64),
+        ("grid_dim", C.c_int64),
+        ("block_dim", C.c_int64),
+        ("stream", C.c_void_p),
+    ]
+
+
 # void (*)(void* user, double time, const lsdem_device_view* view)
 EXTERNAL_FORCE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.POINTER(DeviceView))
 
@@ -281,6 +295,42 @@ class Sim:
         """n steps (even) replayed from a HIP graph captured on the first call with that count."""
         self._call("step_graph", self.ctx, C.c_int32(n))
 
+    STEP_PHASES = ("integrate", "hash", "pairs", "narrow", "forces", "reduce")
+
+    def phase(self, name: str):
+        """One phase of a step driven phase by phase (GPU library only): integrate, hash, pairs,
+        narrow, forces, reduce, in that order; the six make one step."""
+        if name not in self.STEP_PHASES:
+            raise ValueError(f"unknown phase {name!r}")
+        self._call("phase_" + name, self.ctx)
+
+    def cell_table(self) -> dict:
+        """Search grid and cell table of the last broad phase (GPU library only)."""
+        org = (C.c_double * 3)()
+        edge = C.c_double()
+        dims = (C.c_int32 * 3)()
+        self._call("get_cell_table", self.ctx, org, C.byref(edge), dims, None, None, C.c_int64(0), None)
+        n = self.n_particles
+        ncell = int(dims[0]) * int(dims[1]) * int(dims[2])
+        cell_of = np.zeros(n, np.int32)
+        order = np.zeros(n, np.int32)
+        start = np.zeros(ncell + 1, np.int32)
+        self._call("get_cell_table", self.ctx, None, None, None, _p(cell_of), _p(order),
+                   C.c_int64(ncell + 1), _p(start))
+        return {"origin": np.array(org[:]), "cell": edge.value, "dims": np.array(dims[:]), "cell_of": cell_of,
+                "sorted": order, "cell_start": start}
+
+    def pairs(self) -> dict:
+        """Candidate pairs of the last broad phase (GPU library only): master, partner, type."""
+        cnt = C.c_int64()
+        self._call("get_pairs", self.ctx, C.c_int64(0), C.byref(cnt), None, None, None)
+        m = cnt.value
+        out = {k: np.zeros(m, np.int32) for k in ("master", "partner", "type")}
+        if m:
+            self._call("get_pairs", self.ctx, C.c_int64(m), C.byref(cnt), _p(out["master"]),
+                       _p(out["partner"]), _p(out["type"]))
+        return out
+
     def set_external_force(self, fn):
         """fn(time: float, view: DeviceView) runs after the contact and bond forces of every step and
         before the second half-kick; None removes it."""
@@ -290,6 +340,11 @@ class Sim:
             return
         self._hook = EXTERNAL_FORCE_FN(lambda user, time, view: fn(time, view.contents))
         self._call("set_external_force", self.ctx, self._hook, None)
+
+    def wall_device_view(self) -> WallDeviceView:
+        v = WallDeviceView()
+        self._call("get_wall_device_view", self.ctx, C.byref(v))
+        return v
 
     def device_view(self) -> DeviceView:
         v = DeviceView()

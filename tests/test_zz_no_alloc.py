@@ -13,6 +13,7 @@ import pytest
 
 sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
 import surrogate_ref as R  # noqa: E402
+from hip_rt import Hip  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -38,33 +39,6 @@ def _samples(rng, n, half):
     return p, shape, t
 
 
-class _Hip:
-    """hipMalloc / hipMemcpy / hipFree of the HIP runtime already loaded with the library."""
-    H2D, D2H = 1, 2
-
-    def __init__(self):
-        self.rt = C.CDLL("libamdhip64.so.7")
-        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.rt.hipFree.argtypes = [C.c_void_p]
-        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-
-    def put(self, a: np.ndarray) -> C.c_void_p:
-        a = np.ascontiguousarray(a)
-        ptr = C.c_void_p()
-        assert self.rt.hipMalloc(C.byref(ptr), max(a.nbytes, 1)) == 0
-        assert self.rt.hipMemcpy(ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, self.H2D) == 0
-        return ptr
-
-    def get(self, ptr: C.c_void_p, like: np.ndarray) -> np.ndarray:
-        out = np.empty_like(like)
-        assert self.rt.hipMemcpy(out.ctypes.data_as(C.c_void_p), ptr, out.nbytes, self.D2H) == 0
-        return out
-
-    def free(self, *ptrs):
-        for q in ptrs:
-            assert self.rt.hipFree(q) == 0
-
-
 def _noalloc(sim, hip, p, shape, t, n_param):
     g0 = np.full(n_param, np.nan, np.float32)
     l0 = np.full(1, np.nan, np.float64)
@@ -78,7 +52,7 @@ def _noalloc(sim, hip, p, shape, t, n_param):
 
 def test_noalloc_gradient_equals_allocating_entry_point(L, S):
     sim, half = _setup(L, S)
-    hip = _Hip()
+    hip = Hip()
     n_param = L.SURROGATE_N_PARAM + 16 * 3
     rng = np.random.default_rng(5)
     for n in (4096, 1280):  # the second call is smaller than the first
@@ -93,7 +67,7 @@ def test_noalloc_gradient_equals_allocating_entry_point(L, S):
 
 def test_noalloc_errors(L, S):
     sim, half = _setup(L, S)
-    hip = _Hip()
+    hip = Hip()
     n_param = L.SURROGATE_N_PARAM + 16 * 3
     p, shape, t = _samples(np.random.default_rng(9), 64 * 400, half)  # beyond the 16 MiB scratch
     rc, _, _ = _noalloc(sim, hip, p, shape, t, n_param)
