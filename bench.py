@@ -54,7 +54,7 @@ B_BODY = (78 + 6) * 4
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch at the 200k workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a
 # plain copy (profiles/c2_200k_pmc_traffic.md); None for a kernel not measured at its current code.
-KERNEL_TRAFFIC_BYTES = {"k_pair_check": 73.73e6, "k_candidates": 9.83e6, "k_reduce_drift": 88.32e6}
+KERNEL_TRAFFIC_BYTES = {"k_pair_check": 73.85e6, "k_candidates": 9.73e6, "k_reduce_drift": 73.42e6}
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 PROFILE_EVERY = 10
 

@@ -355,10 +355,10 @@ __device__ __forceinline__ void stepHead(const Params& P, const StepHead& H, int
 // both), exactly as two separate half-kicks round.
 // Everything the kick and drift of particle i read, loaded before any of its stores (the stores
 // could alias later loads as far as the compiler knows, which would serialise the round trips).
+// (the inverse inertia is read by kickDrift only for a body with a torque)
 struct BodyIn {
   d3 vel, w, pos, xBuild, xBuildW;
   q4 quat, qBuild;
-  s6 invI;
   double invM;
   int imgBuild;
 };
@@ -369,7 +369,6 @@ __device__ __forceinline__ BodyIn loadBody(const Params& P, int i) {
   b.w = P.angvel[i];
   b.quat = P.quat[i];
   b.invM = P.invM[i];
-  b.invI = P.invI[i];
   b.pos = P.pos[i];
   b.xBuild = P.xBuild[i];
   b.xBuildW = P.xBuildW[i];
@@ -384,18 +383,22 @@ __device__ __forceinline__ void kickDrift(const Params& P, int i, const BodyIn& 
   d3 vel = in.vel, w = in.w;
   const q4 q0 = in.quat;
   const double invM = in.invM;
+  // a body without a torque keeps its angular velocity: R I^-1 R^T 0 adds only zeros, so neither its
+  // inverse inertia (48 bytes) is read nor its angular velocity written back
+  const bool torqued = T.x != 0.0 || T.y != 0.0 || T.z != 0.0;
+  const s6 invI = torqued ? P.invI[i] : s6{};
   if (!isZero(invM) && !P.prime) {
     const double h = 0.5 * P.dt;
     const d3 dv = (F * invM + P.g) * h;
-    const d3 dw = (mul(rotInvInertia(q0, in.invI), T)) * h;
     vel += dv;
-    w += dw;
-    if (kicks == 2) {
-      vel += dv;
-      w += dw;
-    }
+    if (kicks == 2) vel += dv;
     P.vel[i] = vel;
-    P.angvel[i] = w;
+    if (torqued) {
+      const d3 dw = (mul(rotInvInertia(q0, invI), T)) * h;
+      w += dw;
+      if (kicks == 2) w += dw;
+      P.angvel[i] = w;
+    }
   }
   d3 p = in.pos;
   q4 qn = q0;
