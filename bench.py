@@ -407,6 +407,8 @@ def main():
                          "captured HIP graphs, one host thread per drum, not run by default: a host-side fault was "
                          "once seen in that mode under a profiler's kernel trace and its cause is not established; "
                          "sequential: one drum after another)")
+    ap.add_argument("--cfg", nargs="*", default=[], metavar="KEY=VALUE",
+                    help="lsdem_config entries of the 200k run (tuning knobs such as narrow_blocks)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="K",
                     help="instead of the main line: K independent rotating drums in one simulation, stepped by "
                          "graph replays (--drum-steps steps), against the same drums one after another")
@@ -450,6 +452,9 @@ def main():
     sc = S.scene_c2(n=args.n, seed=args.seed, gap=args.gap)
     log(f"[bench] scene C2 n={args.n} built in {time.perf_counter() - t:.1f}s")
     caps = {"pair_capacity": 48 * args.n + 65536, "contact_capacity": 48 * args.n + 65536}
+    for kv in args.cfg:  # lsdem_config tuning entries, e.g. narrow_blocks=512
+        k, v = kv.split("=", 1)
+        caps[k] = float(v) if "." in v or "e" in v else int(v)
     if args.gap < 0:
         t = time.perf_counter()
         moved = S.remove_initial_overlaps(lambda c: L.Sim({**c, **caps}), sc, log=log)

@@ -89,6 +89,8 @@ typedef struct lsdem_config {
                              a body that leaves across x = min (y = min) turns by -90 (+90) degrees
                              about that axis; images turned by +90, 180 and -90 degrees fill the lists
                              LSDEM_LIST_GX, _GY and _GXY.  Not combined with periodic_x / periodic_y. */
+  int32_t narrow_blocks;  /* workgroups of the candidate kernel, 256 threads each (0: one per 2 bodies up to
+                             8 per CU); a tuning knob, results do not depend on it */
 } lsdem_config;
 
 typedef struct lsdem_counters {

@@ -1312,6 +1312,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   long long pairCap = c->cfg.pair_capacity > 0 ? c->cfg.pair_capacity : 64ll * nP + 4096 + (periodic ? 32ll * nP : 0);
   // narrow-phase grid: enough waves for ~16 candidate pairs per particle, at most 8 per SIMD
   c->narrowBlocks = int(std::min<long long>(c->numCU * 8ll, std::max<long long>(1, (16ll * nP + 8 * 4 - 1) / (8 * 4))));
+  if (c->cfg.narrow_blocks > 0) c->narrowBlocks = int(c->cfg.narrow_blocks);
   // single-launch broad phase: a grid whose blocks can all be resident together
   c->coopBlocks = 0;
   if (!c->cfg.broad_phase_launches) {

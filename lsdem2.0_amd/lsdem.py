@@ -52,6 +52,7 @@ class Config(C.Structure):
         ("barrier_spin", C.c_int32),
         ("broad_phase_coop", C.c_int32),
         ("periodic_sector", C.c_int32),
+        ("narrow_blocks", C.c_int32),
     ]
 
 
@@ -203,6 +204,7 @@ class Sim:
         c.barrier_spin = int(cfg.get("barrier_spin", 0))
         c.broad_phase_coop = int(cfg.get("broad_phase_coop", 0))
         c.periodic_sector = int(cfg.get("periodic_sector", 0))
+        c.narrow_blocks = int(cfg.get("narrow_blocks", 0))
         c.skin = float(cfg.get("skin", -1.0))
         self.cfg = c
         self.ctx = C.c_void_p()
