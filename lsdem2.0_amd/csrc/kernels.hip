@@ -1110,14 +1110,17 @@ __device__ __forceinline__ int histInsert(const Params& P, unsigned long long ke
   return -1;
 }
 
-// The previous step's spring of a contact key (zero when it had none), from the previous table.
-__device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key) {
-  unsigned long long slot = mix64(key) & P.hashMask;
+// The previous step's spring of a contact key (zero when it had none), from the previous table,
+// given the key k0 and value v0 already read at the key's first slot slot0.
+__device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key, unsigned long long slot0,
+                                         unsigned long long k0, int v0) {
+  if (k0 == key) return P.prevSpring[v0];
+  unsigned long long slot = slot0, k = k0;
   for (unsigned long long probe = 0; probe <= P.hashMask; ++probe) {
-    const unsigned long long k = P.prevHashKey[slot];
     if (k == key) return P.prevSpring[P.prevHashVal[slot]];
     if (k == kHashEmpty) return mk3(0., 0., 0.);
     slot = (slot + 1) & P.hashMask;
+    k = P.prevHashKey[slot];
   }
   return mk3(0., 0., 0.);  // (a full table: the insert that filled it raised kErrHash)
 }
@@ -1203,9 +1206,8 @@ struct NodeHit {
 
 // Exact node-vs-SDF test of global node `ni` (cluster-ordered node array) in the reference's
 // arithmetic: node transform, radius cull or cell window, grid bounds, trilinear overlap >= 0.
-__device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, int ni, bool full, NodeHit& h,
+__device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, const NodeDev& nd, bool full, NodeHit& h,
                                           unsigned long long& tests, unsigned long long& queries) {
-  const NodeDev nd = P.nodes[ni];
   h.gnode = v.nodeBase + nd.orig;
   const d3 xw = rot(v.qA, mk3(nd.x * v.sA, nd.y * v.sA, nd.z * v.sA)) + v.xA;
   tests++;
@@ -1694,46 +1696,64 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
     return;
   }
   const PairView v = loadPair<SEC>(P, p, pInfo, pA);
-  NodeHit h;
-  unsigned long long tests = 0;
-  const bool hit = exactTest(P, v, node, true, h, tests, statQueries);
-  if (!hit) {
-    P.cInfo[c] = kInvalidInfo;
-    P.hashSlot[c] = -1;
-    return;
-  }
-  statContacts++;
+  // everything the contact law and the tangential history need is read here, before the exact
+  // test, so that its round trips overlap the test's: a candidate that is no contact wastes these
+  // reads, a contact saves their latency
+  const NodeDev nd = P.nodes[node];
   const uint32_t type = v.type;
   const int j = v.partner;
   const int A = v.A;
   const d3 riA = P.pos[A];
   const d3 vA = P.vel[A], wA = P.angvel[A];
-  const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A];
+  const double invMA = P.invM[A], knA = P.kn[A], ksA = P.ks[A], muA = P.mu[A], resA = P.res[A];
   d3 xf, vB, wB;
-  double kn = 0., ks = 0., effM = 0., mu, res;
+  double knB, ksB, invMB, muB, resB;
   if (type == PT_PW) {
     xf = P.wpos[j]; vB = P.wvel[j]; wB = P.wangvel[j];
-    kn = knA; ks = ksA;
-    if (invMA > 0.) effM = 1. / invMA;
-    mu = fmin(P.mu[A], P.wmu[j]);
-    res = fmin(P.res[A], P.wres[j]);
+    knB = ksB = invMB = 0.;
+    muB = P.wmu[j];
+    resB = P.wres[j];
   } else {
     xf = (type == PT_PP) ? P.pos[j] : imgPos<SEC>(P, int(type - PT_GX))[j];
     // an image's velocity (a turned image's is turned), the real body's angular velocity
     // (contactKernel.cu:236-238)
     vB = (type != PT_PP && SEC) ? imgVel(P, int(type - PT_GX))[j] : P.vel[j];
     wB = P.angvel[j];
-    const double knB = P.kn[j], ksB = P.ks[j], invMB = P.invM[j];
+    knB = P.kn[j];
+    ksB = P.ks[j];
+    invMB = P.invM[j];
+    muB = P.mu[j];
+    resB = P.res[j];
+  }
+  // tangential history of the same (list, node, partner) from the previous step (its first slot
+  // read now); this step's contact goes into this step's table for the next one
+  const unsigned long long key = histKey(type, v.nodeBase + nd.orig, j);
+  const unsigned long long slot0 = mix64(key) & P.hashMask;
+  const unsigned long long k0 = P.prevHashKey[slot0];
+  const int v0 = P.prevHashVal[slot0];
+  NodeHit h;
+  unsigned long long tests = 0;
+  const bool hit = exactTest(P, v, nd, true, h, tests, statQueries);
+  if (!hit) {
+    P.cInfo[c] = kInvalidInfo;
+    P.hashSlot[c] = -1;
+    return;
+  }
+  statContacts++;
+  double kn = 0., ks = 0., effM = 0., mu, res;
+  if (type == PT_PW) {
+    kn = knA; ks = ksA;
+    if (invMA > 0.) effM = 1. / invMA;
+    mu = fmin(muA, muB);
+    res = fmin(resA, resB);
+  } else {
     if (knA > 0. && knB > 0.) kn = knA * knB / (knA + knB);
     if (ksA > 0. && ksB > 0.) ks = ksA * ksB / (ksA + ksB);
     if (invMA > 0. || invMB > 0.) effM = 1. / (invMA + invMB);
-    res = fmin(P.res[A], P.res[j]);
-    mu = fmin(P.mu[A], P.mu[j]);
+    res = fmin(resA, resB);
+    mu = fmin(muA, muB);
   }
-  // tangential history of the same (list, node, partner) from the previous step; this step's
-  // contact goes into this step's table for the next one
-  const unsigned long long key = histKey(type, h.gnode, j);
-  d3 spring = histLookup(P, key);
+  d3 spring = histLookup(P, key, slot0, k0, v0);
   // (the slot's stores after every load above)
   P.cInfo[c] = v.info;
   P.cNode[c] = h.gnode;
