@@ -10,6 +10,9 @@ resampled until no two particles overlap).  W warm-up steps, then K timed steps;
 brackets the timed steps (lsdem_step returns after the stream is drained).  Per-phase GPU times
 come from HIP events recorded on the library's stream inside the timed region, on every 10th step
 (each event record costs GPU time of its own; timing every step would add ~13 us to each).
+`graph_replay`: the same workload in a second simulation from the same start, W eager warm-up steps,
+then the timed steps replayed from a captured 100-step HIP graph (one launch per 100 steps, no
+events inside; the capture's own replay untimed): a second throughput figure beside `value`.
 
 Surrogate path: a second context from the same initial state trains the culling network on the
 GPU (not timed in the step rate; reported), then runs W + K steps with the network culling node
@@ -407,6 +410,8 @@ def main():
                          "captured HIP graphs, one host thread per drum, not run by default: a host-side fault was "
                          "once seen in that mode under a profiler's kernel trace and its cause is not established; "
                          "sequential: one drum after another)")
+    ap.add_argument("--no-graph-replay", dest="graph_replay", action="store_false",
+                    help="skip the second run of the workload replayed from captured 100-step graphs")
     ap.add_argument("--cfg", nargs="*", default=[], metavar="KEY=VALUE",
                     help="lsdem_config entries of the 200k run (tuning knobs such as narrow_blocks)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="K",
@@ -480,6 +485,25 @@ def main():
              if k in ("pos", "quat", "vel", "angvel", "force", "torque")}
     sim.close()
     del sim
+    # the same workload in a second simulation from the same start: W eager warm-up steps, then the K
+    # steps replayed from a captured 100-step HIP graph (one launch per 100 steps, no phase events
+    # inside): a second throughput figure beside `value`
+    graph = None
+    if args.graph_replay and args.steps >= 200:
+        gsim = sc.load(L.Sim({**sc.cfg, **caps}))
+        gsim.step(args.warmup)
+        chunk = 100
+        n_g = max(chunk, (args.steps // chunk) * chunk)
+        gsim.step_graph(chunk)  # capture (its first replay is one of the n_g steps)
+        t = time.perf_counter()
+        for _ in range(n_g // chunk - 1):
+            gsim.step_graph(chunk)
+        gs = time.perf_counter() - t
+        timed = n_g - chunk
+        graph = {"value": args.n * timed / gs, "ms_per_step": 1e3 * gs / timed, "steps": timed, "chunk": chunk}
+        log(f"[bench] graph replays: {timed} steps, {graph['ms_per_step']:.4f} ms/step")
+        gsim.close()
+        del gsim
 
     sur = None
     if not args.no_surrogate:
@@ -538,6 +562,7 @@ def main():
         "work_per_step": {k: prof[k] / K for k in ("pairs", "evaluated_pairs", "master_nodes", "cluster_tests",
                                                     "node_tests", "sdf_queries", "contacts", "rebuilds")},
         "node_sdf_queries_per_s": prof["sdf_queries"] / K / (ms_step * 1e-3),
+        "graph_replay": graph,
     }
     print(json.dumps(out))
 

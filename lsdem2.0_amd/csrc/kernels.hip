@@ -1206,8 +1206,10 @@ struct NodeHit {
 
 // Exact node-vs-SDF test of global node `ni` (cluster-ordered node array) in the reference's
 // arithmetic: node transform, radius cull or cell window, grid bounds, trilinear overlap >= 0.
-__device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, const NodeDev& nd, bool full, NodeHit& h,
-                                          unsigned long long& tests, unsigned long long& queries) {
+// (nB and sdfOff: the partner's grid size and its offset in the SDF pool)
+__device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, const NodeDev& nd, i3 nB,
+                                          long long sdfOff, bool full, NodeHit& h, unsigned long long& tests,
+                                          unsigned long long& queries) {
   h.gnode = v.nodeBase + nd.orig;
   const d3 xw = rot(v.qA, mk3(nd.x * v.sA, nd.y * v.sA, nd.z * v.sA)) + v.xA;
   tests++;
@@ -1216,8 +1218,6 @@ __device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, co
   } else {
     if (!keyInWindow(v.type == PT_PW ? P.gridW : P.gridP, v.keyB, xw)) return false;
   }
-  const ShapeDev SB = P.shapes[v.shapeB];
-  const i3 nB{SB.nx, SB.ny, SB.nz};
   const d3 loc = rotInv(xw - v.xB, v.qB);
   const double gx = (loc.x - v.goB.x) * v.invSpB, gy = (loc.y - v.goB.y) * v.invSpB,
                gz = (loc.z - v.goB.z) * v.invSpB;
@@ -1225,7 +1225,7 @@ __device__ __forceinline__ bool exactTest(const Params& P, const PairView& v, co
   if (i0 < 0 || j0 < 0 || k0 < 0 || i0 >= nB.x - 1 || j0 >= nB.y - 1 || k0 >= nB.z - 1) return false;
   const int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
   const double x = gx - double(i0), y = gy - double(j0), z = gz - double(k0);
-  const float* g = P.sdf + SB.sdfOff;
+  const float* g = P.sdf + sdfOff;
   double cc[8];
   cc[0] = double(g[lin3(i3{i0, j0, k0}, nB)]) * v.scB;
   cc[1] = double(g[lin3(i3{i1, j0, k0}, nB)]) * v.scB;
@@ -1687,8 +1687,10 @@ __device__ __forceinline__ bool nodeSdfF32(const PairF32& f, const float (&c)[8]
 // Exact test and contact law of one candidate slot: the reference's node-vs-SDF membership in
 // FP64, then (for contacts) tangential history and the linear spring-dashpot.
 // (p, node, pInfo, pA: the slot's pair and node, and that pair's pairInfo and pairA)
+// fl: the pair's map record when the caller holds it (its copy of the partner's grid size and SDF
+// offset spares the read of the partner's shape record), else null
 template <bool SEC>
-__device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInfo, int pA,
+__device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInfo, int pA, const PairF32* fl,
                           unsigned long long& statQueries, unsigned long long& statContacts) {
   if (p < 0) {
     P.cInfo[c] = kInvalidInfo;
@@ -1733,7 +1735,17 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
   const int v0 = P.prevHashVal[slot0];
   NodeHit h;
   unsigned long long tests = 0;
-  const bool hit = exactTest(P, v, nd, true, h, tests, statQueries);
+  i3 nB;
+  long long sdfOff;
+  if (fl) {
+    nB = i3{fl->nx, fl->ny, fl->nz};
+    sdfOff = fl->sdfOff;
+  } else {
+    const ShapeDev SB = P.shapes[v.shapeB];
+    nB = i3{SB.nx, SB.ny, SB.nz};
+    sdfOff = SB.sdfOff;
+  }
+  const bool hit = exactTest(P, v, nd, nB, sdfOff, true, h, tests, statQueries);
   if (!hit) {
     P.cInfo[c] = kInvalidInfo;
     P.hashSlot[c] = -1;
@@ -2097,10 +2109,10 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       for (int c = first + lane; c < first + total; c += 64) {
         if (c - first < kSlotStage) {
           const int2 st = sStage[c - first];
-          exactSlot<SEC>(P, c, sF[st.x].pair, st.y, sPInfo[st.x], sPA[st.x], statQueries, statContacts);
+          exactSlot<SEC>(P, c, sF[st.x].pair, st.y, sPInfo[st.x], sPA[st.x], &sF[st.x], statQueries, statContacts);
         } else {
           const int pc = P.cPair[c];
-          exactSlot<SEC>(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], statQueries, statContacts);
+          exactSlot<SEC>(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], nullptr, statQueries, statContacts);
         }
       }
     }
