@@ -299,7 +299,9 @@ def ensemble(args):
     is lognormal (median 4,000, within 500-20,000), scaled so that 20,000 fills half of the drum's
     lattice sites (a 0.1 m x 0.05 m drum holds a few hundred 3-5 mm particles); superellipsoids with
     exponent U[2,6] and semi-axes 3-5 mm, SDF 32^3, 300 nodes; 1-30 rpm; mu U[0.1,0.8], e U[0.3,0.9];
-    dt 1e-5.  Against it: the same drums as separate simulations stepped one after another."""
+    dt 1e-5.  The autotuner first picks block size, narrow batch and skin (hence the search-cell
+    edge) for the ensemble's simulation by timing each setting on it (--ensemble-no-tune: defaults).
+    Against it: the same drums as separate simulations with the defaults, one after another."""
     import __graft_entry__ as g
     g.load_package()
     import lsdem_amd.lsdem as L
@@ -318,7 +320,8 @@ def ensemble(args):
         k, v = kv.split("=", 1)
         cfg[k] = float(v) if "." in v or "e" in v else int(v)
     tuned = {}
-    if args.autotune:
+    if not args.ensemble_no_tune:
+        # the configuration's autotuner: one setting for the ensemble's simulation, measured on it
         import lsdem_amd.autotune as A
         t = time.perf_counter()
         base = {k: v for k, v in cfg.items() if k not in sc.cfg}
@@ -421,6 +424,8 @@ def main():
                     help="with --ensemble: steps per captured graph (0: eager steps only)")
     ap.add_argument("--ensemble-cfg", nargs="*", default=[], metavar="KEY=VALUE",
                     help="with --ensemble: lsdem_config entries of the ensemble's simulation")
+    ap.add_argument("--ensemble-no-tune", action="store_true",
+                    help="with --ensemble: run with the default settings instead of the tuned ones")
     ap.add_argument("--ensemble-seq-steps", type=int, default=500,
                     help="with --ensemble: steps per drum of the sequential comparison (0: skip it)")
     ap.add_argument("--autotune", action="store_true",
