@@ -328,7 +328,7 @@ struct lsdem_ctx {
   DBuf<int> dBI, dBJ, dBActive, dBcStart, dBcItem;
   DBuf<double> dBConst, dBOut;
   DBuf<d3> dBLocal, dBPoint, dBF, dBMi, dBMj;
-  DBuf<int> dDevInt, dScanBlocks;
+  DBuf<int> dDevInt, dScanBlocks, dBarrier;
   DBuf<unsigned long long> dDevStat;
   long long hashSize = 0;
   int scanBlocksCap = 0;
@@ -394,7 +394,7 @@ void lsdem_ctx::releaseAll() {
   dBI.release(); dBJ.release(); dBActive.release(); dBcStart.release(); dBcItem.release();
   dBConst.release(); dBOut.release(); dBLocal.release(); dBPoint.release(); dBF.release(); dBMi.release();
   dBMj.release();
-  dDevInt.release(); dScanBlocks.release(); dDevStat.release();
+  dBarrier.release(); dDevInt.release(); dScanBlocks.release(); dDevStat.release();
   dNodeCluster.release();
   dXBuild.release(); dXBuildW.release(); dWBuildPos.release(); dWBuildQ.release();
   dQBuild.release(); dImgBuild.release(); dStep.release(); dPairRec.release();
@@ -927,7 +927,7 @@ void bindPointers(lsdem_ctx* c) {
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
   P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
   P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
-  P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
+  P.barrier = c->dBarrier.p; P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
   setCurrentBuffers(c);
 }
 
@@ -1394,6 +1394,8 @@ int lsdem_initialize(lsdem_ctx* c) {
   c->scanBlocksCap = int((maxScan + 2047) / 2048) + 1;
   HIPCHK(c->dScanBlocks.alloc(c->scanBlocksCap));
   HIPCHK(c->dDevInt.alloc(kDevInts));
+  HIPCHK(c->dBarrier.alloc(kBarrierInts));
+  HIPCHK(hipMemsetAsync(c->dBarrier.p, 0, kBarrierInts * sizeof(int), s));
   HIPCHK(c->dDevStat.alloc(kDevStats));
   HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 5 * kStatSlots * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, kDevInts * sizeof(int), s));
@@ -1506,6 +1508,7 @@ int runCall(lsdem_ctx* c, Work&& work) {
       // leave the context able to step again (from whatever state the halted call left)
       HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
       HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
+    HIPCHK(hipMemset(c->dBarrier.p, 0, kBarrierInts * sizeof(int)));
       c->forceRebuild = true;
       return err;
     }
@@ -1590,6 +1593,7 @@ int lsdem_step_graph(lsdem_ctx* c, int32_t n) {
     const int err = reportFlags(c, flags, step);
     HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
     HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
+    HIPCHK(hipMemset(c->dBarrier.p, 0, kBarrierInts * sizeof(int)));
     c->forceRebuild = true;
     return err;
   }
@@ -1628,6 +1632,7 @@ int phaseDone(lsdem_ctx* c, int next) {
     const int err = reportFlags(c, flags, step);
     HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
     HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
+    HIPCHK(hipMemset(c->dBarrier.p, 0, kBarrierInts * sizeof(int)));
     c->forceRebuild = true;
     c->phaseNext = 0;
     return err;

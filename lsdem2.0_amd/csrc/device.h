@@ -237,6 +237,8 @@ struct Params {
   //         last step that filled history table b, [12] grid-barrier arrivals of the broad phase,
   //         [20] callStep of the step that raised the first error flag (-1: none)
   int* devInt;
+  int* barrier;          // grid barrier of the single-launch broad phase: kBarrierLine-int lines, one per
+                         // block group (blockIdx mod 8), then the groups' counter, then the release word
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
   //          (summed while profiling); [9] audits, [10] audited contacts, [11] missed contacts
@@ -300,6 +302,8 @@ struct SnapDev {
 
 constexpr int kStatSlots = 32;
 constexpr int kDevInts = 32;   // devInt entries
+constexpr int kBarrierLine = 32;                  // ints per barrier word (one 128-byte line each)
+constexpr int kBarrierInts = 10 * kBarrierLine;   // 8 groups, the groups' counter, the release word
 // The step's rebuild flag and its count of ghost images left at their source's position, two sets
 // used in alternate steps (buffer = Params::curBuf): the kernel that closes one step may already
 // raise the next step's.

@@ -340,7 +340,6 @@ __device__ __forceinline__ void stepHead(const Params& P, const StepHead& H, int
     if (H.forceRebuild) atomicOr(&P.devInt[rebuildFlagIdx(H.buf)], 1);
     P.devInt[1] = 0;   // contact slots allocated
     P.devInt[9] = 0;   // pairs the narrow phase evaluates
-    P.devInt[12] = 0;  // arrivals at the broad phase's grid barriers
   }
   const int nClear = P.devInt[10 + H.buf];
   for (int c = i; c < nClear; c += nth) {
@@ -1348,6 +1347,8 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   PairF32* out = reinterpret_cast<PairF32*>(P.pairMap);
   __shared__ int sCount, sBase;
   LSD_HALT(P);  // (a failed broad phase leaves no pair list to read)
+  // the broad phase of this step is over: its grid barrier's words start the next one at zero
+  if (blockIdx.x == 0 && threadIdx.x < 10) P.barrier[threadIdx.x * kBarrierLine] = 0;
   const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
   const bool rebuilt = P.devInt[rebuildFlagIdx(P.curBuf)] != 0;  // a new pair list this step
   const bool fresh = !P.useCert;  // (a new list holds the certificates it inherited)
@@ -2417,10 +2418,11 @@ __global__ void k_snap_restore(Params P, SnapDev S) {
     S.prevNode[c] = S.hNode[c]; S.prevInfo[c] = S.hInfo[c]; S.prevSpring[c] = S.hSpring[c];
   }
   if (g.tid < kDevStats) P.devStat[g.tid] = S.devStat[g.tid];
+  if (g.tid < 10) P.barrier[g.tid * kBarrierLine] = 0;
   if (g.tid == 0) {
     P.devInt[10 + S.prevBuf] = n;  // the restored slots, cleared from their table two steps on
     P.devInt[11 - S.prevBuf] = 0;  // the other table is empty
-    P.devInt[1] = 0; P.devInt[9] = 0; P.devInt[12] = 0;
+    P.devInt[1] = 0; P.devInt[9] = 0;
     P.devInt[rebuildFlagIdx(0)] = 0; P.devInt[rebuildFlagIdx(1)] = 0;
     P.devInt[ghostCountIdx(0)] = 0; P.devInt[ghostCountIdx(1)] = 0;
     P.devInt[3] = 0;
@@ -2511,12 +2513,17 @@ __device__ void stepEnd(const Params& P) {
 // cost of a barrier grows with the number of blocks) while many threads work between them.
 constexpr int kBroadThreads = 1024;
 
-// Grid barrier on the counter devInt[12] (cleared by the drift kernel every step): each block
-// adds one arrival per barrier and waits for nBlocks x (barriers passed).  The wait is bounded
-// (P.barrierSpin polls, about a second by default): a block that gives up raises kErrBarrier.  A
-// block also stops waiting as soon as any error flag is up, and after every barrier all blocks
-// read the flags: sync() returns true when the launch must be abandoned, so every block leaves
-// at the same barrier and no phase runs on the incomplete results of an earlier one.
+// Grid barrier in two levels (its words are zeroed by the pair check, the launch that follows every
+// broad phase, and by the host after a failed call): a block arrives on
+// its group's counter (group = blockIdx mod 8, the XCD the block runs on when the dispatcher deals
+// blocks round-robin over the XCDs); the last arrival of a group arrives on the groups' counter, and
+// the last of those writes the release word, which the blocks poll.  A barrier then serialises about
+// nBlocks / 8 + 8 atomics on any one word instead of nBlocks, and the polls read a word no arrival
+// writes.  The wait is bounded (P.barrierSpin polls, about a second by default): a block that gives
+// up raises kErrBarrier.  A block also stops waiting as soon as any error flag is up, and after
+// every barrier all blocks read the flags: sync() returns true when the launch must be abandoned,
+// so every block leaves at the same barrier and no phase runs on the incomplete results of an
+// earlier one.
 struct GridBarrier {
   const Params& P;
   int nBlocks, passed;
@@ -2525,20 +2532,30 @@ struct GridBarrier {
     __syncthreads();
     if (threadIdx.x == 0) {
       ++passed;
-      const int target = passed * nBlocks;
-      int* count = P.devInt + 12;
+      const int g = int(blockIdx.x) & 7;
+      const int nGroups = min(nBlocks, 8);
+      const int inGroup = (nBlocks - g + 7) >> 3;  // blocks b < nBlocks with b mod 8 == g
+      int* groupCount = P.barrier + g * kBarrierLine;
+      int* groupsDone = P.barrier + 8 * kBarrierLine;
+      int* release = P.barrier + 9 * kBarrierLine;
       __threadfence();  // this block's writes, before the arrival
-      atomicAdd(count, 1);
+      if (atomicAdd(groupCount, 1) == passed * inGroup - 1) {
+        __threadfence();  // (the group's arrivals, before the group's)
+        if (atomicAdd(groupsDone, 1) == passed * nGroups - 1) {
+          __threadfence();
+          __hip_atomic_store(release, passed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
       // relaxed polling (an acquire per poll would invalidate the caches every time), then one
       // acquire fence
       for (unsigned int it = 0;
-           __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
+           __hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < passed; ++it) {
         if (__hip_atomic_load(P.devInt + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (it >= unsigned(P.barrierSpin)) {
           raiseError(P, kErrBarrier);
           break;
         }
-        __builtin_amdgcn_s_sleep(20);  // ~1300 cycles: few polls of the shared counter line
+        __builtin_amdgcn_s_sleep(10);  // ~640 cycles between polls of the release word
       }
       __threadfence();  // the other blocks' writes, after the wait
       abandon = __hip_atomic_load(P.devInt + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
