@@ -927,7 +927,8 @@ void bindPointers(lsdem_ctx* c) {
   P.bI = c->dBI.p; P.bJ = c->dBJ.p; P.bActive = c->dBActive.p; P.bConst = c->dBConst.p; P.bLocal = c->dBLocal.p;
   P.bPoint = c->dBPoint.p; P.bOut = c->dBOut.p; P.bF = c->dBF.p; P.bMi = c->dBMi.p; P.bMj = c->dBMj.p;
   P.bcStart = c->dBcStart.p; P.bcItem = c->dBcItem.p;
-  P.barrier = c->dBarrier.p; P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
+  P.barrier = c->dBarrier.p; P.evalCount = c->dBarrier.p + 10 * kBarrierLine;
+  P.devInt = c->dDevInt.p; P.devStat = c->dDevStat.p;
   setCurrentBuffers(c);
 }
 

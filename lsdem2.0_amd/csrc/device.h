@@ -183,7 +183,7 @@ struct Params {
   float4* certE;    // pairCap: at the last evaluation, dA - dB (x, y, z) and the image's flip state (w)
   float4* certQA;   // pairCap: at the last evaluation, the master's qrel
   float4* certQB;   // pairCap: at the last evaluation, the partner's qrel
-  int* pairRec;     // pair -> position of its map record this step (devInt[9] records)
+  int* pairRec;     // pair -> position of its map record in pairMap this step
   int useCert;
   // reaction CSR (pairs whose partner is a particle that receives the reaction)
   int* rcCount;     // nP
@@ -233,12 +233,15 @@ struct Params {
   //         kernel of the call),
   //         [4] peak pairs, [5] peak candidate slots, [6] slots scanned by the history insert,
   //         [7] images at their source's own position, [8] rebuild flag of the step,
-  //         [9] pairs the narrow phase evaluates this step, [10 + b] contact slots of the
+  //         [9] (unused), [10 + b] contact slots of the
   //         last step that filled history table b, [12] grid-barrier arrivals of the broad phase,
   //         [20] callStep of the step that raised the first error flag (-1: none)
   int* devInt;
   int* barrier;          // grid barrier of the single-launch broad phase: kBarrierLine-int lines, one per
-                         // block group (blockIdx mod 8), then the groups' counter, then the release word
+                         // block group (blockIdx mod 8), then the groups' counter, then the release word;
+                         // then 8 lines of evalCount
+  int* evalCount;        // word g * kBarrierLine: map records the pair check's block group g (blockIdx
+                         // mod 8) wrote this step, in its own region of pairMap
   // devStat: [0] node tests, [1] sdf queries, [2] contacts (this step);
   //          [3] pairs, [4] node tests, [5] sdf queries, [6] contacts, [8] cluster entries
   //          (summed while profiling); [9] audits, [10] audited contacts, [11] missed contacts
@@ -303,7 +306,7 @@ struct SnapDev {
 constexpr int kStatSlots = 32;
 constexpr int kDevInts = 32;   // devInt entries
 constexpr int kBarrierLine = 32;                  // ints per barrier word (one 128-byte line each)
-constexpr int kBarrierInts = 10 * kBarrierLine;   // 8 groups, the groups' counter, the release word
+constexpr int kBarrierInts = 18 * kBarrierLine;   // 8 groups, the groups' counter, the release word, evalCount
 // The step's rebuild flag and its count of ghost images left at their source's position, two sets
 // used in alternate steps (buffer = Params::curBuf): the kernel that closes one step may already
 // raise the next step's.

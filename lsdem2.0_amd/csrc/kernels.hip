@@ -339,8 +339,8 @@ __device__ __forceinline__ void stepHead(const Params& P, const StepHead& H, int
   if (i == 0) {
     if (H.forceRebuild) atomicOr(&P.devInt[rebuildFlagIdx(H.buf)], 1);
     P.devInt[1] = 0;   // contact slots allocated
-    P.devInt[9] = 0;   // pairs the narrow phase evaluates
   }
+  if (i < 8) P.evalCount[i * kBarrierLine] = 0;  // map records of the pair check's block groups
   const int nClear = P.devInt[10 + H.buf];
   for (int c = i; c < nClear; c += nth) {
     const int slot = H.hashSlot[c];
@@ -1338,8 +1338,45 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
 // surrogate) also evaluates every live pair whose certificate rests on the surrogate's slack.
 // A block compacts a tile of 1024 pairs in LDS, appends it with one atomic, then writes the FP32
 // map records of its evaluated pairs (FP64 poses in, one 160-byte record out, in evaluation
-// order).
+// order).  A block group g (blockIdx mod 8) checks one contiguous eighth of the tiles and writes the
+// records of its evaluated pairs from the first pair of its first tile on, with its own counter
+// (evalCount[g]): 8 counters on 8 lines instead of one, since returning atomics on one address are
+// served one after another (~11 ns each) and every tile takes one.
 constexpr int kCheckTile = 1024;
+
+// The evaluated pairs' map records, numbered densely group after group (record a of the narrow
+// phase) and found at their place in the groups' regions of pairMap.
+struct EvalRecords {
+  int n;  // records this step
+  int pre[8], base[8];
+  // position in pairMap of record a (0 <= a < n): the last group whose first record is at or below a
+  // (a group without records has the same first record as the next one)
+  __device__ __forceinline__ int at(int a) const {
+    int pos = a;
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+      if (a >= pre[g]) pos = base[g] + (a - pre[g]);
+    return pos;
+  }
+};
+__device__ __forceinline__ int checkTiles(const Params& P) {
+  const int nPairs = min(P.pairStart[P.nP], (int)P.pairCap);
+  return (nPairs + kCheckTile - 1) / kCheckTile;
+}
+__device__ __forceinline__ int groupFirstTile(int nTiles, int g) { return int((long long)nTiles * g / 8); }
+__device__ __forceinline__ EvalRecords evalRecords(const Params& P) {
+  EvalRecords e;
+  const int nTiles = checkTiles(P);
+  int sum = 0;
+#pragma unroll
+  for (int g = 0; g < 8; ++g) {
+    e.pre[g] = sum;
+    e.base[g] = groupFirstTile(nTiles, g) * kCheckTile;
+    sum += P.evalCount[g * kBarrierLine];
+  }
+  e.n = sum;
+  return e;
+}
 template <bool SEC>
 __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   __shared__ int sList[kCheckTile], sA[kCheckTile];
@@ -1358,7 +1395,8 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
   // contiguous eighth of the pair list and its partners' data stays in that XCD's L2
   const int nTiles = (nPairs + kCheckTile - 1) / kCheckTile;
   const int xcd = int(blockIdx.x) & 7, perX = (int(gridDim.x) - xcd + 7) >> 3;  // blocks on this XCD
-  const int tLo = int((long long)nTiles * xcd / 8), tHi = int((long long)nTiles * (xcd + 1) / 8);
+  const int tLo = groupFirstTile(nTiles, xcd), tHi = groupFirstTile(nTiles, xcd + 1);
+  const int region = tLo * kCheckTile;  // this group's records: at most its pairs, from its first pair on
   unsigned int evNodes = 0;  // boundary nodes of the masters of this thread's evaluated pairs
   for (int t = tLo + (int(blockIdx.x) >> 3); t < tHi; t += perX) {
     const int t0 = t * kCheckTile;
@@ -1451,7 +1489,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) sBase = sCount ? atomicAdd(&P.devInt[9], sCount) : 0;
+    if (threadIdx.x == 0) sBase = sCount ? region + atomicAdd(&P.evalCount[xcd * kBarrierLine], sCount) : 0;
     __syncthreads();
     for (int k = threadIdx.x; k < sCount; k += 256) {
       const int p = sList[k], a = sBase + k;
@@ -1865,9 +1903,11 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const float kInf = __builtin_huge_valf();
   LSD_HALT(P);
   unsigned long long statCand = 0, statEnt = 0, statQueries = 0, statContacts = 0;
-  const int nAct = P.devInt[9];
+  const EvalRecords rec = evalRecords(P);
+  const int nAct = rec.n;
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
+  if (wave == 0 && lane == 0) P.devStat[13] += (unsigned long long)nAct;  // evaluated pairs, summed over steps
   const int iters = (maxC + 7) >> 3;
   const int j = lane >> 3, k = lane & 7;
   const unsigned long long ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1885,7 +1925,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     const bool own = j < nb && a < nAct;
     // the 8 lanes of a pair copy its 160-byte map record into LDS, 20 bytes each
     if (own) {
-      const unsigned int* src = reinterpret_cast<const unsigned int*>(P.pairMap) + (long long)a * 40 + 5 * k;
+      const unsigned int* src = reinterpret_cast<const unsigned int*>(P.pairMap) + (long long)rec.at(a) * 40 + 5 * k;
       unsigned int* dst = reinterpret_cast<unsigned int*>(&sF[j]) + 5 * k;
 #pragma unroll
       for (int w = 0; w < 5; ++w) dst[w] = src[w];
@@ -2140,7 +2180,8 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
   LSD_HALT(P);
   __shared__ int sQ[kNarrowBlock / 64][64];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
-  const int nAct = P.devInt[9];
+  const EvalRecords rec = evalRecords(P);
+  const int nAct = rec.n;
   const int wave = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (kNarrowBlock / 64) + wl));
   const int nWaves = int(gridDim.x * (kNarrowBlock / 64));
   const int iters = (P.maxClusters + 7) >> 3;
@@ -2149,7 +2190,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_sur_masks(Params P) {
     const int a = base + j;  // map record
     const bool own = a < nAct;
     PairF32 f;
-    if (own) f = reinterpret_cast<const PairF32*>(P.pairMap)[a];
+    if (own) f = reinterpret_cast<const PairF32*>(P.pairMap)[rec.at(a)];
     unsigned long long mask = ~0ull;
     float smin = __builtin_huge_valf();  // smallest slack of the clusters the network culls
     for (int it = 0; it < iters; ++it) {
@@ -2418,11 +2459,11 @@ __global__ void k_snap_restore(Params P, SnapDev S) {
     S.prevNode[c] = S.hNode[c]; S.prevInfo[c] = S.hInfo[c]; S.prevSpring[c] = S.hSpring[c];
   }
   if (g.tid < kDevStats) P.devStat[g.tid] = S.devStat[g.tid];
-  if (g.tid < 10) P.barrier[g.tid * kBarrierLine] = 0;
+  if (g.tid < 18) P.barrier[g.tid * kBarrierLine] = 0;  // (the grid barrier's words and evalCount)
   if (g.tid == 0) {
     P.devInt[10 + S.prevBuf] = n;  // the restored slots, cleared from their table two steps on
     P.devInt[11 - S.prevBuf] = 0;  // the other table is empty
-    P.devInt[1] = 0; P.devInt[9] = 0;
+    P.devInt[1] = 0;
     P.devInt[rebuildFlagIdx(0)] = 0; P.devInt[rebuildFlagIdx(1)] = 0;
     P.devInt[ghostCountIdx(0)] = 0; P.devInt[ghostCountIdx(1)] = 0;
     P.devInt[3] = 0;
@@ -2484,7 +2525,6 @@ __device__ void stepEnd(const Params& P) {
       mn += P.statSlots[4 * kStatSlots + k];
     }
     P.devStat[12] += mn;
-    P.devStat[13] += (unsigned long long)P.devInt[9];
     P.devStat[14] += P.devInt[rebuildFlagIdx(P.curBuf)] != 0 ? 1ull : 0ull;
     P.devStat[0] = t;
     P.devStat[1] = q;
@@ -2499,8 +2539,8 @@ __device__ void stepEnd(const Params& P) {
   P.devInt[rebuildFlagIdx(P.curBuf)] = 0;  // rebuild flag (the drift kernel raises it)
   P.devInt[ghostCountIdx(P.curBuf)] = 0;  // valid ghost images at their source's own position
   for (int k = 0; k < 5 * kStatSlots; ++k) P.statSlots[k] = 0;
-  // (the contact slot count [1] and the evaluated-pair count [9] stay readable after the step;
-  // the drift kernel clears them)
+  // (the contact slot count [1] and the pair check's evalCount stay readable after the step; the
+  // drift kernel clears them)
 }
 
 
