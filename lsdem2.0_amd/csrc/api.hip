@@ -1363,7 +1363,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dRcStart.alloc(nP + 1));
   if ((rc = allocPairs(c, pairCap))) return rc;
   if ((rc = allocContacts(c, contactCap))) return rc;
-  HIPCHK(c->dStatSlots.alloc(5 * kStatSlots));
+  HIPCHK(c->dStatSlots.alloc(kStatWords));
 
   // ---- bonds (+ per-particle CSR of bond ends, in bond order) ----
   {
@@ -1398,7 +1398,7 @@ int lsdem_initialize(lsdem_ctx* c) {
   HIPCHK(c->dBarrier.alloc(kBarrierInts));
   HIPCHK(hipMemsetAsync(c->dBarrier.p, 0, kBarrierInts * sizeof(int), s));
   HIPCHK(c->dDevStat.alloc(kDevStats));
-  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, 5 * kStatSlots * sizeof(unsigned long long), s));
+  HIPCHK(hipMemsetAsync(c->dStatSlots.p, 0, kStatWords * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dDevInt.p, 0, kDevInts * sizeof(int), s));
   HIPCHK(hipMemsetAsync(c->dDevStat.p, 0, kDevStats * sizeof(unsigned long long), s));
   HIPCHK(hipMemsetAsync(c->dPairStart.p, 0, (nP + 1) * sizeof(int), s));

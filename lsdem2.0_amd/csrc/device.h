@@ -248,7 +248,8 @@ struct Params {
   //          (cumulative); [12] master nodes of the candidate pairs, [13] evaluated pairs,
   //          [14] rebuild steps (summed while profiling)
   unsigned long long* devStat;
-  unsigned long long* statSlots;  // kStatSlots x 5 spread accumulators of the narrow phase
+  unsigned long long* statSlots;  // 5 families x kStatSlots spread accumulators of the narrow phase,
+                                  // kStatStride words apart
 
   // ---- learned culling surrogate ----
   SurNet sur;              // network (valid when surSlot != nullptr)
@@ -304,6 +305,10 @@ struct SnapDev {
 };
 
 constexpr int kStatSlots = 32;
+// u64 words per stat slot: each slot has a 128-byte line of its own, since atomics on one line are
+// served one after another (~10 ns each), whatever words of the line they touch
+constexpr int kStatStride = 16;
+constexpr int kStatWords = 5 * kStatSlots * kStatStride;
 constexpr int kDevInts = 32;   // devInt entries
 constexpr int kBarrierLine = 32;                  // ints per barrier word (one 128-byte line each)
 constexpr int kBarrierInts = 18 * kBarrierLine;   // 8 groups, the groups' counter, the release word, evalCount

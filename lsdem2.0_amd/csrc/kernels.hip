@@ -1129,6 +1129,11 @@ __device__ __forceinline__ d3 histLookup(const Params& P, unsigned long long key
 // ------------------------------------------------------------------------------------------
 constexpr int kNarrowBlock = 256;
 
+// Stat slot `slot` of accumulator family f (one 128-byte line per slot).
+__device__ __forceinline__ unsigned long long* statSlot(const Params& P, int f, int slot) {
+  return P.statSlots + (f * kStatSlots + slot) * kStatStride;
+}
+
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1521,7 +1526,10 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
     __syncthreads();
   }
   // (only threads that wrote map records count: a few thousand atomics per step at 200k particles)
-  if (evNodes) atomicAdd(&P.statSlots[4 * kStatSlots + (blockIdx.x & (kStatSlots - 1))], (unsigned long long)evNodes);
+  if (__ballot(evNodes != 0)) {  // (one atomic per wavefront)
+    const unsigned long long sum = (unsigned long long)wsum(double(evNodes));
+    if (lane == 0) atomicAdd(statSlot(P, 4, blockIdx.x & (kStatSlots - 1)), sum);
+  }
 }
 
 // The FP32 prefilters may contract a * b + c into one rounding: their margins bound the rounding
@@ -2165,10 +2173,10 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const unsigned long long tc = (unsigned long long)wsum(double(statContacts));
   if (lane == 0 && (t | te)) {
     const int slot = (blockIdx.x + wl) & (kStatSlots - 1);
-    atomicAdd(&P.statSlots[slot], t);
-    atomicAdd(&P.statSlots[kStatSlots + slot], tq);
-    atomicAdd(&P.statSlots[2 * kStatSlots + slot], tc);
-    atomicAdd(&P.statSlots[3 * kStatSlots + slot], te);
+    atomicAdd(statSlot(P, 0, slot), t);
+    atomicAdd(statSlot(P, 1, slot), tq);
+    atomicAdd(statSlot(P, 2, slot), tc);
+    atomicAdd(statSlot(P, 3, slot), te);
   }
 }
 
@@ -2469,7 +2477,7 @@ __global__ void k_snap_restore(Params P, SnapDev S) {
     P.devInt[3] = 0;
     P.devInt[kErrStepIdx] = -1;
   }
-  for (int k = g.tid; k < 5 * kStatSlots; k += g.nth) P.statSlots[k] = 0;
+  for (int k = g.tid; k < 5 * kStatSlots; k += g.nth) P.statSlots[k * kStatStride] = 0;
 }
 
 // The history table of the restored slots: each contact slot's key inserted again (the table was
@@ -2518,11 +2526,11 @@ __device__ void stepEnd(const Params& P) {
     P.devInt[10 + P.curBuf] = min(P.devInt[1], (int)P.contactCap);  // slots to clear two steps on
     unsigned long long t = 0, q = 0, c = 0, e = 0, mn = 0;
     for (int k = 0; k < kStatSlots; ++k) {
-      t += P.statSlots[k];
-      q += P.statSlots[kStatSlots + k];
-      c += P.statSlots[2 * kStatSlots + k];
-      e += P.statSlots[3 * kStatSlots + k];
-      mn += P.statSlots[4 * kStatSlots + k];
+      t += *statSlot(P, 0, k);
+      q += *statSlot(P, 1, k);
+      c += *statSlot(P, 2, k);
+      e += *statSlot(P, 3, k);
+      mn += *statSlot(P, 4, k);
     }
     P.devStat[12] += mn;
     P.devStat[14] += P.devInt[rebuildFlagIdx(P.curBuf)] != 0 ? 1ull : 0ull;
@@ -2538,7 +2546,7 @@ __device__ void stepEnd(const Params& P) {
   }
   P.devInt[rebuildFlagIdx(P.curBuf)] = 0;  // rebuild flag (the drift kernel raises it)
   P.devInt[ghostCountIdx(P.curBuf)] = 0;  // valid ghost images at their source's own position
-  for (int k = 0; k < 5 * kStatSlots; ++k) P.statSlots[k] = 0;
+  for (int k = 0; k < 5 * kStatSlots; ++k) P.statSlots[k * kStatStride] = 0;
   // (the contact slot count [1] and the pair check's evalCount stay readable after the step; the
   // drift kernel clears them)
 }
