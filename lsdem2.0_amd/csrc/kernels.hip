@@ -281,6 +281,7 @@ __device__ __forceinline__ unsigned long long laneLtMask() {
 // into per-lane scratch memory.
 template <class T>
 __device__ __forceinline__ T pick3(int s, T a, T b, T c) { return s == 0 ? a : (s == 1 ? b : c); }
+
 // (an image position array indexed directly compiles to one load of the pointer when no other
 // member is indexed at run time: the per-step kernels without sector periodicity keep that form)
 template <bool SEC>
@@ -1202,6 +1203,12 @@ __device__ __forceinline__ PairView loadPair(const Params& P, int p, uint32_t in
 
 constexpr int kContactChunk = 1024;  // candidate slots reserved per atomic
 
+// The step record's key of a pair's partner of type t (the body's own key, or its image's): a
+// select, since a runtime index into the record's fields would go through scratch memory.
+__device__ __forceinline__ int partnerKey(int4 key, uint32_t t) {
+  return t == PT_PP ? key.x : (t == PT_GX ? key.y : (t == PT_GY ? key.z : key.w));
+}
+
 struct NodeHit {
   d3 point, normal;
   double ov;
@@ -1323,7 +1330,7 @@ __device__ __forceinline__ void makeF32(const Params& P, const PairView& v, Pair
   else {
     const BodyStep sb = P.step[v.partner];
     mB = sb.motion;
-    if (v.type >= PT_GX) flip = (((&sb.key.y)[v.type - PT_GX] >> kFlipBit) & 1) != 0;
+    if (v.type >= PT_GX) flip = ((partnerKey(sb.key, v.type) >> kFlipBit) & 1) != 0;
   }
   const double ex = double(mA.x) - double(mB.x), ey = double(mA.y) - double(mB.y), ez = double(mA.z) - double(mB.z);
   const double del = sqrt(ex * ex + ey * ey + ez * ez);
@@ -1409,10 +1416,15 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
     __syncthreads();
     // the tile's pairs of this thread, with the loads of one kind issued together
     constexpr int R = kCheckTile / 256;
-    int pv[R], Av[R], jv[R], key[R];
-    uint32_t tv[R];
+    // (vector types: an element picked by a runtime index stays in registers, where an array's
+    // would be read from scratch)
+    typedef int IntR __attribute__((ext_vector_type(R)));
+    typedef uint32_t UintR __attribute__((ext_vector_type(R)));
+    IntR pv, Av, jv;
+    UintR tv;
+    int key[R];
     float4 cv[R], mB[R], mA[R];
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
       pv[r] = t0 + r * 256 + int(threadIdx.x);
       const bool in = pv[r] < nPairs;
@@ -1422,7 +1434,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       Av[r] = in ? P.pairA[pv[r]] : 0;
       cv[r] = (in && !fresh) ? P.cert[pv[r]] : make_float4(-1.f, 0.f, 0.f, -1.f);
     }
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
       // the partner's step record (liveness key with the image's flip, motion): one 32-byte read
       if (pv[r] >= nPairs) {
@@ -1433,13 +1445,30 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         mB[r] = P.wMotion[jv[r]];
       } else {
         const BodyStep sb = P.step[jv[r]];
-        key[r] = tv[r] == PT_PP ? sb.key.x : (&sb.key.y)[tv[r] - PT_GX];
+        key[r] = partnerKey(sb.key, tv[r]);
         mB[r] = sb.motion;
       }
       mA[r] = (cv[r].x > 0.f || cv[r].w > 0.f) ? P.step[Av[r]].motion : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    // A pair whose bound against the motion since the build is spent (c.x <= 0) is checked against
+    // the motion since its evaluation whatever the partner's motion: the inputs of that bound are
+    // read now, with the partner's record, for the first such pair of the thread (about 1 % of the
+    // pairs are; a thread's others read them one round trip later).
+    int early = -1;
+#pragma clang loop unroll(full)
+    for (int r = R - 1; r >= 0; --r)
+      if (pv[r] < nPairs && cv[r].x <= 0.f && cv[r].w > 0.f) early = r;
+    float4 e0e, qa0e, qb0e, qae, qbe;
+    if (early >= 0) {
+      const int p = pv[early], A = Av[early], j = jv[early];
+      e0e = P.certE[p];
+      qa0e = P.certQA[p];
+      qb0e = P.certQB[p];
+      qae = P.qrel[A];
+      qbe = tv[early] == PT_PW ? P.wQrel[j] : P.qrel[j];
+    }
     bool actv[R];
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
       const int p = pv[r];
       const bool live = key[r] >= 0;
@@ -1453,19 +1482,28 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         c.z = fabsf(c.z);
         const bool flip = tv[r] >= PT_GX && ((key[r] >> kFlipBit) & 1);
         const float ex = mA[r].x - mB[r].x, ey = mA[r].y - mB[r].y, ez = mA[r].z - mB[r].z;
-        // against the motion since the build (the slack net of the motion at the evaluation)
         if (!flip && c.x > 0.f) {
+          // against the motion since the build (the slack net of the motion at the evaluation); a
+          // pair that fails it is evaluated, which issues it a new certificate (checking it against
+          // the motion since the evaluation instead would cost one more round trip)
           const float bound = (sqrtf(ex * ex + ey * ey + ez * ez) + mB[r].w * c.y + mA[r].w * c.z) * (1.f + 1e-5f);
           act = !(bound < c.x);
-        }
-        // against the motion since the evaluation: the same bound with the displacement and the
-        // rotations taken from that step
-        if (act && c.w > 0.f) {
-          const float4 e0 = P.certE[p];
+        } else if (c.x <= 0.f && c.w > 0.f) {
+          // against the motion since the evaluation: the same bound with the displacement and the
+          // rotations taken from that step
+          float4 e0, qa0, qb0, qa, qb;
+          if (r == early) {
+            e0 = e0e; qa0 = qa0e; qb0 = qb0e; qa = qae; qb = qbe;
+          } else {
+            e0 = P.certE[p];
+            qa0 = P.certQA[p];
+            qb0 = P.certQB[p];
+            qa = P.qrel[Av[r]];
+            qb = tv[r] == PT_PW ? P.wQrel[jv[r]] : P.qrel[jv[r]];
+          }
           if ((e0.w != 0.f) == flip) {
-            const float4 qB = tv[r] == PT_PW ? P.wQrel[jv[r]] : P.qrel[jv[r]];
             const float fx = ex - e0.x, fy = ey - e0.y, fz = ez - e0.z;
-            const float chA = relChordF(P.certQA[p], P.qrel[Av[r]]), chB = relChordF(P.certQB[p], qB);
+            const float chA = relChordF(qa0, qa), chB = relChordF(qb0, qb);
             const float bound = (sqrtf(fx * fx + fy * fy + fz * fz) + chB * c.y + chA * c.z) * (1.f + 1e-5f);
             act = !(bound < c.w);
           }
@@ -1480,7 +1518,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
       }
       actv[r] = act && p < nPairs;
     }
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int r = 0; r < R; ++r) {
       const unsigned long long m = __ballot(actv[r]);
       int first = 0;
@@ -1512,7 +1550,7 @@ __global__ __launch_bounds__(256) void k_pair_check(Params P, int full) {
         const BodyStep sb = P.step[v.partner];
         mB = sb.motion;
         qB = P.qrel[v.partner];
-        if (v.type >= PT_GX) flip = float(((&sb.key.y)[v.type - PT_GX] >> kFlipBit) & 1);
+        if (v.type >= PT_GX) flip = float((partnerKey(sb.key, v.type) >> kFlipBit) & 1);
       }
       PairF32 f;
       makeF32(P, v, f);
