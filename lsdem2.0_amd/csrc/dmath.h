@@ -152,8 +152,14 @@ LSD_HD d3 slidingSpring(d3 s0, d3 sv, d3 n, d3 Fn, double mu, double k, double c
   }
   return spring;
 }
-LSD_HD void linearContact(d3& F, d3& spring, d3 vc, d3 n, double delta, double dt, double kn, double ks,
-                          double mu, double res, double effM) {
+// Force and updated tangential spring of a contact (returned by value: reference outputs kept the
+// pair in scratch memory on the GPU).
+struct ContactLaw {
+  d3 F, spring;
+};
+LSD_HD ContactLaw linearContact(d3 spring, d3 vc, d3 n, double delta, double dt, double kn, double ks, double mu,
+                                double res, double effM) {
+  d3 F = mk3(0., 0., 0.);
   if (delta > 0.) {
     const double logRes = log(res);
     const double diss = -logRes / double(sqrtf(float(logRes * logRes + kPi * kPi)));
@@ -168,6 +174,7 @@ LSD_HD void linearContact(d3& F, d3& spring, d3 vc, d3 n, double delta, double d
   } else {
     spring = mk3(0., 0., 0.);
   }
+  return ContactLaw{F, spring};
 }
 
 }  // namespace lsd

@@ -1772,10 +1772,11 @@ __device__ __forceinline__ bool nodeSdfF32(const PairF32& f, const float (&c)[8]
 // Exact test and contact law of one candidate slot: the reference's node-vs-SDF membership in
 // FP64, then (for contacts) tangential history and the linear spring-dashpot.
 // (p, node, pInfo, pA: the slot's pair and node, and that pair's pairInfo and pairA)
-// fl: the pair's map record when the caller holds it (its copy of the partner's grid size and SDF
-// offset spares the read of the partner's shape record), else null
+// grid: the partner's grid size and SDF offset when the caller holds the pair's map record (which
+// spares the read of the partner's shape record), else x < 0.  (Values, not a pointer to the record:
+// a pointer that is either to LDS or to global memory makes flat loads.)
 template <bool SEC>
-__device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInfo, int pA, const PairF32* fl,
+__device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInfo, int pA, int4 grid,
                           unsigned long long& statQueries, unsigned long long& statContacts) {
   if (p < 0) {
     P.cInfo[c] = kInvalidInfo;
@@ -1822,9 +1823,9 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
   unsigned long long tests = 0;
   i3 nB;
   long long sdfOff;
-  if (fl) {
-    nB = i3{fl->nx, fl->ny, fl->nz};
-    sdfOff = fl->sdfOff;
+  if (grid.x >= 0) {
+    nB = i3{grid.x, grid.y, grid.z};
+    sdfOff = grid.w;
   } else {
     const ShapeDev SB = P.shapes[v.shapeB];
     nB = i3{SB.nx, SB.ny, SB.nz};
@@ -1871,7 +1872,9 @@ __device__ void exactSlot(const Params& P, int c, int p, int node, uint32_t pInf
     P.cTB[c] = F;
     return;
   }
-  linearContact(F, spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
+  const ContactLaw law = linearContact(spring, vc, h.normal, h.ov, P.dt, kn, ks, mu, res, effM);
+  F = law.F;
+  spring = law.spring;
   P.cSpring[c] = spring;
   P.cEn[c] = 0.5 * h.ov * h.ov * kn;
   P.cEs[c] = 0.5 * len2(spring) * ks;
@@ -1934,7 +1937,7 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
   const int entCap = kBatch * maxC, subCap = kMaxSub * entCap;
   unsigned char* wb = candLds + wl * candWaveBytes(maxC);
   PairF32* sF = reinterpret_cast<PairF32*>(wb);
-  int2* sEnt = reinterpret_cast<int2*>(wb + sizeof(PairF32) * kBatch);  // (cluster index, pair j)
+  int2* sEnt = reinterpret_cast<int2*>(wb + sizeof(PairF32) * kBatch);  // (first sub-cluster, count | pair j << 4)
   int2* sSub = sEnt + entCap;                                            // (first node, count | j << 16)
   unsigned int* sMask = reinterpret_cast<unsigned int*>(sSub + subCap);  // 16-bit node masks
   int* sCnt = reinterpret_cast<int*>(sMask + subCap);
@@ -1992,8 +1995,11 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
     for (int it = 0; it < iters; ++it) {
       const int c = it * 8 + k;
       bool keep = false;
+      int subFirst = 0, nSub = 0;
       if (own && c < nCl) {
         const ClusterF32 cl = P.clusters32[clusterOff + c];
+        subFirst = cl.subFirst;
+        nSub = cl.nSub;
         float gx, gy, gz, rho, slack;
         bool inside;
         keep = clusterGeomF32(f, cl, gx, gy, gz, rho, inside, slack);
@@ -2005,7 +2011,8 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         if (!keep) lmin = fminf(lmin, fmaxf(slack, 0.f));
       }
       const unsigned long long m = __ballot(keep);
-      if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(clusterOff + c, j);
+      // (the entry carries the cluster's sub-cluster range: level 2 reads no cluster record)
+      if (keep) sEnt[E + __popcll(m & ltMask)] = make_int2(subFirst, nSub | (j << 4));
       E += __popcll(m);
     }
     lmin = groupMin<8>(lmin);
@@ -2021,10 +2028,9 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       int first = 0, count = 0, jj = 0;
       if (e < E) {
         const int2 en = sEnt[e];
-        const ClusterF32 cl = P.clusters32[en.x];
-        jj = en.y;
-        if (u < cl.nSub) {
-          const ClusterF32 sc = P.subs32[cl.subFirst + u];
+        jj = en.y >> 4;
+        if (u < (en.y & 15)) {
+          const ClusterF32 sc = P.subs32[en.x + u];
           const PairF32& fj = sF[jj];
           float gx, gy, gz, rho, slack;
           bool inside;
@@ -2108,39 +2114,18 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         total += __popcll(m);
       }
     }
-    __builtin_amdgcn_wave_barrier();
-    // ---- one reservation per batch, exact size ----
-    int first = 0;
-    if (total > 0) {
-      if (lane == 0) first = atomicAdd(&P.devInt[1], total);
-      first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
-      if ((long long)first + total > P.contactCap) {
-        // the call halts with a capacity error; slots reserved below the capacity stay empty
-        if (lane == 0) {
-          atomicMax(&P.devInt[5], first + total);
-          raiseError(P, kErrContacts);
-        }
-        for (int c = first + lane; c < (int)P.contactCap; c += 64) {
-          if (c >= first + total) break;
-          P.cInfo[c] = kInvalidInfo;
-          P.hashSlot[c] = -1;
-        }
-        total = 0;
-      }
+    // the batch's pair entries (loaded at level 1) for the contact flags and the exact test
+    if (k == 0 && j < kBatch) {
+      sPInfo[j] = myInfo;
+      sPA[j] = myA;
     }
+    __builtin_amdgcn_wave_barrier();
+    // ---- one reservation per batch, exact size (its result is first needed after the
+    // certificates) ----
+    int first = 0;
+    if (total > 0 && lane == 0) first = atomicAdd(&P.devInt[1], total);
     if (lane < nb && bi + lane * NB < nAct) {
       const PairF32& fl = sF[lane];
-      int off = first;
-      for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
-      P.pairCoff[fl.pair] = off;
-      P.pairCcnt[fl.pair] = total ? sCnt[lane] : 0;
-      if (total && sCnt[lane] > 0) {
-        // both bodies have contact terms to sum this step
-        P.bodyHit[P.pairA[fl.pair]] = 1;
-        const uint32_t info = P.pairInfo[fl.pair];
-        if ((info >> 28) != PT_PW) P.bodyHit[info & kPartnerMask] = 1;
-      }
-      sOff[lane] = off;  // running write offset of each pair
       // separation certificate in world units, net of the motion since the build
       float sg = __uint_as_float(sSlack[lane]);
       bool surDep = false;  // the surrogate's slack is the smaller one
@@ -2160,6 +2145,36 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
       // (the sign of .z marks a certificate that rests on the surrogate's slack)
       P.cert[fl.pair] = make_float4(sw > 0.f ? sw : -1.f, fl.certCB, surDep ? -fl.certCA : fl.certCA,
                                     s0 > 0.f ? s0 : -1.f);
+    }
+    if (total > 0) {
+      first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
+      if ((long long)first + total > P.contactCap) {
+        // the call halts with a capacity error; slots reserved below the capacity stay empty
+        if (lane == 0) {
+          atomicMax(&P.devInt[5], first + total);
+          raiseError(P, kErrContacts);
+        }
+        for (int c = first + lane; c < (int)P.contactCap; c += 64) {
+          if (c >= first + total) break;
+          P.cInfo[c] = kInvalidInfo;
+          P.hashSlot[c] = -1;
+        }
+        total = 0;
+      }
+    }
+    if (lane < nb && bi + lane * NB < nAct) {
+      const int pr = sF[lane].pair;
+      int off = first;
+      for (int jj = 0; jj < lane; ++jj) off += sCnt[jj];
+      P.pairCoff[pr] = off;
+      P.pairCcnt[pr] = total ? sCnt[lane] : 0;
+      if (total && sCnt[lane] > 0) {
+        // both bodies have contact terms to sum this step
+        P.bodyHit[sPA[lane]] = 1;
+        const uint32_t info = sPInfo[lane];
+        if ((info >> 28) != PT_PW) P.bodyHit[info & kPartnerMask] = 1;
+      }
+      sOff[lane] = off;  // running write offset of each pair
     }
     __builtin_amdgcn_wave_barrier();
     // ---- write candidate slots; a pair's candidates are contiguous ----
@@ -2187,19 +2202,18 @@ __global__ __launch_bounds__(kNarrowBlock) void k_candidates(Params P) {
         __builtin_amdgcn_wave_barrier();
       }
       // the exact test and contact law of this batch's candidates, by the whole wavefront
-      if (k == 0 && j < kBatch) {
-        sPInfo[j] = myInfo;
-        sPA[j] = myA;
-      }
       __threadfence_block();
       __builtin_amdgcn_wave_barrier();
       for (int c = first + lane; c < first + total; c += 64) {
         if (c - first < kSlotStage) {
           const int2 st = sStage[c - first];
-          exactSlot<SEC>(P, c, sF[st.x].pair, st.y, sPInfo[st.x], sPA[st.x], &sF[st.x], statQueries, statContacts);
+          const PairF32& fs = sF[st.x];
+          exactSlot<SEC>(P, c, fs.pair, st.y, sPInfo[st.x], sPA[st.x], make_int4(fs.nx, fs.ny, fs.nz, fs.sdfOff),
+                         statQueries, statContacts);
         } else {
           const int pc = P.cPair[c];
-          exactSlot<SEC>(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], nullptr, statQueries, statContacts);
+          exactSlot<SEC>(P, c, pc, P.cNode[c], P.pairInfo[pc], P.pairA[pc], make_int4(-1, 0, 0, 0), statQueries,
+                         statContacts);
         }
       }
     }
