@@ -1509,7 +1509,7 @@ int runCall(lsdem_ctx* c, Work&& work) {
       // leave the context able to step again (from whatever state the halted call left)
       HIPCHK(hipMemset(c->dDevInt.p + 3, 0, sizeof(int)));
       HIPCHK(hipMemset(c->dDevInt.p + kErrStepIdx, 0xFF, sizeof(int)));
-    HIPCHK(hipMemset(c->dBarrier.p, 0, kBarrierInts * sizeof(int)));
+      HIPCHK(hipMemset(c->dBarrier.p, 0, kBarrierInts * sizeof(int)));
       c->forceRebuild = true;
       return err;
     }
