@@ -57,7 +57,7 @@ B_BODY = (78 + 6) * 4
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch at the 200k workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a
 # plain copy (profiles/c2_200k_pmc_traffic.md); None for a kernel not measured at its current code.
-KERNEL_TRAFFIC_BYTES = {"k_pair_check": 73.85e6, "k_candidates": 9.73e6, "k_reduce_drift": 73.42e6}
+KERNEL_TRAFFIC_BYTES = {"k_pair_check": 58.61e6, "k_candidates": 12.79e6, "k_reduce_drift": 73.34e6}
 METRIC = "particle-steps/s at 200k LS-DEM particles: plain vs surrogate-culled, same GPU"
 PROFILE_EVERY = 10
 
@@ -390,8 +390,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5000)
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--seed", type=int, default=20261204)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps timed on all host cores")
-    ap.add_argument("--cpu-serial-steps", type=int, default=1, help="oracle steps timed on one core")
+    ap.add_argument("--cpu-steps", type=int, default=30, help="oracle steps timed on all host cores")
+    ap.add_argument("--cpu-serial-steps", type=int, default=4, help="oracle steps timed on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sur-iters", type=int, default=3000)
