@@ -22,6 +22,7 @@ __global__ void k_scan_apply(const int* in, long long n, const int* blockSum, in
 template <bool SEC> __global__ void k_kick_drift(Params P, int forceRebuild);
 __global__ void k_search_cells(Params P);
 template <bool SEC> __global__ void k_reduce_drift(Params P, StepHead H);
+__global__ void k_reduce_drift_small(Params P, StepHead H);
 __global__ void k_broad_phase(Params P);
 __global__ void k_bucket_count(const int* bucket, int n, int* count, int* rank, const int* gate);
 __global__ void k_bucket_scatter(const int* bucket, int n, const int* start, const int* rank, int* out,
@@ -560,6 +561,8 @@ void launchReduceDrift(lsdem_ctx* c, const StepHead& H) {
   const int bs = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
   if (P.periodicSector)
     hipLaunchKernelGGL(k_reduce_drift<true>, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, H);
+  else if (P.nP <= kWaveEndBodies)
+    hipLaunchKernelGGL(k_reduce_drift_small, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, H);
   else
     hipLaunchKernelGGL(k_reduce_drift<false>, dim3(blocksFor(P.nP + P.nW, bs)), dim3(bs), 0, c->stream, P, H);
 }

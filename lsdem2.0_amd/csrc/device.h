@@ -305,6 +305,8 @@ struct SnapDev {
 };
 
 constexpr int kStatSlots = 32;
+// bodies up to which the drift after a step closes the step's counters with a wavefront (k_reduce_drift_small)
+constexpr int kWaveEndBodies = 65536;
 // u64 words per stat slot: each slot has a 128-byte line of its own, since atomics on one line are
 // served one after another (~10 ns each), whatever words of the line they touch
 constexpr int kStatStride = 16;

@@ -2387,6 +2387,7 @@ __global__ void k_bonds(Params P) {
 // Per-body segmented reduction + second half-kick
 // ------------------------------------------------------------------------------------------
 __device__ void stepEnd(const Params& P);
+__device__ void stepEndWave(const Params& P, int l);
 
 // Contact and bond sums on body b in a fixed order: its master pairs contact by contact, then the
 // reactions of the pairs where it is the partner, then its bonds.
@@ -2467,6 +2468,35 @@ __global__ void k_reduce_drift(Params P, StepHead H) {
   const bool any = reduceBody(P, b, F, T);
   kickDrift<SEC>(P, b, in, F, T, 2, H.buf);
   // the stored sums of a body without terms are already zero unless written otherwise since
+  if (any || set) {
+    P.force[b] = F;
+    P.torque[b] = T;
+    P.forceSet[b] = any ? 1 : 0;
+  }
+}
+// The same for a small simulation (up to kWaveEndBodies bodies, no sector periodicity), whose
+// counters the first wavefront closes (stepEndWave): its body pass is short, and thread 0's rounds of
+// counter reads were its last wavefront's (256 drums of 19k bodies in all: 0.0472 -> 0.0433 ms per
+// step).  In the 200k-body pass thread 0's version is the faster, and the kernel above is left as it
+// compiles alone (any change to it shifts its code generation).
+__global__ void k_reduce_drift_small(Params P, StepHead H) {
+  LSD_HALT(P);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  BodyIn in;
+  bool set = false;
+  if (b < P.nP) {
+    in = loadBody<false>(P, b);
+    set = P.forceSet[b] != 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) stepEndWave(P, int(threadIdx.x));
+  stepHead(P, H, b, int(gridDim.x * blockDim.x));
+  if (b >= P.nP) {
+    if (b - P.nP < P.nW) wallDrift(P, b - P.nP, H.buf);
+    return;
+  }
+  d3 F, T;
+  const bool any = reduceBody(P, b, F, T);
+  kickDrift<false>(P, b, in, F, T, 2, H.buf);
   if (any || set) {
     P.force[b] = F;
     P.torque[b] = T;
@@ -2568,6 +2598,49 @@ __global__ void k_kick(Params P) {
 // ------------------------------------------------------------------------------------------
 // Step bookkeeping
 // ------------------------------------------------------------------------------------------
+// stepEnd by the first wavefront of block 0 (lanes l < min(64, blockDim.x)): the lanes read the stat
+// slots (a line each) together and sum them across lanes (in FP64, exact for counts below 2^53), where
+// thread 0 alone takes 8 dependent rounds of scalar loads.  Totals, flags and counters as stepEnd.
+__device__ void stepEndWave(const Params& P, int l) {
+  const int nl = min(64, int(blockDim.x));
+  double s[5] = {0., 0., 0., 0., 0.};
+  for (int k = l; k < kStatSlots; k += nl) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) s[f] += double(*statSlot(P, f, k));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      const double o = __shfl(s[f], l + off, 64);
+      if (l + off < nl) s[f] += o;
+    }
+  }
+  if (l == 0) {
+    const int np = P.pairStart[P.nP];
+    P.devInt[4] = max(P.devInt[4], np);
+    P.devInt[5] = max(P.devInt[5], P.devInt[1]);
+    P.devInt[10 + P.curBuf] = min(P.devInt[1], (int)P.contactCap);
+    const unsigned long long t = (unsigned long long)s[0], q = (unsigned long long)s[1],
+                             c = (unsigned long long)s[2], e = (unsigned long long)s[3],
+                             mn = (unsigned long long)s[4];
+    P.devStat[12] += mn;
+    P.devStat[14] += P.devInt[rebuildFlagIdx(P.curBuf)] != 0 ? 1ull : 0ull;
+    P.devStat[0] = t;
+    P.devStat[1] = q;
+    P.devStat[2] = c;
+    P.devStat[3] += (unsigned long long)np;
+    P.devStat[4] += t;
+    P.devStat[5] += q;
+    P.devStat[6] += c;
+    P.devStat[8] += e;
+    P.devInt[rebuildFlagIdx(P.curBuf)] = 0;
+    P.devInt[ghostCountIdx(P.curBuf)] = 0;
+  }
+  // (after every lane's loads: their sums were exchanged above)
+  for (int k = l; k < 5 * kStatSlots; k += nl) P.statSlots[k * kStatStride] = 0;
+}
+
 // Step totals into the counters, then the per-step counters that the drift kernel accumulates into
 // cleared for the next step.
 __device__ void stepEnd(const Params& P) {
