@@ -49,6 +49,28 @@ def test_c2_configuration(L, S):
     print("worst", worst)
 
 
+def test_c2_large_body_pass(L, S):
+    """More bodies (70,000) than the small-simulation body pass takes (65,536): the large body pass,
+    whose step close (counter sums, flags) runs in a block of its own, against the oracle."""
+    sc = S.scene_c2(n=70_000, cell_xy=0.2, gap=-0.15, seed=37)
+    assert sc.n_particles > 65_536
+    moved = S.remove_initial_overlaps(lambda c: L.Sim(c), sc)
+    rng = np.random.default_rng(5)
+    sc.particles["vel"] = rng.normal(0.0, 0.3, (sc.n_particles, 3))
+    # the lowest layer 10 um into the floor: floor contacts
+    P = sc.particles
+    bottom = np.nonzero(P["pos"][:, 2] < P["pos"][:, 2].min() + 0.01)[0]
+    low = min(float(P["pos"][i, 2] + (S.quat_rotate(P["quat"][i][None, :], sc.shapes[P["shape"][i]]["nodes"])[:, 2]
+                                      * P["scale"][i]).min()) for i in bottom)
+    P["pos"][:, 2] -= low + 1e-5
+    print(f"{sc.n_particles} particles, {moved} relocated, domain {sc.cfg['domain_max']}", flush=True)
+    gpu, ref = _pair(sc, L, threads=16)
+    counts, worst = step_and_compare(gpu, ref, 8, lists=(0, 1, 2, 3, 4), log=print)
+    assert sum(c["pp"] for c in counts) > 500
+    assert sum(c["pw"] for c in counts) > 0
+    print("worst", worst)
+
+
 def test_c3_configuration(L, S):
     # a lattice tighter than the largest particles (pitch 5.6 mm for radii up to 3 mm): the packing
     # starts compressed, its contacts push the bonded particles apart and bonds break over many steps
